@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Benchmark of the GIP fractional-step Navier–Stokes hot path on one MI355X.
+
+Workload (the mixed-precision configuration, 256^3): the uniform 256^3
+lid-driven-cavity pressure Laplacian (7-point, 117,440,512 nnz, pinned),
+right-hand side a seeded N(0,1) field projected to zero mean.  A "step" is
+one Jacobi-PCG solve of that system to a 1e-10 FP64 relative true residual
+||b - A x||_2/||b||_2 with the mixed-precision solver (FP32 operator,
+vectors and SpMV inside the FP64 defect-correction loop); `value` is its
+time in ms.  Reported beside it: the same solve with the FP64 solver, the
+FP64 SpMV HBM rate on the Laplacian (the dominant kernel, `roofline`), the
+time-step rate of full Navier–Stokes steps at 256^3 (Re=1000, PCG to 1e-10,
+FP64 vs mixed), and the reference CPU path's PCG on the same system
+(`cpu_baseline`).  Inputs are resident in HBM before the timed region.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--ns-steps 20]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "PCG ms to 1e-10 resid + steps/s, FP64 vs mixed, 256³ cavity; SpMV HBM GB/s"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmv_bytes(nnz, rows, cols):
+    """Algorithmic bytes of one FP64 CSR SpMV: values+indices, row pointers,
+    the output and one read of every input entry."""
+    return 12 * nnz + 4 * (rows + 1) + 8 * rows + 8 * cols
+
+
+def time_spmv(hip, dev, A, x, y, reps):
+    e0, e1 = hip.Event(dev), hip.Event(dev)
+    for _ in range(3):
+        A.spmv(x, y)
+    e0.record()
+    for _ in range(reps):
+        A.spmv(x, y)
+    e1.record()
+    return e0.elapsed_ms(e1) / reps
+
+
+def true_relres(hip, dev, A, b_dev, x_dev, n, n_all, bnorm):
+    ax = dev.array(n_all)
+    hip.fill(ax, 0.0)
+    A.spmv(x_dev, ax)
+    hip.axpby(1.0, b_dev, -1.0, ax, n)
+    return np.sqrt(hip.dot(ax, ax, n)) / bnorm
+
+
+def pcg_leg(hip, dev, A, b_dev, n, n_all, precision, steps, warmup, maxit, bnorm):
+    solver = hip.PCG(A, n_all)
+    x = dev.array(n_all)
+    z = dev.array(n_all)
+    hip.fill(z, 0.0)
+    e0, e1 = hip.Event(dev), hip.Event(dev)
+    times, res = [], None
+    for k in range(warmup + steps):
+        e0.record()
+        res = solver.solve(b_dev, x, z, maxit=maxit, tol=1e-10, precision=precision, inner_tol=1e-3, poll=32)
+        e1.record()
+        ms = e0.elapsed_ms(e1)
+        if k >= warmup:
+            times.append(ms)
+        log(f"  pcg {'mixed' if precision else 'fp64'} solve {k}: {ms:.1f} ms, {res.iters} it, "
+            f"{res.corrections} corr, status {res.status}")
+    rel = true_relres(hip, dev, A, b_dev, x, n, n_all, bnorm)
+    solver.destroy()
+    return {"ms": float(np.mean(times)), "ms_min": float(np.min(times)), "iters": res.iters,
+            "corrections": res.corrections, "true_relres": float(rel), "status": res.status}
+
+
+def ns_leg(dns, cav, mode, steps, warmup):
+    sim = dns.Simulation(cavity=cav, gamma=1e-3, pcg_mode=mode, pcg_tol=1e-10, pcg_maxit=20000)
+    for _ in range(warmup):
+        sim.step()
+    sim.sync()
+    t0 = time.perf_counter()
+    its = []
+    for k in range(steps):
+        s = sim.step()
+        its.append(s["pcg_iters"])
+    sim.sync()
+    dt = time.perf_counter() - t0
+    sim.close()
+    return {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps, "pcg_iters_mean": float(np.mean(its))}
+
+
+def cpu_baseline(n, iters, seed, fp64_iters):
+    """The reference's own GIP_LinearSolver + Multicore spec (oracle/_ref, one
+    rank, one core) on the same Laplacian and RHS, for a bounded number of
+    fixed iterations; extrapolated to the FP64 iteration count."""
+    exe = ROOT / "oracle" / "_ref" / "gip_ref_harness"
+    if not exe.exists():
+        return None
+    out = subprocess.run([str(exe), "pcg", str(n), str(iters), str(seed)], capture_output=True, text=True,
+                         timeout=600, env=dict(os.environ, OMP_NUM_THREADS="1"))
+    if out.returncode != 0:
+        log("cpu baseline failed:", out.stderr[-500:])
+        return None
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    return {"value": r["ms_per_iter"] * fp64_iters, "unit": "ms", "cores": 1, "kind": "reference",
+            "ms_per_iter": r["ms_per_iter"],
+            "sample": f"reference GIP_LinearSolver::solve (Multicore spec, 1 rank, -O2) on the {n}^3 Laplacian, "
+                      f"{iters} fixed Jacobi-PCG iterations timed ({r['solve_ms']:.0f} ms), extrapolated to the "
+                      f"{fp64_iters} iterations the FP64 solve needs to 1e-10",
+            "host": os.uname().nodename, "cpu_count": os.cpu_count()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--ns-steps", type=int, default=20)
+    ap.add_argument("--ns-n", type=int, default=None)
+    ap.add_argument("--seed", type=int, default=20240601)
+    ap.add_argument("--cpu-iters", type=int, default=60)
+    ap.add_argument("--skip-ns", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs one partition on one device "
+              f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
+        sys.exit(2)
+
+    from gip_amd import cavity, dns, hip
+
+    n = args.n
+    t0 = time.perf_counter()
+    cav = cavity.generate(n, gamma=1e-3, ops=cavity.OP_BITS["L"])
+    L = cav.op("L")
+    N, NA = cav.n_inner, cav.n_all
+    b = np.zeros(NA)
+    b[:N] = cavity.rhs_normal(N, args.seed)
+    bnorm = float(np.linalg.norm(b[:N]))
+    log(f"generated {n}^3 Laplacian: {L.nnz} nnz in {time.perf_counter() - t0:.1f} s")
+
+    dev = hip.Device(0)
+    A = hip.Matrix.from_csr(dev, L, fp32=True)
+    info = A.info()
+    b_dev = dev.to_device(b)
+    del cav, L
+
+    # FP64 SpMV on the Laplacian (dominant kernel)
+    x = dev.to_device(np.random.default_rng(1).standard_normal(NA))
+    y = dev.array(NA)
+    spmv_ms = time_spmv(hip, dev, A, x, y, 50)
+    alg = spmv_bytes(info["nnz"], N, NA)
+    spmv_gbs = alg / (spmv_ms * 1e-3) / 1e9
+    log(f"spmv fp64: {spmv_ms:.4f} ms, {spmv_gbs:.0f} GB/s algorithmic ({alg / 1e9:.3f} GB)")
+    del x, y
+
+    fp64 = pcg_leg(hip, dev, A, b_dev, N, NA, hip.PCG_FP64, args.steps, args.warmup, 200000, bnorm)
+    mixed = pcg_leg(hip, dev, A, b_dev, N, NA, hip.PCG_MIXED, args.steps, args.warmup, 200000, bnorm)
+    log(f"fp64 pcg {fp64['ms']:.1f} ms ({fp64['iters']} it, true {fp64['true_relres']:.2e}); "
+        f"mixed {mixed['ms']:.1f} ms ({mixed['iters']} it, true {mixed['true_relres']:.2e}); "
+        f"speedup {fp64['ms'] / mixed['ms']:.2f}x")
+    A.destroy()
+    dev.sync()
+
+    ns = None
+    if not args.skip_ns and args.ns_steps > 0:
+        ns_n = args.ns_n or n
+        t0 = time.perf_counter()
+        full = cavity.generate(ns_n, gamma=1e-3)
+        log(f"generated full {ns_n}^3 cavity operators in {time.perf_counter() - t0:.1f} s")
+        ns = {"n": ns_n, "re": 1000, "steps": args.ns_steps,
+              "fp64": ns_leg(dns, full, "fused", args.ns_steps, 1),
+              "mixed": ns_leg(dns, full, "mixed", args.ns_steps, 1)}
+        log(f"ns steps/s fp64 {ns['fp64']['steps_per_s']:.3f}, mixed {ns['mixed']['steps_per_s']:.3f}")
+        del full
+
+    cpu = None if args.skip_cpu else cpu_baseline(n, args.cpu_iters, args.seed, fp64["iters"])
+
+    out = {
+        "metric": METRIC,
+        "value": round(mixed["ms"], 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["ms"], 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-inner/f64-outer",
+        "data": "synthetic: generated 256^3 uniform cavity Laplacian, seeded N(0,1) zero-mean RHS",
+        "config": {"workload": f"{n}^3 cavity Poisson, Jacobi-PCG to 1e-10 FP64 relative true residual "
+                               f"(mixed precision), one partition", "n": n, "nnz": info["nnz"],
+                   "layout": {1: "csr", 2: "sliced-ell", 3: "diag"}.get(info["format"], "?")},
+        "pcg_fp64_ms": round(fp64["ms"], 3),
+        "pcg_mixed_ms": round(mixed["ms"], 3),
+        "mixed_speedup": round(fp64["ms"] / mixed["ms"], 3),
+        "pcg_fp64": fp64,
+        "pcg_mixed": mixed,
+        "spmv_fp64": {"ms": spmv_ms, "gbs": spmv_gbs, "algorithmic_bytes": alg},
+        "ns_steps": ns,
+        "roofline": {"bound": "hbm", "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_ell<0,7> (FP64 sliced-ELL SpMV, 256^3 Laplacian)"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,85 @@
+// Device-side reduction helpers shared by the vector and PCG kernels.
+// Wave64 shuffle tree, then an LDS tree across the workgroup's wavefronts,
+// then the last workgroup to finish combines the per-workgroup partials.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "gip_internal.h"
+
+namespace gip {
+
+enum { RED_SUM = 0, RED_MAXABS = 1, RED_MIN = 2 };
+
+template <int OP>
+struct RedOp;
+
+template <>
+struct RedOp<RED_SUM> {
+    __device__ static double identity() { return 0.0; }
+    __device__ static double combine(double a, double b) { return a + b; }
+    __device__ static double term(const double *x, const double *y, int i) { return x[i] * y[i]; }
+};
+
+template <>
+struct RedOp<RED_MAXABS> {
+    __device__ static double identity() { return 0.0; }
+    // NaN terms never replace the running value, as in GIP_VectorMulticore.cpp:151-153
+    __device__ static double combine(double a, double b) { return b > a ? b : a; }
+    __device__ static double term(const double *x, const double *, int i) { return fabs(x[i]); }
+};
+
+template <>
+struct RedOp<RED_MIN> {
+    __device__ static double identity() { return 1e20; }
+    __device__ static double combine(double a, double b) { return b < a ? b : a; }
+    __device__ static double term(const double *x, const double *, int i) { return x[i]; }
+};
+
+// Reduce v over the workgroup; the result is valid in thread 0.
+template <int OP>
+__device__ inline double block_reduce(double v) {
+    __shared__ double sh[kBlock / 64];
+    __syncthreads();  // protect sh against a previous call
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = RedOp<OP>::combine(v, __shfl_down(v, off, 64));
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        v = sh[0];
+#pragma unroll
+        for (int k = 1; k < kBlock / 64; ++k) v = RedOp<OP>::combine(v, sh[k]);
+    }
+    return v;
+}
+
+// Publish this workgroup's value (thread 0's v) and, in the workgroup that
+// finishes last, combine all partials in index order into *out (thread 0).
+// Returns true in every thread of that last workgroup.  The counter is
+// reset to zero by the last workgroup, ready for the next launch.
+template <int OP>
+__device__ inline bool last_block_reduce(double v, double *partials, unsigned int *counter, double *out) {
+    __shared__ unsigned int s_ticket;
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = v;
+        __threadfence();
+        s_ticket = atomicAdd(counter, 1u);
+    }
+    __syncthreads();
+    if (s_ticket != gridDim.x - 1) return false;
+    __threadfence();
+    double acc = RedOp<OP>::identity();
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock)
+        acc = RedOp<OP>::combine(acc, __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    acc = block_reduce<OP>(acc);
+    if (threadIdx.x == 0) {
+        *out = acc;
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return true;
+}
+
+int launch_reduce(gip_ctx *c, int op, int n, const double *x, const double *y, double *dres);
+
+}  // namespace gip

@@ -1,0 +1,161 @@
+// Vector kernels of libgip_hip: fill, axpy, axpby, dot, infinity norm, CFL.
+// Semantics of GIP_VectorMulticore.cpp:44-195 and GIP_NonLinearMulticore.cpp:3-55;
+// reductions are deterministic: fixed grid, fixed per-thread element sets,
+// fixed wavefront/LDS combination tree, and the last workgroup to finish
+// combines the per-workgroup partials in index order.
+#include "gip_blas1.h"
+#include "gip_internal.h"
+
+namespace gip {
+
+__global__ __launch_bounds__(kBlock) void k_fill(int n, double *__restrict__ x, double v) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_axpy(int n, double a, const double *__restrict__ x, double *__restrict__ y) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) y[i] = y[i] + a * x[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_axpby(int n, double a, const double *__restrict__ x, double b,
+                                                  double *__restrict__ y) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) y[i] = a * x[i] + b * y[i];
+}
+
+template <int OP>
+__global__ __launch_bounds__(kBlock) void k_reduce(int n, const double *__restrict__ x, const double *__restrict__ y,
+                                                   double *__restrict__ partials, unsigned int *__restrict__ counter,
+                                                   double *__restrict__ result) {
+    double acc = RedOp<OP>::identity();
+    const int stride = gridDim.x * kBlock;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) acc = RedOp<OP>::combine(acc, RedOp<OP>::term(x, y, i));
+    acc = block_reduce<OP>(acc);
+    if (last_block_reduce<OP>(acc, partials, counter, &acc)) {
+        if (threadIdx.x == 0) *result = acc;
+    }
+}
+
+struct CflArgs {
+    const double *u, *v, *w, *dxs;
+    double gamma, rho;
+};
+
+__global__ __launch_bounds__(kBlock) void k_cfl(int n, CflArgs a, double *__restrict__ partials,
+                                                unsigned int *__restrict__ counter, double *__restrict__ result) {
+    // GIP_NonLinearMulticore.cpp:14-43: the running minimum starts at 1e20 and
+    // takes each admissible convective and diffusive limit.
+    double acc = 1e20;
+    const int stride = gridDim.x * kBlock;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        double ut = a.u[i], vt = a.v[i], wt = a.w[i], dx = a.dxs[i];
+        double normit = fabs(ut) > fabs(vt) ? fabs(ut) : fabs(vt);
+        if (normit < fabs(wt)) normit = fabs(wt);
+        if (!(dx / normit < 1e-17)) {
+            double c = 0.25 * dx / normit;
+            if (acc > c) acc = c;
+        }
+        if (!(dx * dx * a.rho / a.gamma < 1e-17)) {
+            double c = 0.2 * dx * dx * a.rho / a.gamma;
+            if (acc > c) acc = c;
+        }
+    }
+    acc = block_reduce<RED_MIN>(acc);
+    if (last_block_reduce<RED_MIN>(acc, partials, counter, &acc)) {
+        if (threadIdx.x == 0) *result = 0.8 * acc > 1e-1 ? 1e-1 : 0.8 * acc;  // :47-50
+    }
+}
+
+static inline int blocks_for(int n) { return (n + kBlock - 1) / kBlock; }
+
+int launch_reduce(gip_ctx *c, int op, int n, const double *x, const double *y, double *dres) {
+    int g = reduce_grid(n);
+    double *part = c->d_partials;
+    unsigned int *cnt = c->d_counter + op;
+    switch (op) {
+        case RED_SUM:
+            k_reduce<RED_SUM><<<g, kBlock, 0, c->stream>>>(n, x, y, part, cnt, dres);
+            break;
+        case RED_MAXABS:
+            k_reduce<RED_MAXABS><<<g, kBlock, 0, c->stream>>>(n, x, y, part, cnt, dres);
+            break;
+        default:
+            return set_error(GIP_EINVAL, "unknown reduction");
+    }
+    GIP_LAUNCH_CHECK("reduce kernel");
+    return GIP_OK;
+}
+
+static int fetch_result(gip_ctx *c, const double *dres, double *out) {
+    GIP_HIP_TRY(hipMemcpyAsync(c->h_result, dres, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GIP_HIP_TRY(hipStreamSynchronize(c->stream));
+    *out = c->h_result[0];
+    return GIP_OK;
+}
+
+}  // namespace gip
+
+using namespace gip;
+
+extern "C" {
+
+int gip_dfill(gip_ctx *c, int32_t n, double *x, double value) {
+    if (!c || n < 0 || (n && !x)) return set_error(GIP_EINVAL, "gip_dfill: bad argument");
+    if (!n) return GIP_OK;
+    k_fill<<<blocks_for(n), kBlock, 0, c->stream>>>(n, x, value);
+    GIP_LAUNCH_CHECK("gip_dfill");
+    return GIP_OK;
+}
+
+int gip_daxpy(gip_ctx *c, int32_t n, double alpha, const double *x, double *y) {
+    if (!c || n < 0 || (n && (!x || !y))) return set_error(GIP_EINVAL, "gip_daxpy: bad argument");
+    if (!n) return GIP_OK;
+    k_axpy<<<blocks_for(n), kBlock, 0, c->stream>>>(n, alpha, x, y);
+    GIP_LAUNCH_CHECK("gip_daxpy");
+    return GIP_OK;
+}
+
+int gip_daxpby(gip_ctx *c, int32_t n, double alpha, const double *x, double beta, double *y) {
+    if (!c || n < 0 || (n && (!x || !y))) return set_error(GIP_EINVAL, "gip_daxpby: bad argument");
+    if (!n) return GIP_OK;
+    k_axpby<<<blocks_for(n), kBlock, 0, c->stream>>>(n, alpha, x, beta, y);
+    GIP_LAUNCH_CHECK("gip_daxpby");
+    return GIP_OK;
+}
+
+int gip_ddot_dev(gip_ctx *c, int32_t n, const double *x, const double *y, double *dres) {
+    if (!c || n < 0 || (n && (!x || !y)) || !dres) return set_error(GIP_EINVAL, "gip_ddot: bad argument");
+    return launch_reduce(c, RED_SUM, n, x, y, dres);
+}
+
+int gip_ddot(gip_ctx *c, int32_t n, const double *x, const double *y, double *result) {
+    if (!result) return set_error(GIP_EINVAL, "gip_ddot: null result");
+    int rc = gip_ddot_dev(c, n, x, y, c ? c->d_scalars + kSlotResult : nullptr);
+    if (rc) return rc;
+    return fetch_result(c, c->d_scalars + kSlotResult, result);
+}
+
+int gip_dnrm_inf_dev(gip_ctx *c, int32_t n, const double *x, double *dres) {
+    if (!c || n < 0 || (n && !x) || !dres) return set_error(GIP_EINVAL, "gip_dnrm_inf: bad argument");
+    return launch_reduce(c, RED_MAXABS, n, x, x, dres);
+}
+
+int gip_dnrm_inf(gip_ctx *c, int32_t n, const double *x, double *result) {
+    if (!result) return set_error(GIP_EINVAL, "gip_dnrm_inf: null result");
+    int rc = gip_dnrm_inf_dev(c, n, x, c ? c->d_scalars + kSlotResult : nullptr);
+    if (rc) return rc;
+    return fetch_result(c, c->d_scalars + kSlotResult, result);
+}
+
+int gip_cfl(gip_ctx *c, int32_t n, const double *u, const double *v, const double *w, const double *dxs,
+            double gamma, double rho, double *dt) {
+    if (!c || n < 0 || (n && (!u || !v || !w || !dxs)) || !dt) return set_error(GIP_EINVAL, "gip_cfl: bad argument");
+    CflArgs a{u, v, w, dxs, gamma, rho};
+    int g = reduce_grid(n);
+    k_cfl<<<g, kBlock, 0, c->stream>>>(n, a, c->d_partials, c->d_counter + RED_MIN, c->d_scalars + kSlotResult);
+    GIP_LAUNCH_CHECK("gip_cfl");
+    return fetch_result(c, c->d_scalars + kSlotResult, dt);
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+// Internal declarations shared by the HIP translation units of libgip_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "gip_hip.h"
+
+namespace gip {
+
+int set_error(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define GIP_HIP_TRY(expr)                                   \
+    do {                                                    \
+        hipError_t e__ = (expr);                            \
+        if (e__ != hipSuccess) return gip::hip_fail(e__, #expr); \
+    } while (0)
+
+#define GIP_LAUNCH_CHECK(name)                                         \
+    do {                                                               \
+        hipError_t e__ = hipGetLastError();                            \
+        if (e__ != hipSuccess) return gip::hip_fail(e__, name);        \
+    } while (0)
+
+constexpr int kBlock = 256;        // threads per workgroup for streaming kernels
+constexpr int kMaxRedBlocks = 2048; // partial sums per reduction
+constexpr int kSlice = 64;          // ELL slice height = one wavefront
+
+// Device scalar slots of a context (doubles), used by host-returning reductions.
+enum { kSlotResult = 0, kSlotCount = 8 };
+
+}  // namespace gip
+
+struct gip_ctx {
+    int device;
+    hipStream_t stream;
+    double *d_partials;     // kMaxRedBlocks * 4 doubles of reduction partials
+    unsigned int *d_counter; // 16 completion counters (zeroed, self-resetting)
+    double *d_scalars;      // kSlotCount device scalars
+    double *h_result;       // pinned host landing slot for reduction results
+    int num_cu;
+};
+
+struct gip_mat {
+    gip_ctx *ctx;
+    int32_t nrows, ncols, nnz;
+    int32_t format;
+    // CSR (kept for every format: value get/set, transposed products, diagonal)
+    int32_t *d_rowptr;
+    int32_t *d_col;
+    double *d_val;
+    float *d_val32;
+    // sliced ELL: slice s covers rows [64 s, 64 s + 64); entry k of row
+    // 64 s + l sits at slice_off[s] + 64 k + l; padding has col = -1.
+    int32_t nslices;
+    int32_t uniform_width; // > 0 when every slice has this width and no padding
+    int64_t ell_stored;
+    int32_t *d_slice_off; // nslices + 1
+    int32_t *d_ell_col;
+    double *d_ell_val;
+    float *d_ell_val32;
+    int32_t *d_csr2ell;   // CSR nnz index -> ELL position (value get/set)
+    // a_ii per row (the stored values of a DIAG operator), for Jacobi
+    double *d_diag;
+    int32_t diag_missing;
+    // lazily built transpose: structure once, values refreshed per use
+    gip_mat *transpose;
+    int32_t *d_tsrc;  // per transposed entry: position in this operator's value storage
+    int32_t *d_tdst;  // per transposed entry: position in the transpose's value storage
+};
+
+namespace gip {
+int reduce_grid(int n);
+}

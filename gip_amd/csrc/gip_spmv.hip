@@ -1,0 +1,565 @@
+// Sparse operators of libgip_hip: device layouts (sliced ELL, CSR,
+// diagonal) and the SpMV family of GIP_MatrixMulticore.cpp:64-143.
+//
+// Every kernel sums a row's products in stored (CSR) order with separately
+// rounded multiplies and adds (the library is built without contraction),
+// starting from 0.0, and leaves empty rows untouched: results are bitwise
+// those of the CPU specification.
+//
+// Sliced ELL: slice s holds rows [64 s, 64 s + 64) (one wavefront); entry k
+// of row 64 s + l is stored at slice_off[s] + 64 k + l, so the 64 lanes of a
+// wavefront read 64 consecutive values and indices per entry.  Rows shorter
+// than the slice width are padded with column -1.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "gip_internal.h"
+#include "gip_spmv.h"
+
+namespace gip {
+
+// Workgroup b of G is placed on XCD b % 8 by the dispatcher; renumber the
+// workgroups so that each XCD sweeps one contiguous range of rows and the
+// neighbour gathers of a row range stay in that XCD's L2.
+__device__ inline int xcd_block(int b, int G) {
+    const int x = b & 7, idx = b >> 3, q = G >> 3, r = G & 7;
+    return x * q + (x < r ? x : r) + idx;
+}
+
+template <int MODE, int W>
+__global__ __launch_bounds__(kBlock) void k_ell(int nrows, const int32_t *__restrict__ soff,
+                                               const int32_t *__restrict__ col, const double *__restrict__ val,
+                                               const double *__restrict__ x, double *__restrict__ y, double a,
+                                               double b, const int32_t *__restrict__ scatter) {
+    const int row = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (row >= nrows) return;
+    const int s = row >> 6, l = row & 63;
+    const int base = soff[s] + l;
+    double sum = 0.0;
+    bool any;
+    if (W > 0) {
+        int c[W > 0 ? W : 1];
+        double v[W > 0 ? W : 1];
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            c[k] = __builtin_nontemporal_load(&col[base + 64 * k]);
+            v[k] = __builtin_nontemporal_load(&val[base + 64 * k]);
+        }
+#pragma unroll
+        for (int k = 0; k < W; ++k) sum = sum + x[c[k]] * v[k];
+        any = true;
+    } else {
+        const int width = (soff[s + 1] - soff[s]) >> 6;
+        any = false;
+        for (int k = 0; k < width; ++k) {
+            const int cc = col[base + 64 * k];
+            if (cc < 0) break;
+            sum = sum + x[cc] * val[base + 64 * k];
+            any = true;
+        }
+    }
+    if (!any) return;
+    const int out = scatter ? scatter[row] : row;
+    if (MODE == 0)
+        y[out] = sum;
+    else
+        y[out] = a * sum + b * y[out];
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_csr(int nrows, const int32_t *__restrict__ rp,
+                                               const int32_t *__restrict__ col, const double *__restrict__ val,
+                                               const double *__restrict__ x, double *__restrict__ y, double a,
+                                               double b, const int32_t *__restrict__ scatter) {
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    if (row >= nrows) return;
+    const int jb = rp[row], je = rp[row + 1];
+    if (jb == je) return;
+    double sum = 0.0;
+    for (int j = jb; j < je; ++j) sum = sum + x[col[j]] * val[j];
+    const int out = scatter ? scatter[row] : row;
+    if (MODE == 0)
+        y[out] = sum;
+    else
+        y[out] = a * sum + b * y[out];
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_diag(int nrows, const double *__restrict__ d, const double *__restrict__ x,
+                                                double *__restrict__ y, double a, double b,
+                                                const int32_t *__restrict__ scatter) {
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    if (row >= nrows) return;
+    double sum = 0.0;
+    sum = sum + x[row] * d[row];
+    const int out = scatter ? scatter[row] : row;
+    if (MODE == 0)
+        y[out] = sum;
+    else
+        y[out] = a * sum + b * y[out];
+}
+
+// FP32 products for the mixed-precision inner solver (no empty-row rule needed:
+// rows without entries produce 0).
+__global__ __launch_bounds__(kBlock) void k_ell32(int nrows, const int32_t *__restrict__ soff,
+                                                 const int32_t *__restrict__ col, const float *__restrict__ val,
+                                                 const float *__restrict__ x, float *__restrict__ y) {
+    const int row = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (row >= nrows) return;
+    const int s = row >> 6, l = row & 63;
+    const int base = soff[s] + l, width = (soff[s + 1] - soff[s]) >> 6;
+    float sum = 0.0f;
+    for (int k = 0; k < width; ++k) {
+        const int cc = col[base + 64 * k];
+        if (cc < 0) break;
+        sum = sum + x[cc] * val[base + 64 * k];
+    }
+    y[row] = sum;
+}
+
+__global__ __launch_bounds__(kBlock) void k_csr32(int nrows, const int32_t *__restrict__ rp,
+                                                 const int32_t *__restrict__ col, const float *__restrict__ val,
+                                                 const float *__restrict__ x, float *__restrict__ y) {
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    if (row >= nrows) return;
+    float sum = 0.0f;
+    for (int j = rp[row]; j < rp[row + 1]; ++j) sum = sum + x[col[j]] * val[j];
+    y[row] = sum;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_vals(int n, const double *__restrict__ src,
+                                                       const int32_t *__restrict__ map, double *__restrict__ dst) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) dst[i] = src[map[i]];
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter_vals(int n, const double *__restrict__ src,
+                                                        const int32_t *__restrict__ map, double *__restrict__ dst) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) dst[map[i]] = src[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_inv(int n, const double *__restrict__ d, double *__restrict__ out) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) out[i] = 1.0 / d[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_f32(int64_t n, const double *__restrict__ src, float *__restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) dst[i] = (float)src[i];
+}
+
+static inline int nblocks(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
+
+template <class T>
+static int upload(gip_ctx *c, T **dptr, const T *host, size_t count) {
+    GIP_HIP_TRY(hipMalloc(dptr, sizeof(T) * (count ? count : 1)));
+    if (count) GIP_HIP_TRY(hipMemcpyAsync(*dptr, host, sizeof(T) * count, hipMemcpyHostToDevice, c->stream));
+    return GIP_OK;
+}
+
+static void free_mat_arrays(gip_mat *A) {
+    void *ptrs[] = {A->d_rowptr, A->d_col,     A->d_val,  A->d_val32, A->d_slice_off, A->d_ell_col,
+                    A->d_ell_val, A->d_ell_val32, A->d_csr2ell, A->d_diag, A->d_tsrc,  A->d_tdst};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+}
+
+// Dispatch y (= or +=) A x in FP64.  scatter: optional output index per row.
+int spmv_launch(gip_ctx *c, const gip_mat *A, const double *x, double *y, double a, double b, int mode,
+                const int32_t *scatter) {
+    if (A->nrows == 0) return GIP_OK;
+    const int g = nblocks(A->nrows);
+    hipStream_t st = c->stream;
+    if (A->format == GIP_FMT_ELL) {
+#define GIP_ELL_CASE(Wc)                                                                                        \
+    case Wc:                                                                                                    \
+        if (mode == 0)                                                                                          \
+            k_ell<0, Wc><<<g, kBlock, 0, st>>>(A->nrows, A->d_slice_off, A->d_ell_col, A->d_ell_val, x, y, a, b, \
+                                               scatter);                                                        \
+        else                                                                                                    \
+            k_ell<1, Wc><<<g, kBlock, 0, st>>>(A->nrows, A->d_slice_off, A->d_ell_col, A->d_ell_val, x, y, a, b, \
+                                               scatter);                                                        \
+        break;
+        switch (A->uniform_width) {
+            GIP_ELL_CASE(1)
+            GIP_ELL_CASE(2)
+            GIP_ELL_CASE(3)
+            GIP_ELL_CASE(4)
+            GIP_ELL_CASE(5)
+            GIP_ELL_CASE(6)
+            GIP_ELL_CASE(7)
+            GIP_ELL_CASE(8)
+            default:
+                if (mode == 0)
+                    k_ell<0, 0><<<g, kBlock, 0, st>>>(A->nrows, A->d_slice_off, A->d_ell_col, A->d_ell_val, x, y, a,
+                                                     b, scatter);
+                else
+                    k_ell<1, 0><<<g, kBlock, 0, st>>>(A->nrows, A->d_slice_off, A->d_ell_col, A->d_ell_val, x, y, a,
+                                                     b, scatter);
+        }
+#undef GIP_ELL_CASE
+    } else if (A->format == GIP_FMT_DIAG) {
+        if (mode == 0)
+            k_diag<0><<<g, kBlock, 0, st>>>(A->nrows, A->d_diag, x, y, a, b, scatter);
+        else
+            k_diag<1><<<g, kBlock, 0, st>>>(A->nrows, A->d_diag, x, y, a, b, scatter);
+    } else {
+        if (mode == 0)
+            k_csr<0><<<g, kBlock, 0, st>>>(A->nrows, A->d_rowptr, A->d_col, A->d_val, x, y, a, b, scatter);
+        else
+            k_csr<1><<<g, kBlock, 0, st>>>(A->nrows, A->d_rowptr, A->d_col, A->d_val, x, y, a, b, scatter);
+    }
+    GIP_LAUNCH_CHECK("spmv kernel");
+    return GIP_OK;
+}
+
+int spmv32_launch(gip_ctx *c, const gip_mat *A, const float *x, float *y) {
+    if (A->nrows == 0) return GIP_OK;
+    if (A->format == GIP_FMT_ELL) {
+        if (!A->d_ell_val32) return set_error(GIP_EINVAL, "operator has no FP32 values");
+        k_ell32<<<nblocks(A->nrows), kBlock, 0, c->stream>>>(A->nrows, A->d_slice_off, A->d_ell_col, A->d_ell_val32, x, y);
+    } else if (A->format == GIP_FMT_CSR) {
+        if (!A->d_val32) return set_error(GIP_EINVAL, "operator has no FP32 values");
+        k_csr32<<<nblocks(A->nrows), kBlock, 0, c->stream>>>(A->nrows, A->d_rowptr, A->d_col, A->d_val32, x, y);
+    } else {
+        return set_error(GIP_EINVAL, "FP32 SpMV needs an ELL or CSR operator");
+    }
+    GIP_LAUNCH_CHECK("spmv32 kernel");
+    return GIP_OK;
+}
+
+}  // namespace gip
+
+using namespace gip;
+
+extern "C" {
+
+int gip_mat_create(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, const int32_t *rowptr, const int32_t *col,
+                   const double *val, int32_t flags, gip_mat **out) {
+    if (!c || !out || nrows < 0 || nnz < 0 || !rowptr || (nnz && (!col || !val)))
+        return set_error(GIP_EINVAL, "gip_mat_create: bad argument");
+    *out = nullptr;
+    if (rowptr[0] != 0 || rowptr[nrows] != nnz) return set_error(GIP_EINVAL, "gip_mat_create: rowptr does not span nnz");
+    for (int32_t i = 0; i < nrows; ++i)
+        if (rowptr[i + 1] < rowptr[i]) return set_error(GIP_EINVAL, "gip_mat_create: rowptr not monotone");
+    for (int32_t j = 0; j < nnz; ++j)
+        if (col[j] < 0) return set_error(GIP_EINVAL, "gip_mat_create: negative column index");
+    GIP_HIP_TRY(hipSetDevice(c->device));
+    gip_mat *A = new (std::nothrow) gip_mat();
+    if (!A) return set_error(GIP_ENOMEM, "gip_mat_create: allocation failed");
+    A->ctx = c;
+    A->nrows = nrows;
+    A->ncols = ncols;
+    A->nnz = nnz;
+    const int32_t want = flags & 0xff;
+    const bool fp32 = (flags & GIP_MAT_FP32) != 0;
+
+    // diagonal value per row (last diagonal entry wins, GIP_MatrixMulticore.cpp:191-204)
+    std::vector<double> diag((size_t)nrows, 0.0);
+    A->diag_missing = 0;
+    bool is_diag = (nnz == nrows);
+    for (int32_t i = 0; i < nrows; ++i) {
+        bool found = false;
+        for (int32_t j = rowptr[i]; j < rowptr[i + 1]; ++j)
+            if (col[j] == i) {
+                diag[(size_t)i] = val[j];
+                found = true;
+            }
+        if (!found) A->diag_missing = 1;
+        if (is_diag && !(rowptr[i] == i && col[i] == i)) is_diag = false;
+    }
+    // slice widths
+    int32_t ns = (nrows + kSlice - 1) / kSlice;
+    std::vector<int32_t> soff((size_t)ns + 1, 0);
+    int32_t uw = -1;
+    bool uniform = true;
+    int64_t stored = 0;
+    for (int32_t s = 0; s < ns; ++s) {
+        int32_t w = 0;
+        for (int32_t r = s * kSlice; r < std::min(nrows, (s + 1) * kSlice); ++r) {
+            int32_t len = rowptr[r + 1] - rowptr[r];
+            w = std::max(w, len);
+            if (uw < 0) uw = len;
+            if (len != uw) uniform = false;
+        }
+        stored += (int64_t)w * kSlice;
+        if (stored > INT32_MAX) uniform = false;
+        soff[(size_t)s + 1] = (int32_t)std::min<int64_t>(stored, INT32_MAX);
+    }
+    int32_t fmt = want;
+    if (fmt == GIP_FMT_AUTO) {
+        if (is_diag && nrows > 0)
+            fmt = GIP_FMT_DIAG;
+        else if (stored <= INT32_MAX && stored <= (int64_t)(1.2 * nnz) + kSlice * 8)
+            fmt = GIP_FMT_ELL;
+        else
+            fmt = GIP_FMT_CSR;
+    }
+    if (fmt == GIP_FMT_DIAG && !is_diag) fmt = GIP_FMT_CSR;
+    if (fmt == GIP_FMT_ELL && stored > INT32_MAX) fmt = GIP_FMT_CSR;
+    A->format = fmt;
+    int rc = GIP_OK;
+    // CSR structure always kept (transposes, value maps)
+    rc = upload(c, &A->d_rowptr, rowptr, (size_t)nrows + 1);
+    if (!rc) rc = upload(c, &A->d_diag, diag.data(), diag.size());
+    if (!rc && fmt == GIP_FMT_ELL) {
+        A->nslices = ns;
+        A->ell_stored = stored;
+        A->uniform_width = (uniform && uw > 0 && uw <= 8 && (int64_t)uw * kSlice * ns == stored) ? uw : 0;
+        if (uniform && uw > 0 && uw <= 8 && nrows % kSlice != 0) A->uniform_width = uw;  // tail rows never run
+        std::vector<int32_t> ecol((size_t)stored, -1);
+        std::vector<double> evalv((size_t)stored, 0.0);
+        std::vector<int32_t> c2e((size_t)nnz);
+        for (int32_t r = 0; r < nrows; ++r) {
+            int32_t s = r / kSlice, l = r % kSlice;
+            for (int32_t j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+                int64_t pos = soff[(size_t)s] + (int64_t)kSlice * (j - rowptr[r]) + l;
+                ecol[(size_t)pos] = col[j];
+                evalv[(size_t)pos] = val[j];
+                c2e[(size_t)j] = (int32_t)pos;
+            }
+        }
+        // padding slots of the tail slice must not be read by a uniform kernel: rows >= nrows return early
+        rc = upload(c, &A->d_slice_off, soff.data(), soff.size());
+        if (!rc) rc = upload(c, &A->d_ell_col, ecol.data(), ecol.size());
+        if (!rc) rc = upload(c, &A->d_ell_val, evalv.data(), evalv.size());
+        if (!rc) rc = upload(c, &A->d_csr2ell, c2e.data(), c2e.size());
+        if (!rc && fp32) {
+            GIP_HIP_TRY(hipStreamSynchronize(c->stream));
+            hipError_t e = hipMalloc(&A->d_ell_val32, sizeof(float) * (size_t)std::max<int64_t>(stored, 1));
+            if (e != hipSuccess) rc = hip_fail(e, "hipMalloc fp32 values");
+            else if (stored) k_to_f32<<<nblocks(stored), kBlock, 0, c->stream>>>(stored, A->d_ell_val, A->d_ell_val32);
+        }
+        // CSR columns for transposes
+        if (!rc) rc = upload(c, &A->d_col, col, (size_t)nnz);
+    } else if (!rc) {
+        A->ell_stored = nnz;
+        rc = upload(c, &A->d_col, col, (size_t)nnz);
+        if (!rc) rc = upload(c, &A->d_val, val, (size_t)nnz);
+        if (!rc && fp32 && fmt == GIP_FMT_CSR) {
+            GIP_HIP_TRY(hipStreamSynchronize(c->stream));
+            hipError_t e = hipMalloc(&A->d_val32, sizeof(float) * (size_t)std::max(nnz, 1));
+            if (e != hipSuccess) rc = hip_fail(e, "hipMalloc fp32 values");
+            else if (nnz) k_to_f32<<<nblocks(nnz), kBlock, 0, c->stream>>>(nnz, A->d_val, A->d_val32);
+        }
+    }
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "gip_mat_create upload");
+    }
+    if (rc) {
+        free_mat_arrays(A);
+        delete A;
+        return rc;
+    }
+    *out = A;
+    return GIP_OK;
+}
+
+int gip_mat_destroy(gip_mat *A) {
+    if (!A) return GIP_OK;
+    hipSetDevice(A->ctx->device);
+    hipStreamSynchronize(A->ctx->stream);
+    if (A->transpose) gip_mat_destroy(A->transpose);
+    free_mat_arrays(A);
+    delete A;
+    return GIP_OK;
+}
+
+int gip_mat_info(const gip_mat *A, int32_t *nrows, int32_t *nnz, int32_t *format, int64_t *stored) {
+    if (!A) return set_error(GIP_EINVAL, "null operator");
+    if (nrows) *nrows = A->nrows;
+    if (nnz) *nnz = A->nnz;
+    if (format) *format = A->format;
+    if (stored) *stored = A->ell_stored;
+    return GIP_OK;
+}
+
+// Device pointer to the values in CSR order, or an ELL map.
+static int values_csr_order(const gip_mat *A, double *dst_dev) {
+    gip_ctx *c = A->ctx;
+    if (A->nnz == 0) return GIP_OK;
+    if (A->format == GIP_FMT_ELL) {
+        k_gather_vals<<<nblocks(A->nnz), kBlock, 0, c->stream>>>(A->nnz, A->d_ell_val, A->d_csr2ell, dst_dev);
+        GIP_LAUNCH_CHECK("gather values");
+    } else if (A->format == GIP_FMT_DIAG) {
+        GIP_HIP_TRY(hipMemcpyAsync(dst_dev, A->d_diag, sizeof(double) * A->nnz, hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        GIP_HIP_TRY(hipMemcpyAsync(dst_dev, A->d_val, sizeof(double) * A->nnz, hipMemcpyDeviceToDevice, c->stream));
+    }
+    return GIP_OK;
+}
+
+int gip_mat_get_values(const gip_mat *A, double *host) {
+    if (!A || (A->nnz && !host)) return set_error(GIP_EINVAL, "gip_mat_get_values: bad argument");
+    if (!A->nnz) return GIP_OK;
+    gip_ctx *c = A->ctx;
+    double *tmp = nullptr;
+    GIP_HIP_TRY(hipMalloc(&tmp, sizeof(double) * A->nnz));
+    int rc = values_csr_order(A, tmp);
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(host, tmp, sizeof(double) * A->nnz, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "gip_mat_get_values");
+    }
+    hipFree(tmp);
+    return rc;
+}
+
+int gip_mat_set_values(gip_mat *A, const double *host) {
+    if (!A || (A->nnz && !host)) return set_error(GIP_EINVAL, "gip_mat_set_values: bad argument");
+    if (!A->nnz) return GIP_OK;
+    gip_ctx *c = A->ctx;
+    if (A->format == GIP_FMT_ELL) {
+        double *tmp = nullptr;
+        GIP_HIP_TRY(hipMalloc(&tmp, sizeof(double) * A->nnz));
+        hipError_t e = hipMemcpyAsync(tmp, host, sizeof(double) * A->nnz, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) {
+            k_scatter_vals<<<nblocks(A->nnz), kBlock, 0, c->stream>>>(A->nnz, tmp, A->d_csr2ell, A->d_ell_val);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        hipFree(tmp);
+        if (e != hipSuccess) return hip_fail(e, "gip_mat_set_values");
+        if (A->d_ell_val32) k_to_f32<<<nblocks(A->ell_stored), kBlock, 0, c->stream>>>(A->ell_stored, A->d_ell_val, A->d_ell_val32);
+    } else {
+        double *dst = A->format == GIP_FMT_DIAG ? A->d_diag : A->d_val;
+        GIP_HIP_TRY(hipMemcpyAsync(dst, host, sizeof(double) * A->nnz, hipMemcpyHostToDevice, c->stream));
+        GIP_HIP_TRY(hipStreamSynchronize(c->stream));
+        if (A->d_val32) k_to_f32<<<nblocks(A->nnz), kBlock, 0, c->stream>>>(A->nnz, A->d_val, A->d_val32);
+    }
+    GIP_LAUNCH_CHECK("gip_mat_set_values");
+    return GIP_OK;
+}
+
+int gip_mat_diag_inv(const gip_mat *A, double *dinv) {
+    if (!A || (A->nrows && !dinv)) return set_error(GIP_EINVAL, "gip_mat_diag_inv: bad argument");
+    if (A->diag_missing) return set_error(GIP_EINVAL, "gip_mat_diag_inv: a row has no diagonal entry");
+    if (!A->nrows) return GIP_OK;
+    k_inv<<<nblocks(A->nrows), kBlock, 0, A->ctx->stream>>>(A->nrows, A->d_diag, dinv);
+    GIP_LAUNCH_CHECK("gip_mat_diag_inv");
+    return GIP_OK;
+}
+
+int gip_dcsrmv(gip_ctx *c, const gip_mat *A, const double *x, double *y) {
+    if (!c || !A || (A->nrows && (!x || !y))) return set_error(GIP_EINVAL, "gip_dcsrmv: bad argument");
+    return spmv_launch(c, A, x, y, 1.0, 0.0, 0, nullptr);
+}
+
+int gip_dcsrmv_ab(gip_ctx *c, const gip_mat *A, const double *x, double *y, double alpha, double beta) {
+    if (!c || !A || (A->nrows && (!x || !y))) return set_error(GIP_EINVAL, "gip_dcsrmv_ab: bad argument");
+    return spmv_launch(c, A, x, y, alpha, beta, 1, nullptr);
+}
+
+int gip_dcsrmv_vals(gip_ctx *c, const gip_mat *A, const double *x, gip_mat *B) {
+    if (!c || !A || !B || (A->nrows && !x)) return set_error(GIP_EINVAL, "gip_dcsrmv_vals: bad argument");
+    if (A->nrows != B->nnz) return set_error(GIP_EINVAL, "gip_dcsrmv_vals: rows of A must equal nnz of B");
+    int rc;
+    if (B->format == GIP_FMT_ELL)
+        rc = spmv_launch(c, A, x, B->d_ell_val, 1.0, 0.0, 0, B->d_csr2ell);
+    else if (B->format == GIP_FMT_DIAG)
+        rc = spmv_launch(c, A, x, B->d_diag, 1.0, 0.0, 0, nullptr);
+    else
+        rc = spmv_launch(c, A, x, B->d_val, 1.0, 0.0, 0, nullptr);
+    if (rc) return rc;
+    if (B->d_ell_val32) k_to_f32<<<nblocks(B->ell_stored), kBlock, 0, c->stream>>>(B->ell_stored, B->d_ell_val, B->d_ell_val32);
+    if (B->d_val32) k_to_f32<<<nblocks(B->nnz), kBlock, 0, c->stream>>>(B->nnz, B->d_val, B->d_val32);
+    GIP_LAUNCH_CHECK("gip_dcsrmv_vals");
+    return GIP_OK;
+}
+
+int gip_scsrmv(gip_ctx *c, const gip_mat *A, const float *x, float *y) {
+    if (!c || !A || (A->nrows && (!x || !y))) return set_error(GIP_EINVAL, "gip_scsrmv: bad argument");
+    return spmv32_launch(c, A, x, y);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill0(int n, double *__restrict__ y) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) y[i] = 0.0;
+}
+
+static double *value_store(const gip_mat *A) {
+    return A->format == GIP_FMT_ELL ? A->d_ell_val : A->format == GIP_FMT_DIAG ? A->d_diag : A->d_val;
+}
+
+__global__ __launch_bounds__(kBlock) void k_refresh(int n, const double *__restrict__ src, const int32_t *__restrict__ sidx,
+                                                   double *__restrict__ dst, const int32_t *__restrict__ didx) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) dst[didx[i]] = src[sidx[i]];
+}
+
+// Build the transpose of A once: structure by a counting sort on the host
+// (a row of A^T lists the rows of A in ascending order), plus the maps that
+// let its values be refreshed from A's current values on the device.
+static int build_transpose(gip_ctx *c, gip_mat *A) {
+    std::vector<int32_t> rp((size_t)A->nrows + 1), cl((size_t)A->nnz), c2e;
+    hipError_t e = hipMemcpyAsync(rp.data(), A->d_rowptr, sizeof(int32_t) * rp.size(), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && A->nnz)
+        e = hipMemcpyAsync(cl.data(), A->d_col, sizeof(int32_t) * cl.size(), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && A->format == GIP_FMT_ELL && A->nnz) {
+        c2e.resize((size_t)A->nnz);
+        e = hipMemcpyAsync(c2e.data(), A->d_csr2ell, sizeof(int32_t) * c2e.size(), hipMemcpyDeviceToHost, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(e, "transpose structure");
+    int32_t nc = 0;
+    for (int32_t v : cl) nc = std::max(nc, v + 1);
+    std::vector<int32_t> trp((size_t)nc + 1, 0), tcl((size_t)A->nnz), tsrc((size_t)A->nnz);
+    for (int32_t v : cl) trp[(size_t)v + 1]++;
+    for (int32_t k = 0; k < nc; ++k) trp[(size_t)k + 1] += trp[(size_t)k];
+    std::vector<int32_t> fillp(trp.begin(), trp.end() - 1);
+    for (int32_t r = 0; r < A->nrows; ++r)
+        for (int32_t j = rp[(size_t)r]; j < rp[(size_t)r + 1]; ++j) {
+            int32_t p = fillp[(size_t)cl[(size_t)j]]++;
+            tcl[(size_t)p] = r;
+            tsrc[(size_t)p] = c2e.empty() ? j : c2e[(size_t)j];
+        }
+    std::vector<double> zeros((size_t)A->nnz, 0.0);
+    gip_mat *T = nullptr;
+    int rc = gip_mat_create(c, nc, A->nrows, A->nnz, trp.data(), tcl.data(), zeros.data(), GIP_FMT_AUTO, &T);
+    if (rc) return rc;
+    std::vector<int32_t> tdst((size_t)A->nnz);
+    if (T->format == GIP_FMT_ELL && A->nnz) {
+        e = hipMemcpy(tdst.data(), T->d_csr2ell, sizeof(int32_t) * tdst.size(), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            gip_mat_destroy(T);
+            return hip_fail(e, "transpose map");
+        }
+    } else {
+        for (int32_t p = 0; p < A->nnz; ++p) tdst[(size_t)p] = p;
+    }
+    rc = upload(c, &A->d_tsrc, tsrc.data(), tsrc.size());
+    if (!rc) rc = upload(c, &A->d_tdst, tdst.data(), tdst.size());
+    if (rc) {
+        gip_mat_destroy(T);
+        return rc;
+    }
+    A->transpose = T;
+    return GIP_OK;
+}
+
+int gip_dcsrmv_t(gip_ctx *c, const gip_mat *Ac, const double *x, double *y, int32_t ncols_out) {
+    gip_mat *A = const_cast<gip_mat *>(Ac);
+    if (!c || !A || ncols_out < 0 || (ncols_out && !y) || (A->nrows && !x))
+        return set_error(GIP_EINVAL, "gip_dcsrmv_t: bad argument");
+    if (!A->transpose) {
+        int rc = build_transpose(c, A);
+        if (rc) return rc;
+    }
+    gip_mat *T = A->transpose;
+    if (T->nrows > ncols_out) return set_error(GIP_EINVAL, "gip_dcsrmv_t: ncols_out smaller than the largest column + 1");
+    if (A->nnz) {
+        k_refresh<<<nblocks(A->nnz), kBlock, 0, c->stream>>>(A->nnz, value_store(A), A->d_tsrc, value_store(T), A->d_tdst);
+        GIP_LAUNCH_CHECK("gip_dcsrmv_t refresh");
+    }
+    if (ncols_out) {
+        k_fill0<<<nblocks(ncols_out), kBlock, 0, c->stream>>>(ncols_out, y);
+        GIP_LAUNCH_CHECK("gip_dcsrmv_t fill");
+    }
+    return spmv_launch(c, T, x, y, 1.0, 0.0, 0, nullptr);
+}
+
+}  // extern "C"

@@ -1,0 +1,306 @@
+"""Python view of the gfx950 kernel library's C ABI (include/gip_hip.h).
+
+Thin ctypes wrappers: a Device (context + stream), DeviceArray buffers
+moved to/from numpy, Matrix operators, the vector operations and the fused
+PCG.  Every call goes through libgip_hip.so; there is no other code path.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import POINTER, byref, c_double, c_float, c_int, c_int32, c_int64, c_size_t, c_void_p, load
+
+FMT_AUTO, FMT_CSR, FMT_ELL, FMT_DIAG = 0, 1, 2, 3
+MAT_FP32 = 1 << 8
+PCG_FP64, PCG_MIXED = 0, 1
+OK, EINVAL, EHIP, ENOMEM, EBREAKDOWN, ENOCONV = 0, -1, -2, -3, -4, -5
+
+
+class PcgOpts(ctypes.Structure):
+    _fields_ = [("maxit", c_int32), ("tol", c_double), ("precond", c_int32), ("precision", c_int32),
+                ("inner_tol", c_double), ("poll", c_int32)]
+
+
+class PcgStats(ctypes.Structure):
+    _fields_ = [("iters", c_int32), ("corrections", c_int32), ("relres", c_double), ("status", c_int32)]
+
+
+# exported entry points and their argument types (kept in step with include/gip_hip.h)
+SIGNATURES = {
+    "gip_last_error": ([], ctypes.c_char_p),
+    "gip_init": ([c_int, POINTER(c_void_p)], c_int),
+    "gip_finalize": ([c_void_p], c_int),
+    "gip_sync": ([c_void_p], c_int),
+    "gip_stream": ([c_void_p], c_void_p),
+    "gip_device_count": ([POINTER(c_int)], c_int),
+    "gip_malloc": ([c_void_p, c_size_t, POINTER(c_void_p)], c_int),
+    "gip_free": ([c_void_p, c_void_p], c_int),
+    "gip_memcpy_h2d": ([c_void_p, c_void_p, c_void_p, c_size_t], c_int),
+    "gip_memcpy_d2h": ([c_void_p, c_void_p, c_void_p, c_size_t], c_int),
+    "gip_memcpy_d2d": ([c_void_p, c_void_p, c_void_p, c_size_t], c_int),
+    "gip_event_create": ([c_void_p, POINTER(c_void_p)], c_int),
+    "gip_event_record": ([c_void_p, c_void_p], c_int),
+    "gip_event_elapsed_ms": ([c_void_p, c_void_p, POINTER(c_float)], c_int),
+    "gip_event_destroy": ([c_void_p], c_int),
+    "gip_mat_create": ([c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_int32,
+                        POINTER(c_void_p)], c_int),
+    "gip_mat_destroy": ([c_void_p], c_int),
+    "gip_mat_info": ([c_void_p, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32), POINTER(c_int64)], c_int),
+    "gip_mat_get_values": ([c_void_p, c_void_p], c_int),
+    "gip_mat_set_values": ([c_void_p, c_void_p], c_int),
+    "gip_mat_diag_inv": ([c_void_p, c_void_p], c_int),
+    "gip_dcsrmv": ([c_void_p, c_void_p, c_void_p, c_void_p], c_int),
+    "gip_dcsrmv_ab": ([c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double], c_int),
+    "gip_dcsrmv_vals": ([c_void_p, c_void_p, c_void_p, c_void_p], c_int),
+    "gip_dcsrmv_t": ([c_void_p, c_void_p, c_void_p, c_void_p, c_int32], c_int),
+    "gip_scsrmv": ([c_void_p, c_void_p, c_void_p, c_void_p], c_int),
+    "gip_dfill": ([c_void_p, c_int32, c_void_p, c_double], c_int),
+    "gip_daxpy": ([c_void_p, c_int32, c_double, c_void_p, c_void_p], c_int),
+    "gip_daxpby": ([c_void_p, c_int32, c_double, c_void_p, c_double, c_void_p], c_int),
+    "gip_ddot": ([c_void_p, c_int32, c_void_p, c_void_p, POINTER(c_double)], c_int),
+    "gip_dnrm_inf": ([c_void_p, c_int32, c_void_p, POINTER(c_double)], c_int),
+    "gip_cfl": ([c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double,
+                 POINTER(c_double)], c_int),
+    "gip_ddot_dev": ([c_void_p, c_int32, c_void_p, c_void_p, c_void_p], c_int),
+    "gip_dnrm_inf_dev": ([c_void_p, c_int32, c_void_p, c_void_p], c_int),
+    "gip_pcg_create": ([c_void_p, c_void_p, c_int32, POINTER(c_void_p)], c_int),
+    "gip_pcg_solve": ([c_void_p, c_void_p, c_void_p, c_void_p, POINTER(PcgOpts), POINTER(PcgStats)], c_int),
+    "gip_pcg_destroy": ([c_void_p], c_int),
+}
+
+
+def lib():
+    L = load("gip_hip")
+    if not getattr(L, "_gip_typed", False):
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        L._gip_typed = True
+    return L
+
+
+class HipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"gip_hip error {code}: {msg}")
+        self.code = code
+
+
+def check(rc):
+    if rc != 0:
+        raise HipError(rc, lib().gip_last_error().decode())
+    return rc
+
+
+class Device:
+    def __init__(self, device: int = 0):
+        self.ctx = c_void_p()
+        check(lib().gip_init(device, byref(self.ctx)))
+
+    def close(self):
+        if self.ctx:
+            lib().gip_finalize(self.ctx)
+            self.ctx = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        check(lib().gip_sync(self.ctx))
+
+    def array(self, n, dtype=np.float64):
+        return DeviceArray(self, n, dtype)
+
+    def to_device(self, host):
+        host = np.ascontiguousarray(host)
+        a = DeviceArray(self, host.size, host.dtype)
+        a.upload(host)
+        return a
+
+
+class DeviceArray:
+    def __init__(self, dev: Device, n: int, dtype=np.float64):
+        self.dev, self.n, self.dtype = dev, int(n), np.dtype(dtype)
+        self.ptr = c_void_p()
+        check(lib().gip_malloc(dev.ctx, max(1, self.n) * self.dtype.itemsize, byref(self.ptr)))
+
+    @property
+    def nbytes(self):
+        return self.n * self.dtype.itemsize
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, dtype=self.dtype)
+        assert host.size == self.n, (host.size, self.n)
+        check(lib().gip_memcpy_h2d(self.dev.ctx, self.ptr, host.ctypes.data, self.nbytes))
+        return self
+
+    def download(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        check(lib().gip_memcpy_d2h(self.dev.ctx, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().gip_free(self.dev.ctx, self.ptr)
+            self.ptr = c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Matrix:
+    """A device operator built from host CSR arrays (int32 rowptr/col, float64 val)."""
+
+    def __init__(self, dev: Device, rowptr, col, val, nrows=None, ncols=0, fmt=FMT_AUTO, fp32=False):
+        self.dev = dev
+        rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        col = np.ascontiguousarray(col, dtype=np.int32)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        self.nrows = int(nrows if nrows is not None else rowptr.size - 1)
+        self.nnz = int(val.size)
+        self.ptr = c_void_p()
+        flags = fmt | (MAT_FP32 if fp32 else 0)
+        check(lib().gip_mat_create(dev.ctx, self.nrows, int(ncols), self.nnz, rowptr.ctypes.data, col.ctypes.data,
+                                   val.ctypes.data, flags, byref(self.ptr)))
+
+    @classmethod
+    def from_csr(cls, dev, csr, **kw):
+        return cls(dev, csr.rowptr, csr.col, csr.val, nrows=csr.nrows, ncols=csr.ncols, **kw)
+
+    def info(self):
+        nr, nz, fm, st = c_int32(), c_int32(), c_int32(), c_int64()
+        check(lib().gip_mat_info(self.ptr, byref(nr), byref(nz), byref(fm), byref(st)))
+        return {"nrows": nr.value, "nnz": nz.value, "format": fm.value, "stored": st.value}
+
+    def values(self):
+        out = np.empty(self.nnz, np.float64)
+        check(lib().gip_mat_get_values(self.ptr, out.ctypes.data))
+        return out
+
+    def set_values(self, vals):
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        check(lib().gip_mat_set_values(self.ptr, vals.ctypes.data))
+
+    def spmv(self, x: DeviceArray, y: DeviceArray):
+        check(lib().gip_dcsrmv(self.dev.ctx, self.ptr, x.ptr, y.ptr))
+
+    def spmv_ab(self, x, y, alpha, beta):
+        check(lib().gip_dcsrmv_ab(self.dev.ctx, self.ptr, x.ptr, y.ptr, alpha, beta))
+
+    def spmv_vals(self, x, B: "Matrix"):
+        check(lib().gip_dcsrmv_vals(self.dev.ctx, self.ptr, x.ptr, B.ptr))
+
+    def spmv_t(self, x, y, ncols_out):
+        check(lib().gip_dcsrmv_t(self.dev.ctx, self.ptr, x.ptr, y.ptr, ncols_out))
+
+    def spmv32(self, x, y):
+        check(lib().gip_scsrmv(self.dev.ctx, self.ptr, x.ptr, y.ptr))
+
+    def diag_inv(self, out: DeviceArray):
+        check(lib().gip_mat_diag_inv(self.ptr, out.ptr))
+
+    def destroy(self):
+        if self.ptr:
+            lib().gip_mat_destroy(self.ptr)
+            self.ptr = c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def fill(x: DeviceArray, value, n=None):
+    check(lib().gip_dfill(x.dev.ctx, x.n if n is None else n, x.ptr, value))
+
+
+def axpy(alpha, x, y, n=None):
+    check(lib().gip_daxpy(y.dev.ctx, y.n if n is None else n, alpha, x.ptr, y.ptr))
+
+
+def axpby(alpha, x, beta, y, n=None):
+    check(lib().gip_daxpby(y.dev.ctx, y.n if n is None else n, alpha, x.ptr, beta, y.ptr))
+
+
+def dot(x, y, n=None):
+    r = c_double()
+    check(lib().gip_ddot(x.dev.ctx, x.n if n is None else n, x.ptr, y.ptr, byref(r)))
+    return r.value
+
+
+def nrm_inf(x, n=None):
+    r = c_double()
+    check(lib().gip_dnrm_inf(x.dev.ctx, x.n if n is None else n, x.ptr, byref(r)))
+    return r.value
+
+
+def cfl(u, v, w, dxs, n, gamma, rho):
+    r = c_double()
+    check(lib().gip_cfl(u.dev.ctx, n, u.ptr, v.ptr, w.ptr, dxs.ptr, gamma, rho, byref(r)))
+    return r.value
+
+
+@dataclass
+class PcgResult:
+    iters: int
+    corrections: int
+    relres: float
+    status: int
+
+
+class PCG:
+    def __init__(self, A: Matrix, n_all: int):
+        self.A, self.dev = A, A.dev
+        self.ptr = c_void_p()
+        check(lib().gip_pcg_create(self.dev.ctx, A.ptr, n_all, byref(self.ptr)))
+
+    def solve(self, b, x, x0, *, maxit=100, tol=0.0, precond=True, precision=PCG_FP64, inner_tol=1e-3,
+              poll=16, raise_on_breakdown=True):
+        o = PcgOpts(maxit, tol, 1 if precond else 0, precision, inner_tol, poll)
+        st = PcgStats()
+        rc = lib().gip_pcg_solve(self.ptr, b.ptr, x.ptr, x0.ptr, byref(o), byref(st))
+        if rc != 0 and (raise_on_breakdown or rc != EBREAKDOWN):
+            check(rc)
+        return PcgResult(st.iters, st.corrections, st.relres, st.status)
+
+    def destroy(self):
+        if self.ptr:
+            lib().gip_pcg_destroy(self.ptr)
+            self.ptr = c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self, dev: Device):
+        self.dev = dev
+        self.ptr = c_void_p()
+        check(lib().gip_event_create(dev.ctx, byref(self.ptr)))
+
+    def record(self):
+        check(lib().gip_event_record(self.dev.ctx, self.ptr))
+
+    def elapsed_ms(self, later: "Event"):
+        ms = c_float()
+        check(lib().gip_event_elapsed_ms(self.ptr, later.ptr, byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            lib().gip_event_destroy(self.ptr)
+        except Exception:
+            pass

@@ -1,0 +1,163 @@
+/*
+ * gip_hip.h — C ABI of the MI355X (gfx950) kernel library behind the HIP
+ * specification of GIP's algebraic fractional-step Navier–Stokes path.
+ *
+ * It replaces the reference's CUDA kernel library interface
+ * (src/Specifications/Cuda/Headers/GIP_gpu.h:1-47, kernels in
+ * src/Specifications/Cuda/Libs/libgpu.cu), with one C name per overload,
+ * status returns instead of silent failures, and the CPU semantics of the
+ * Multicore specification where the CUDA one differs (empty CSR rows are
+ * left untouched; the infinity norm covers every element; the CFL seed is
+ * 1e20).  Floating-point work is FP64 unless a name says otherwise and is
+ * compiled without multiply-add contraction: every SpMV row sums its
+ * products in stored order with separately rounded multiplies and adds,
+ * exactly as GIP_MatrixMulticore.cpp:64-103 does.  Reductions are
+ * deterministic (fixed grid, fixed combination order).
+ *
+ * All calls are ordered on the context's stream.  Calls that return a host
+ * scalar (gip_ddot, gip_dnrm_inf, gip_cfl) wait for it; the rest are
+ * asynchronous.  Device pointers are plain pointers from gip_malloc.
+ * Every entry returns GIP_OK (0) or a negative gip_status; the message of
+ * the last failure on the calling thread is gip_last_error().
+ */
+#ifndef GIP_HIP_H
+#define GIP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gip_status {
+    GIP_OK = 0,
+    GIP_EINVAL = -1,     /* bad argument or shape                          */
+    GIP_EHIP = -2,       /* HIP runtime error                              */
+    GIP_ENOMEM = -3,     /* device or host allocation failed               */
+    GIP_EBREAKDOWN = -4, /* PCG breakdown: p.Ap == 0 or a non-finite step  */
+    GIP_ENOCONV = -5     /* PCG stopped at maxit above tolerance           */
+} gip_status;
+
+typedef struct gip_ctx gip_ctx; /* one device + one stream + reduction scratch */
+typedef struct gip_mat gip_mat; /* a device-resident sparse operator           */
+typedef struct gip_pcg gip_pcg; /* a device-resident fused PCG solver          */
+
+const char *gip_last_error(void);
+
+/* ---- device and context ------------------------------------------------
+ * gip_init replaces GIP_cudaSetDevice (Headers/GIP_gpu.h:43) as called by
+ * GIP_ArchGPU::setUp (GIP_ArchGPU.cpp:19-26); the context owns a
+ * default-priority stream and the scratch of the reductions. */
+int gip_init(int device, gip_ctx **ctx);
+int gip_finalize(gip_ctx *ctx);
+int gip_sync(gip_ctx *ctx);
+void *gip_stream(gip_ctx *ctx); /* the hipStream_t the context launches on */
+int gip_device_count(int *count);
+
+/* ---- memory -------------------------------------------------------------
+ * GIP_cudaMallocDouble/Int (Headers/GIP_gpu.h:4,6), GIP_cudaFree (:8),
+ * GIP_cudaMemcpyDToGpu/IToGpu/IToCpu/DToCpu (:10-16), GIP_cudaDcopy (:20).
+ * Copies are stream-ordered; d2h waits for the copy. */
+int gip_malloc(gip_ctx *ctx, size_t bytes, void **dptr);
+int gip_free(gip_ctx *ctx, void *dptr);
+int gip_memcpy_h2d(gip_ctx *ctx, void *dst, const void *src, size_t bytes);
+int gip_memcpy_d2h(gip_ctx *ctx, void *dst, const void *src, size_t bytes);
+int gip_memcpy_d2d(gip_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* ---- events (GIP_cudaEvent* of Headers/GIP_gpu.h:32-41, GIP_TimerCuda.cpp) */
+typedef struct gip_event gip_event;
+int gip_event_create(gip_ctx *ctx, gip_event **ev);
+int gip_event_record(gip_ctx *ctx, gip_event *ev);
+int gip_event_elapsed_ms(gip_event *start, gip_event *stop, float *ms); /* waits for stop */
+int gip_event_destroy(gip_event *ev);
+
+/* ---- sparse operators ----------------------------------------------------
+ * Device copy of a CSR operator (GIP_MatrixCuda.cpp:14-53).  The host
+ * arrays are read once and may be freed afterwards.  The layout is chosen
+ * per operator: sliced ELL (64-row slices, column-major inside a slice)
+ * when padding stays small, a diagonal vector when the operator is
+ * diagonal, CSR otherwise.  `format` forces one of GIP_FMT_*.
+ * `ncols` is informational (columns may exceed it, as in the reference). */
+enum { GIP_FMT_AUTO = 0, GIP_FMT_CSR = 1, GIP_FMT_ELL = 2, GIP_FMT_DIAG = 3 };
+enum { GIP_MAT_FP32 = 1 << 8 }; /* also keep an FP32 copy of the values (mixed precision) */
+int gip_mat_create(gip_ctx *ctx, int32_t nrows, int32_t ncols, int32_t nnz, const int32_t *rowptr, const int32_t *col,
+                   const double *val, int32_t flags, gip_mat **A);
+int gip_mat_destroy(gip_mat *A);
+/* nrows, nnz, chosen format, stored (padded) entries */
+int gip_mat_info(const gip_mat *A, int32_t *nrows, int32_t *nnz, int32_t *format, int64_t *stored);
+/* Values in CSR order, device <-> host. */
+int gip_mat_get_values(const gip_mat *A, double *host_vals);
+int gip_mat_set_values(gip_mat *A, const double *host_vals);
+/* 1/a_ii per row into a device vector of nrows values, as
+ * GIP_MatrixMulticore::setUpDiagonal (GIP_MatrixMulticore.cpp:177-210);
+ * GIP_EINVAL if a row has no diagonal entry. */
+int gip_mat_diag_inv(const gip_mat *A, double *dinv);
+
+/* ---- SpMV ----------------------------------------------------------------
+ * gip_dcsrmv:      y_i = (Ax)_i for every non-empty row           GIP_cudaDCsrSpMV 8-arg (Headers/GIP_gpu.h:22),
+ *                                                                  semantics GIP_MatrixMulticore.cpp:64-83
+ * gip_dcsrmv_ab:   y_i = alpha*(Ax)_i + beta*y_i, non-empty rows   GIP_cudaDCsrSpMV 10-arg (:24), :85-103
+ * gip_dcsrmv_vals: B's values (CSR order) <- Ax, non-empty rows   GIP_MatrixMulticore.cpp:105-123
+ *                  (nrows(A) must equal nnz(B))
+ * gip_dcsrmv_t:    y = A^T x over all columns < ncols_out, y zeroed first (adjoint operators)
+ * gip_scsrmv:      FP32 y = A32 x32 (needs GIP_MAT_FP32)                                               */
+int gip_dcsrmv(gip_ctx *ctx, const gip_mat *A, const double *x, double *y);
+int gip_dcsrmv_ab(gip_ctx *ctx, const gip_mat *A, const double *x, double *y, double alpha, double beta);
+int gip_dcsrmv_vals(gip_ctx *ctx, const gip_mat *A, const double *x, gip_mat *B);
+int gip_dcsrmv_t(gip_ctx *ctx, const gip_mat *A, const double *x, double *y, int32_t ncols_out);
+int gip_scsrmv(gip_ctx *ctx, const gip_mat *A, const float *x, float *y);
+
+/* ---- vector operations ---------------------------------------------------
+ * GIP_cudaDaxpy 6/7-arg (Headers/GIP_gpu.h:26,28), GIP_cudaDdot (:30),
+ * GIP_gpuNormi (:46), GIP_gpuGetCFL (:48); semantics
+ * GIP_VectorMulticore.cpp:44-195, GIP_NonLinearMulticore.cpp:3-55. */
+int gip_dfill(gip_ctx *ctx, int32_t n, double *x, double value);                      /* x = value     */
+int gip_daxpy(gip_ctx *ctx, int32_t n, double alpha, const double *x, double *y);      /* y += alpha x  */
+int gip_daxpby(gip_ctx *ctx, int32_t n, double alpha, const double *x, double beta, double *y); /* y = ax+by */
+int gip_ddot(gip_ctx *ctx, int32_t n, const double *x, const double *y, double *result);         /* host result */
+int gip_dnrm_inf(gip_ctx *ctx, int32_t n, const double *x, double *result);                    /* max |x_i|  */
+int gip_cfl(gip_ctx *ctx, int32_t n, const double *u, const double *v, const double *w, const double *dxs,
+            double gamma, double rho, double *dt);
+/* Same reductions with the result left in device memory (no host wait). */
+int gip_ddot_dev(gip_ctx *ctx, int32_t n, const double *x, const double *y, double *dresult);
+int gip_dnrm_inf_dev(gip_ctx *ctx, int32_t n, const double *x, double *dresult);
+
+/* ---- fused Jacobi-PCG ----------------------------------------------------
+ * GIP_LinearSolver::solve (GIP_LinearSolver.h:45-117) with every scalar
+ * kept on the device: per iteration one SpMV+dot kernel and one
+ * update+precondition+dot kernel; the host only polls a convergence flag.
+ * precision GIP_PCG_FP64: the reference recurrences in FP64.
+ * precision GIP_PCG_MIXED: FP32 operator values, vectors and SpMV inside an
+ *   FP64 defect-correction loop: the FP64 true residual r = b - A x is
+ *   recomputed whenever the FP32 residual has dropped by `inner_tol`, the
+ *   correction is accumulated into the FP64 x, and convergence is decided
+ *   on the FP64 true residual only.  Dots and norms accumulate in FP64.
+ * tol > 0 stops once ||r||_2 <= tol ||b||_2 (FP64 recursive residual for
+ * FP64, FP64 true residual for MIXED); tol <= 0 runs maxit iterations.  */
+enum { GIP_PCG_FP64 = 0, GIP_PCG_MIXED = 1 };
+typedef struct gip_pcg_opts {
+    int32_t maxit;     /* iteration limit (inner iterations for MIXED)          */
+    double tol;        /* relative residual target, <= 0: fixed maxit           */
+    int32_t precond;   /* 1: Jacobi, 0: none                                    */
+    int32_t precision; /* GIP_PCG_FP64 or GIP_PCG_MIXED                         */
+    double inner_tol;  /* MIXED: residual drop that triggers an FP64 correction */
+    int32_t poll;      /* iterations queued between host polls (0: 16)          */
+} gip_pcg_opts;
+typedef struct gip_pcg_stats {
+    int32_t iters;       /* iterations done                                     */
+    int32_t corrections; /* FP64 true-residual corrections (MIXED)              */
+    double relres;       /* ||r||_2/||b||_2 at exit (recursive for FP64)        */
+    int32_t status;      /* GIP_OK, GIP_EBREAKDOWN or GIP_ENOCONV               */
+} gip_pcg_stats;
+int gip_pcg_create(gip_ctx *ctx, const gip_mat *A, int32_t n_all, gip_pcg **solver);
+/* Solves rows [0,nrows(A)) of A x = b; x0 is copied into x first; vectors are
+ * n_all long (entries past nrows(A) are carried over from x0). */
+int gip_pcg_solve(gip_pcg *solver, const double *b, double *x, const double *x0, const gip_pcg_opts *opts,
+                  gip_pcg_stats *stats);
+int gip_pcg_destroy(gip_pcg *solver);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,184 @@
+"""The hexahedral cavity generator against the operator relations the
+reference's shipped C100K operators satisfy, closed forms of the uniform
+grid, and file round trips through the reference's formats."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from gip_amd import cavity
+
+
+@pytest.fixture(scope="module", params=[(6, 0.0), (5, 1.5)], ids=["uniform6", "tanh5"])
+def cav(request):
+    n, beta = request.param
+    return cavity.generate(n, beta=beta, gamma=2e-2, pin=True)
+
+
+def S(c, name, ncols=None):
+    A = c.op(name)
+    return A.to_scipy(ncols or max(c.n_all, c.n_faces))
+
+
+def test_sizes(cav):
+    n = cav.nx
+    assert cav.n_inner == n ** 3 and cav.n_ghost == 6 * n * n and cav.n_faces == 3 * n * n * (n + 1)
+    assert cav.op("L").nnz == 7 * n ** 3 and cav.op("Ec").nnz == 12 * n ** 3
+    assert cav.op("Mres").nnz == 6 * n ** 3 and cav.op("Ec").nrows == cav.op("D").nnz
+
+
+def test_laplacian_is_minus_mres_so_except_the_pin(cav):
+    N, A, F = cav.n_inner, cav.n_all, cav.n_faces
+    L = S(cav, "L", A)
+    Mres = cav.op("Mres").to_scipy(F)
+    SO = cav.op("SO").to_scipy(A)
+    L0 = -(Mres @ SO)
+    D = (L - L0).tocoo()
+    big = np.abs(D.data) > 1e-12 * np.abs(L.data).max()
+    assert list(zip(D.row[big], D.col[big])) == [(0, 0)]  # the 1% pin on cell 0
+    assert L[0, 0] == pytest.approx(1.01 * L0[0, 0], rel=1e-14)
+    # symmetric on inner x inner, rows sum to zero except the pin, ghost columns hold zeros
+    Li = L[:, :N]
+    assert abs(Li - Li.T).max() <= 1e-12 * abs(Li).max()
+    rs = np.asarray(L.sum(axis=1)).ravel()
+    assert np.all(np.abs(rs[1:]) <= 1e-12 * abs(Li).max()) and rs[0] > 0
+    assert np.all(L[:, N:].data == 0)
+
+
+def test_divergence_is_minus_mres_sf_and_gradient_is_its_adjoint(cav):
+    N, A, F = cav.n_inner, cav.n_all, cav.n_faces
+    Mres = cav.op("Mres").to_scipy(F)
+    vol = cav.field("vol")
+    for ax in "xyz":
+        M = S(cav, "M" + ax, A)
+        SF = cav.op("SF" + ax).to_scipy(A)
+        assert abs(M + Mres @ SF).max() <= 1e-14 * abs(M).max()
+        G = S(cav, "G" + ax, A)
+        assert G[N:].nnz == 0  # ghost rows empty
+        GM = sp.diags(1.0 / vol) @ M[:, :N].T
+        assert abs(G[:N, :N] - GM).max() <= 1e-12 * abs(GM).max()
+        # the gradient of a constant vanishes; of a linear field it is exact on a uniform grid
+        one = np.ones(A)
+        assert np.abs(G @ one).max() <= 1e-10 * abs(G).max()
+
+
+def test_diffusion_rows_and_ghost_coefficients(cav):
+    N, A = cav.n_inner, cav.n_all
+    D = S(cav, "D", A)
+    L = S(cav, "L", A)
+    vol = cav.field("vol")
+    rs = np.asarray(D.sum(axis=1)).ravel()
+    assert np.abs(rs).max() <= 1e-10 * abs(D).max()
+    # interior couplings: D_ij = -(gamma/V_i) L_ij off the diagonal
+    Dc = D[:, :N].tocoo()
+    off = Dc.row != Dc.col
+    expect = -(2e-2 / vol[Dc.row[off]]) * np.asarray(L[Dc.row[off], Dc.col[off]]).ravel()
+    np.testing.assert_allclose(Dc.data[off], expect, rtol=1e-12)
+    # every ghost column of D carries a positive Dirichlet coefficient
+    assert np.all(D[:, N:].data > 0) and D[:, N:].nnz == cav.n_ghost
+
+
+def test_convection_rebuild_closed_form(cav):
+    """C = Ec.mf: C_ij = 0.5 Mres(i,f) mf_f / V_i across face f, C_ii = 0."""
+    N, A, F = cav.n_inner, cav.n_all, cav.n_faces
+    rng = np.random.default_rng(5)
+    mf = rng.standard_normal(F)
+    Ec = cav.op("Ec").to_scipy(F)
+    cvals = Ec @ mf
+    D = cav.op("D")
+    Mres = cav.op("Mres").to_scipy(F).tocsr()
+    vol = cav.field("vol")
+    for i in range(0, N, max(1, N // 37)):
+        cols = D.col[D.rowptr[i]:D.rowptr[i + 1]]
+        vals = cvals[D.rowptr[i]:D.rowptr[i + 1]]
+        assert vals[list(cols).index(i)] == 0.0
+        faces = Mres.indices[Mres.indptr[i]:Mres.indptr[i + 1]]
+        signs = Mres.data[Mres.indptr[i]:Mres.indptr[i + 1]]
+        expect = sorted(0.5 * s * mf[f] / vol[i] for f, s in zip(faces, signs))
+        got = sorted(v for c, v in zip(cols, vals) if c != i)
+        np.testing.assert_allclose(got, expect, rtol=1e-14)
+
+
+def test_convection_is_skew_when_fluxes_are_divergence_free():
+    """V.C is skew-symmetric on inner x inner when Mres.mf = 0 (energy-conserving convection)."""
+    c = cavity.generate(5, gamma=1e-2)
+    N, A, F = c.n_inner, c.n_all, c.n_faces
+    rng = np.random.default_rng(9)
+    phi = rng.standard_normal(A)
+    mf = c.op("SO").to_scipy(A) @ phi  # a gradient flux ...
+    L0 = -(c.op("Mres").to_scipy(F) @ c.op("SO").to_scipy(A))[:, :N]
+    # ... made divergence free by subtracting the flux of the Poisson solution
+    import scipy.sparse.linalg as spla
+    div = c.op("Mres").to_scipy(F) @ mf
+    Lp = L0.tolil()
+    Lp[0, 0] = Lp[0, 0] * 1.01
+    psi = spla.spsolve(Lp.tocsc(), div)
+    full = np.zeros(A)
+    full[:N] = psi
+    mf = mf + c.op("SO").to_scipy(A) @ full
+    assert np.abs(c.op("Mres").to_scipy(F) @ mf).max() < 1e-9 * np.abs(mf).max() + 1e-12
+    D = c.op("D")
+    Cv = c.op("Ec").to_scipy(F) @ mf
+    C = sp.csr_matrix((Cv, D.col, D.rowptr), shape=(N, A))[:, :N]
+    VC = sp.diags(c.field("vol")) @ C
+    assert abs(VC + VC.T).max() <= 1e-8 * abs(VC).max()
+
+
+def test_lb_maps_ghosts_to_adjacent_cells(cav):
+    N, A = cav.n_inner, cav.n_all
+    Lb = cav.op("Lb")
+    assert np.all(np.diff(Lb.rowptr)[:N] == 0) and np.all(np.diff(Lb.rowptr)[N:] == 1)
+    assert np.all(Lb.val == 1.0)
+    # the adjacent cell of each ghost is the one whose D row couples to it
+    D = S(cav, "D", A).tocsc()
+    for g in range(N, A, max(1, (A - N) // 50)):
+        rows = D.indices[D.indptr[g]:D.indptr[g + 1]]
+        assert list(rows) == [Lb.col[Lb.rowptr[g]]]
+
+
+def test_fields_and_cfl_length(cav):
+    u = cav.field("u")
+    N = cav.n_inner
+    assert np.all(u[:N] == 0) and np.count_nonzero(u[N:]) == cav.nx * cav.ny
+    assert np.all(u[cav.field("lid_ghost")] == 1.0)
+    dxs = cav.field("dxs")
+    assert np.all(dxs[:N] > 0) and np.all(dxs[N:] == 0)
+
+
+def test_write_read_round_trip(tmp_path):
+    c = cavity.generate(4, gamma=1e-2)
+    c.write(tmp_path, "rt")
+    r = cavity.read(tmp_path, "rt")
+    assert (r.n_inner, r.n_ghost, r.n_faces) == (c.n_inner, c.n_ghost, c.n_faces)
+    for op in cavity.OPS:
+        a, b = c.op(op), r.op(op)
+        assert np.array_equal(a.rowptr, b.rowptr) and np.array_equal(a.col, b.col) and np.array_equal(a.val, b.val), op
+    for f in ("u", "v", "w", "dxs"):
+        assert np.array_equal(c.field(f), r.field(f))
+
+
+def test_merge_c100k_sizes(c100k_root):
+    r = cavity.read(c100k_root, "C100K")
+    assert (r.n_inner, r.n_ghost, r.n_faces) == (97521, 5750, 200939)
+    assert r.op("L").nnz == 5 * 97521
+
+
+def test_errors_are_reported(tmp_path):
+    with pytest.raises(cavity.CavityError, match="cannot open"):
+        cavity.read(tmp_path, "missing")
+    with pytest.raises(cavity.CavityError, match="at least 2"):
+        cavity.generate(1)
+    # a truncated operator file is rejected
+    c = cavity.generate(3, gamma=1e-2)
+    c.write(tmp_path, "tr")
+    f = tmp_path / "InputFiles/Operators/L/tr/tr_1p/Ltr_1p-0.csr"
+    f.write_bytes(f.read_bytes()[:-9])
+    with pytest.raises(cavity.CavityError, match="truncated"):
+        cavity.read(tmp_path, "tr")
+
+
+def test_rhs_normal_is_seeded_and_zero_mean():
+    a = cavity.rhs_normal(100000, 42)
+    b = cavity.rhs_normal(100000, 42)
+    assert np.array_equal(a, b)
+    assert abs(a.mean()) < 1e-15 and abs(a.std() - 1.0) < 0.01
+    assert not np.array_equal(a, cavity.rhs_normal(100000, 43))

@@ -1,0 +1,144 @@
+"""The templated integrator with the HIP specification (GIP_DNS over
+GIP_MatrixHIP/GIP_VectorHIP/GIP_ArchHIP/GIP_NonLinearHIP, through
+include/gip_dns.h) against the oracle on identical operators.
+
+Bar (north star): per-step massr, dt, time and the u, v, w, p fields within
+1e-10 relative of the CPU path in FP64; tolerance-mode PCG iteration
+counts within +-1; the README golden lines for C100K.
+"""
+import json
+import subprocess
+
+import numpy as np
+import pytest
+
+from gip_amd import cavity, dns
+from oracle import pyoracle
+
+from .conftest import GOLDEN, ROOT
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-10
+
+
+def _rel(a, b):
+    return abs(a - b) / abs(b) if b != 0 else abs(a)
+
+
+def _field_rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / nb if nb else np.linalg.norm(a)
+
+
+def _compare_run(sim, orc, steps, tol_iters=None):
+    for k in range(steps):
+        g = sim.step()
+        w = orc.step()
+        print(f"[step {k + 1}] gpu massr {g['massr']:.9e} dt {g['dt']:.9e} pcg {g['pcg_iters']} | "
+              f"cpu massr {w['massr']:.9e} dt {w['dt']:.9e} pcg {w['pcg_iters']}", flush=True)
+        for key in ("time", "dt", "massr"):
+            assert _rel(g[key], w[key]) <= REL, (k, key, g[key], w[key])
+        if tol_iters is None:
+            assert g["pcg_iters"] == w["pcg_iters"]
+        else:
+            assert abs(g["pcg_iters"] - w["pcg_iters"]) <= tol_iters, (k, g["pcg_iters"], w["pcg_iters"])
+
+
+def _oracle_fields(orc):
+    return {f: orc.field(f) for f in ("u", "v", "w", "p", "mf")}
+
+
+def _compare_fields(sim, orc):
+    want = _oracle_fields(orc)
+    for f in ("u", "v", "w", "p", "mf"):
+        err = _field_rel(sim.field(f), want[f])
+        print(f"[field {f}] rel L2 {err:.3e}", flush=True)
+        assert err <= REL, (f, err)
+
+
+def test_dns_cav46_reference_sequence_vs_oracle(cav46_dir):
+    sim = dns.Simulation(root=cav46_dir, name="cav46", gamma=1e-2)
+    orc = pyoracle.Dns(cav46_dir, "cav46", gamma=1e-2)
+    _compare_run(sim, orc, 10)
+    _compare_fields(sim, orc)
+
+
+def test_dns_cav46_matches_reference_binary_fixture(cav46_dir):
+    ref = json.loads((GOLDEN / "ref_cav46_1p.json").read_text())
+    sim = dns.Simulation(root=cav46_dir, name="cav46", gamma=1e-2)
+    for w in ref["steps_out"]:
+        g = sim.step()
+        for key in ("time", "dt", "massr"):
+            assert _rel(g[key], w[key]) <= REL, (key, g[key], w[key])
+    for f in ("u", "v", "w", "p"):
+        vals = sim.field(f)
+        summ = ref["fields"][f]
+        assert abs(np.linalg.norm(vals) - summ["l2"]) <= REL * summ["l2"]
+        np.testing.assert_allclose(vals[summ["sample_index"]], summ["sample"], rtol=0,
+                                   atol=REL * summ["max_abs"] * 10)
+
+
+def test_dns_c100k_readme_and_oracle(c100k_root):
+    sim = dns.Simulation(root=c100k_root, name="C100K", gamma=1e-3)
+    orc = pyoracle.Dns(c100k_root, "C100K", gamma=1e-3)
+    readme = pyoracle.parse_lines((GOLDEN / "readme_c100k_np4.txt").read_text())
+    for w in readme:
+        g = sim.step()
+        o = orc.step()
+        for key in ("time", "dt", "massr"):
+            assert f"{g[key]:.6e}" == f"{w[key]:.6e}", (key, g[key], w[key])
+            assert _rel(g[key], o[key]) <= REL
+    _compare_fields(sim, orc)
+
+
+def test_dns_fused_pcg_matches_reference_sequence(cav46_dir):
+    sim = dns.Simulation(root=cav46_dir, name="cav46", gamma=1e-2, pcg_mode="fused")
+    orc = pyoracle.Dns(cav46_dir, "cav46", gamma=1e-2)
+    _compare_run(sim, orc, 10)
+    _compare_fields(sim, orc)
+
+
+@pytest.mark.parametrize("mode", ["reference", "fused"])
+def test_dns_tolerance_mode_iteration_counts(cav46_dir, mode):
+    sim = dns.Simulation(root=cav46_dir, name="cav46", gamma=1e-2, pcg_mode=mode, pcg_tol=1e-8, pcg_maxit=2000)
+    orc = pyoracle.Dns(cav46_dir, "cav46", gamma=1e-2, pcg_maxit=2000, pcg_tol=1e-8)
+    _compare_run(sim, orc, 10, tol_iters=1)
+
+
+def test_dns_mixed_precision_error_per_field(cav46_dir):
+    """Mixed-precision PCG (FP32 inner, FP64 defect correction) to the same
+    tolerance as the FP64 run: the error per field against the FP64 oracle
+    is bounded by the solver tolerance, not by FP32 round-off."""
+    sim = dns.Simulation(root=cav46_dir, name="cav46", gamma=1e-2, pcg_mode="mixed", pcg_tol=1e-10, pcg_maxit=5000)
+    orc = pyoracle.Dns(cav46_dir, "cav46", gamma=1e-2, pcg_maxit=5000, pcg_tol=1e-10)
+    for k in range(10):
+        g, w = sim.step(), orc.step()
+        assert g["pcg_status"] == 0
+        assert _rel(g["massr"], w["massr"]) <= 1e-6
+    want = _oracle_fields(orc)
+    for f in ("u", "v", "w", "p"):
+        err = _field_rel(sim.field(f), want[f])
+        print(f"[mixed field {f}] rel L2 vs FP64 oracle {err:.3e}", flush=True)
+        assert err <= 1e-7, (f, err)
+
+
+def test_dns_in_memory_case_equals_file_case(cav46_dir):
+    a = dns.Simulation(root=cav46_dir, name="cav46", gamma=1e-2)
+    b = dns.Simulation(cavity=cavity.generate(46, gamma=1e-2), gamma=1e-2)
+    for _ in range(3):
+        sa, sb = a.step(), b.step()
+        assert (sa["time"], sa["dt"], sa["massr"]) == (sb["time"], sb["dt"], sb["massr"])
+    for f in ("u", "p"):
+        assert np.array_equal(a.field(f), b.field(f))
+
+
+def test_drivencavityhip_executable(cav46_dir):
+    out = subprocess.run([str(ROOT / "gip_amd/bin/DrivenCavityHIP"), "cav46", "--root", str(cav46_dir), "--steps",
+                          "4", "--gamma", "0.01", "--full"], capture_output=True, text=True, check=True, timeout=120)
+    got = pyoracle.parse_lines(out.stdout)
+    want = pyoracle.run_cli(cav46_dir, "cav46", 4, 1e-2)
+    assert len(got) == 4
+    for g, w in zip(got, want):
+        for key in ("time", "dt", "massr"):
+            assert _rel(g[key], w[key]) <= REL

@@ -1,0 +1,242 @@
+"""gfx950 kernels through the C ABI against the oracle (C restatement of the
+reference CPU path) on the same inputs.
+
+Bar: SpMV (every layout), axpy, axpby, infinity norm and CFL are bitwise
+equal to the oracle (same summation order, no contraction); dot products
+differ only by summation order (1e-13 relative); the fused FP64 PCG agrees
+with the oracle's PCG to 1e-10 relative and its tolerance-mode iteration
+counts to +-1; the mixed-precision PCG reaches a 1e-10 FP64 true residual.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from gip_amd import cavity, hip
+from oracle import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = hip.Device(0)
+    yield d
+    d.close()
+
+
+@pytest.fixture(scope="module")
+def cav46():
+    return cavity.generate(46, gamma=1e-2)
+
+
+@pytest.fixture(scope="module")
+def c100k(c100k_root):
+    return cavity.read(c100k_root, "C100K")
+
+
+def _ncols(c, A):
+    return max(c.n_all, c.n_faces, int(A.col.max()) + 1 if A.nnz else 1)
+
+
+def _check_op(dev, c, name, fmt=hip.FMT_AUTO, seed=0):
+    A = c.op(name)
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal(_ncols(c, A))
+    y0 = rng.standard_normal(A.nrows)
+    M = hip.Matrix.from_csr(dev, A, fmt=fmt)
+    dx, dy = dev.to_device(x), dev.to_device(y0)
+    M.spmv(dx, dy)
+    got = dy.download()
+    want = y0.copy()
+    pyoracle.spmv(pyoracle.Csr.of(A), x, want)
+    assert np.array_equal(got, want), (name, fmt, np.abs(got - want).max())
+    dy.upload(y0)
+    M.spmv_ab(dx, dy, 0.7, -1.3)
+    got = dy.download()
+    want = y0.copy()
+    pyoracle.spmv_ab(pyoracle.Csr.of(A), x, want, 0.7, -1.3)
+    assert np.array_equal(got, want), (name, fmt, "ab")
+    return M.info()
+
+
+@pytest.mark.parametrize("name", cavity.OPS)
+def test_spmv_bitwise_cav46(dev, cav46, name):
+    print(f"[spmv cav46 {name}]", flush=True)
+    _check_op(dev, cav46, name)
+
+
+@pytest.mark.parametrize("name", cavity.OPS)
+def test_spmv_bitwise_c100k(dev, c100k, name):
+    print(f"[spmv c100k {name}]", flush=True)
+    _check_op(dev, c100k, name)
+
+
+@pytest.mark.parametrize("fmt", [hip.FMT_CSR, hip.FMT_ELL])
+def test_spmv_forced_layouts(dev, cav46, fmt):
+    for name in ("L", "Ec", "SO", "Gx", "Lb"):
+        info = _check_op(dev, cav46, name, fmt=fmt)
+        assert info["format"] == fmt
+
+
+def test_layout_choice(dev, cav46):
+    assert hip.Matrix.from_csr(dev, cav46.op("L")).info()["format"] == hip.FMT_ELL
+    A = cav46.op("L")
+    n = A.nrows
+    D = hip.Matrix(dev, np.arange(n + 1), np.arange(n), np.ones(n))
+    assert D.info()["format"] == hip.FMT_DIAG
+
+
+def test_spmv_random_with_empty_rows(dev):
+    rng = np.random.default_rng(4)
+    A = sp.random(1000, 777, density=0.01, random_state=rng, format="lil")
+    for r in (0, 5, 63, 64, 999):
+        A.rows[r], A.data[r] = [], []
+    A = A.tocsr()
+    A.sort_indices()
+    A.data = rng.standard_normal(A.nnz)
+    x = rng.standard_normal(777)
+    y0 = rng.standard_normal(1000)
+    for fmt in (hip.FMT_AUTO, hip.FMT_CSR, hip.FMT_ELL):
+        M = hip.Matrix(dev, A.indptr, A.indices, A.data, ncols=777, fmt=fmt)
+        dy = dev.to_device(y0)
+        M.spmv(dev.to_device(x), dy)
+        want = y0.copy()
+        pyoracle.spmv(pyoracle.Csr(A.indptr, A.indices, A.data), x, want)
+        assert np.array_equal(dy.download(), want), fmt
+
+
+def test_spmv_into_values_rebuilds_convection(dev, cav46):
+    """Ec.mf written into C's values (GIP_MatrixMulticore.cpp:105-123)."""
+    Ec, D = cav46.op("Ec"), cav46.op("D")
+    mf = np.random.default_rng(1).standard_normal(cav46.n_faces)
+    E = hip.Matrix.from_csr(dev, Ec)
+    C = hip.Matrix.from_csr(dev, D)
+    E.spmv_vals(dev.to_device(mf), C)
+    want = D.val.copy()
+    pyoracle.spmv(pyoracle.Csr.of(Ec), mf, want)
+    assert np.array_equal(C.values(), want)
+
+
+def test_transposed_spmv(dev, cav46):
+    for name in ("Mx", "SO", "Ec"):
+        A = cav46.op(name)
+        nc = _ncols(cav46, A)
+        x = np.random.default_rng(2).standard_normal(A.nrows)
+        M = hip.Matrix.from_csr(dev, A)
+        dy = dev.array(nc)
+        M.spmv_t(dev.to_device(x), dy, nc)
+        want = A.to_scipy(nc).T @ x
+        np.testing.assert_allclose(dy.download(), want, rtol=1e-13, atol=1e-13 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("n", [0, 1, 63, 64, 65, 1000, 1 << 20])
+def test_vector_ops(dev, n):
+    rng = np.random.default_rng(n)
+    x, y = rng.standard_normal(n), rng.standard_normal(n)
+    dx, dy = dev.to_device(x), dev.to_device(y)
+    hip.axpy(0.37, dx, dy)
+    assert np.array_equal(dy.download(), y + 0.37 * x)
+    dy.upload(y)
+    hip.axpby(-1.25, dx, 0.5, dy)
+    assert np.array_equal(dy.download(), -1.25 * x + 0.5 * y)
+    d = hip.dot(dx, dev.to_device(y))
+    want = pyoracle.dot(x, y)
+    assert abs(d - want) <= 1e-13 * max(1.0, np.abs(x * y).sum())
+    assert hip.nrm_inf(dx) == (np.abs(x).max() if n else 0.0)
+    hip.fill(dy, 3.5)
+    assert np.all(dy.download() == 3.5)
+
+
+def test_norm_ignores_nan_like_the_cpu(dev):
+    a = np.array([1.0, np.nan, -4.0, 2.0])
+    assert hip.nrm_inf(dev.to_device(a)) == pyoracle.norm_inf(a) == 4.0
+
+
+def test_cfl_bitwise(dev, cav46):
+    rng = np.random.default_rng(6)
+    n = cav46.n_inner
+    u, v, w = (rng.standard_normal(cav46.n_all) for _ in range(3))
+    u[:100] = 0.0
+    v[:100] = 0.0
+    w[:100] = 0.0
+    dxs = cav46.field("dxs").copy()
+    d = [dev.to_device(a) for a in (u, v, w, dxs)]
+    for gamma in (1e-2, 1e-3, 1e3):
+        got = hip.cfl(*d, n, gamma, 1.0)
+        assert got == pyoracle.cfl(u, v, w, dxs, n, gamma, 1.0)
+
+
+def _pcg_setup(dev, c, seed=11):
+    L = c.op("L")
+    n, nall = c.n_inner, c.n_all
+    b = np.zeros(nall)
+    b[:n] = cavity.rhs_normal(n, seed)
+    A = hip.Matrix.from_csr(dev, L, fp32=True)
+    return L, n, nall, b, A
+
+
+def test_pcg_fixed_iterations_vs_oracle(dev, cav46):
+    L, n, nall, b, A = _pcg_setup(dev, cav46)
+    x0 = np.zeros(nall)
+    x0[:n] = 0.01 * np.random.default_rng(3).standard_normal(n)
+    solver = hip.PCG(A, nall)
+    dx = dev.array(nall)
+    r = solver.solve(dev.to_device(b), dx, dev.to_device(x0), maxit=100)
+    want, st, it, rel = pyoracle.pcg(pyoracle.Csr.of(L), b, x0, n, 100)
+    got = dx.download()
+    assert r.iters == it == 100 and r.status == 0
+    assert np.linalg.norm(got - want) <= 1e-10 * np.linalg.norm(want)
+    assert np.array_equal(got[n:], x0[n:])
+    assert abs(r.relres - rel) <= 1e-8 * rel
+
+
+@pytest.mark.parametrize("tol", [1e-6, 1e-8, 1e-10])
+def test_pcg_tolerance_iteration_count_within_one(dev, cav46, tol):
+    L, n, nall, b, A = _pcg_setup(dev, cav46)
+    x0 = np.zeros(nall)
+    solver = hip.PCG(A, nall)
+    dx = dev.array(nall)
+    r = solver.solve(dev.to_device(b), dx, dev.to_device(x0), maxit=2000, tol=tol)
+    want, st, it, rel = pyoracle.pcg(pyoracle.Csr.of(L), b, x0, n, 2000, tol=tol)
+    assert st == 0 and r.status == 0
+    assert abs(r.iters - it) <= 1, (r.iters, it)
+    assert r.relres <= tol
+    assert np.linalg.norm(dx.download() - want) <= 1e-6 * np.linalg.norm(want)
+
+
+def test_pcg_mixed_precision_reaches_fp64_residual(dev, cav46):
+    L, n, nall, b, A = _pcg_setup(dev, cav46)
+    x0 = np.zeros(nall)
+    solver = hip.PCG(A, nall)
+    dx = dev.array(nall)
+    r = solver.solve(dev.to_device(b), dx, dev.to_device(x0), maxit=5000, tol=1e-10, precision=hip.PCG_MIXED)
+    x = dx.download()
+    Ax = np.zeros(nall)
+    pyoracle.spmv(pyoracle.Csr.of(L), x, Ax)
+    true_rel = np.linalg.norm(b[:n] - Ax[:n]) / np.linalg.norm(b[:n])
+    assert r.status == 0 and r.corrections >= 1
+    assert true_rel <= 1e-10 * 1.01, true_rel
+    want, st, it, rel = pyoracle.pcg(pyoracle.Csr.of(L), b, x0, n, 5000, tol=1e-12)
+    assert np.linalg.norm(x - want) <= 1e-7 * np.linalg.norm(want)
+    print(f"[mixed] iters {r.iters} corrections {r.corrections} true {true_rel:.3e} fp64 iters {it}")
+
+
+def test_pcg_breakdown_is_reported(dev, cav46):
+    L, n, nall, b, A = _pcg_setup(dev, cav46)
+    z = dev.array(nall)
+    hip.fill(z, 0.0)
+    solver = hip.PCG(A, nall)
+    dx = dev.array(nall)
+    with pytest.raises(hip.HipError) as e:
+        solver.solve(z, dx, z, maxit=10)
+    assert e.value.code == hip.EBREAKDOWN
+    r = solver.solve(z, dx, z, maxit=10, tol=1e-8)
+    assert r.iters == 0 and r.status == 0
+
+
+def test_invalid_arguments_are_rejected(dev):
+    with pytest.raises(hip.HipError) as e:
+        hip.Matrix(dev, np.array([0, 2]), np.array([0, -1]), np.array([1.0, 2.0]))
+    assert e.value.code == hip.EINVAL
+    with pytest.raises(hip.HipError):
+        hip.Matrix(dev, np.array([0, 3]), np.array([0, 1]), np.array([1.0, 2.0]))
