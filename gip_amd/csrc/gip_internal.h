@@ -47,6 +47,7 @@ struct gip_ctx {
 struct gip_mat {
     gip_ctx *ctx;
     int32_t nrows, ncols, nnz;
+    int32_t cols_needed;  // largest column index + 1
     int32_t format;
     // CSR (kept for every format: value get/set, transposed products, diagonal)
     int32_t *d_rowptr;

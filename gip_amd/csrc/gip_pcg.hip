@@ -127,7 +127,10 @@ struct PcgDev {
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_setup_resid(EllView A, PcgDev d, const double *__restrict__ x0, int n_all) {
     const int stride = gridDim.x * kBlock;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n_all; i += stride) d.x[i] = x0[i];
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n_all; i += stride) {
+        d.x[i] = x0[i];
+        if (i >= d.n) d.p[i] = x0[i];  // the direction keeps x0 outside the solved rows (:55-56)
+    }
     // A reads x0 directly so no grid-wide ordering with the copy is needed
     FOR_ROWS(d.n, row) {
         double sum;
@@ -429,6 +432,7 @@ extern "C" {
 
 int gip_pcg_create(gip_ctx *c, const gip_mat *A, int32_t n_all, gip_pcg **out) {
     if (!c || !A || !out || n_all < A->nrows) return set_error(GIP_EINVAL, "gip_pcg_create: bad argument");
+    if (n_all < A->cols_needed) return set_error(GIP_EINVAL, "gip_pcg_create: vectors shorter than the operator's columns");
     if (A->format != GIP_FMT_ELL && A->format != GIP_FMT_CSR)
         return set_error(GIP_EINVAL, "gip_pcg_create: operator must be ELL or CSR");
     gip_pcg *S = new (std::nothrow) gip_pcg();
@@ -439,12 +443,13 @@ int gip_pcg_create(gip_ctx *c, const gip_mat *A, int32_t n_all, gip_pcg **out) {
     S->n_all = n_all;
     S->grid = pcg_grid(A->nrows);
     size_t n = (size_t)std::max(1, A->nrows);
+    size_t na = (size_t)std::max(1, n_all);  // SpMV inputs also cover the ghost columns
     hipError_t e = hipSetDevice(c->device);
     auto dalloc = [&](void **p, size_t bytes) {
         if (e == hipSuccess) e = hipMalloc(p, bytes);
     };
     dalloc((void **)&S->r, n * 8);
-    dalloc((void **)&S->p, n * 8);
+    dalloc((void **)&S->p, na * 8);
     dalloc((void **)&S->s, n * 8);
     dalloc((void **)&S->Ap, n * 8);
     dalloc((void **)&S->dinv, n * 8);
@@ -493,8 +498,12 @@ static int ensure_mixed(gip_pcg *S) {
     if ((A->format == GIP_FMT_ELL && !A->d_ell_val32) || (A->format == GIP_FMT_CSR && !A->d_val32))
         return set_error(GIP_EINVAL, "mixed precision needs an operator created with GIP_MAT_FP32");
     size_t n = (size_t)std::max(1, S->n);
-    float **bufs[] = {&S->r32, &S->p32, &S->s32, &S->Ap32, &S->xc32, &S->dinv32};
+    float **bufs[] = {&S->r32, &S->s32, &S->Ap32, &S->xc32, &S->dinv32};
     for (float **b : bufs) GIP_HIP_TRY(hipMalloc(b, n * 4));
+    // the FP32 direction is an SpMV input: ghost columns included, and zero there
+    // (the correction leaves the ghost values of x unchanged)
+    GIP_HIP_TRY(hipMalloc(&S->p32, (size_t)std::max(1, S->n_all) * 4));
+    GIP_HIP_TRY(hipMemsetAsync(S->p32, 0, (size_t)std::max(1, S->n_all) * 4, S->ctx->stream));
     if (S->n) {
         k_to32<<<(S->n + kBlock - 1) / kBlock, kBlock, 0, S->ctx->stream>>>(S->n, S->dinv, S->dinv32);
         GIP_LAUNCH_CHECK("dinv32");

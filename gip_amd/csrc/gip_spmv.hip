@@ -245,8 +245,11 @@ int gip_mat_create(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, const 
     if (rowptr[0] != 0 || rowptr[nrows] != nnz) return set_error(GIP_EINVAL, "gip_mat_create: rowptr does not span nnz");
     for (int32_t i = 0; i < nrows; ++i)
         if (rowptr[i + 1] < rowptr[i]) return set_error(GIP_EINVAL, "gip_mat_create: rowptr not monotone");
-    for (int32_t j = 0; j < nnz; ++j)
+    int32_t maxc = -1;
+    for (int32_t j = 0; j < nnz; ++j) {
         if (col[j] < 0) return set_error(GIP_EINVAL, "gip_mat_create: negative column index");
+        maxc = std::max(maxc, col[j]);
+    }
     GIP_HIP_TRY(hipSetDevice(c->device));
     gip_mat *A = new (std::nothrow) gip_mat();
     if (!A) return set_error(GIP_ENOMEM, "gip_mat_create: allocation failed");
@@ -254,6 +257,7 @@ int gip_mat_create(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, const 
     A->nrows = nrows;
     A->ncols = ncols;
     A->nnz = nnz;
+    A->cols_needed = maxc + 1;
     const int32_t want = flags & 0xff;
     const bool fp32 = (flags & GIP_MAT_FP32) != 0;
 
@@ -375,6 +379,12 @@ int gip_mat_info(const gip_mat *A, int32_t *nrows, int32_t *nnz, int32_t *format
     if (nnz) *nnz = A->nnz;
     if (format) *format = A->format;
     if (stored) *stored = A->ell_stored;
+    return GIP_OK;
+}
+
+int gip_mat_cols_needed(const gip_mat *A, int32_t *ncols) {
+    if (!A || !ncols) return set_error(GIP_EINVAL, "null argument");
+    *ncols = A->cols_needed;
     return GIP_OK;
 }
 

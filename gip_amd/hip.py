@@ -7,6 +7,7 @@ PCG.  Every call goes through libgip_hip.so; there is no other code path.
 from __future__ import annotations
 
 import ctypes
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -49,6 +50,7 @@ SIGNATURES = {
                         POINTER(c_void_p)], c_int),
     "gip_mat_destroy": ([c_void_p], c_int),
     "gip_mat_info": ([c_void_p, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32), POINTER(c_int64)], c_int),
+    "gip_mat_cols_needed": ([c_void_p, POINTER(c_int32)], c_int),
     "gip_mat_get_values": ([c_void_p, c_void_p], c_int),
     "gip_mat_set_values": ([c_void_p, c_void_p], c_int),
     "gip_mat_diag_inv": ([c_void_p, c_void_p], c_int),
@@ -98,10 +100,17 @@ def check(rc):
 class Device:
     def __init__(self, device: int = 0):
         self.ctx = c_void_p()
+        self._children = weakref.WeakSet()
         check(lib().gip_init(device, byref(self.ctx)))
 
+    def _adopt(self, obj):
+        self._children.add(obj)
+
     def close(self):
+        """Release every live buffer, operator and solver, then the context."""
         if self.ctx:
+            for ch in list(self._children):
+                ch._release()
             lib().gip_finalize(self.ctx)
             self.ctx = c_void_p()
 
@@ -129,6 +138,7 @@ class DeviceArray:
         self.dev, self.n, self.dtype = dev, int(n), np.dtype(dtype)
         self.ptr = c_void_p()
         check(lib().gip_malloc(dev.ctx, max(1, self.n) * self.dtype.itemsize, byref(self.ptr)))
+        dev._adopt(self)
 
     @property
     def nbytes(self):
@@ -146,9 +156,11 @@ class DeviceArray:
         return out
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self.dev.ctx:
             lib().gip_free(self.dev.ctx, self.ptr)
-            self.ptr = c_void_p()
+        self.ptr = c_void_p()
+
+    _release = free
 
     def __del__(self):
         try:
@@ -171,6 +183,7 @@ class Matrix:
         flags = fmt | (MAT_FP32 if fp32 else 0)
         check(lib().gip_mat_create(dev.ctx, self.nrows, int(ncols), self.nnz, rowptr.ctypes.data, col.ctypes.data,
                                    val.ctypes.data, flags, byref(self.ptr)))
+        dev._adopt(self)
 
     @classmethod
     def from_csr(cls, dev, csr, **kw):
@@ -190,28 +203,54 @@ class Matrix:
         vals = np.ascontiguousarray(vals, dtype=np.float64)
         check(lib().gip_mat_set_values(self.ptr, vals.ctypes.data))
 
+    @property
+    def cols_needed(self):
+        n = c_int32()
+        check(lib().gip_mat_cols_needed(self.ptr, byref(n)))
+        return n.value
+
+    def _shapes(self, x, y, dtype=np.float64):
+        # host-side bounds check before any launch: the kernels trust these lengths
+        if x.dtype != dtype or y.dtype != dtype:
+            raise ValueError("vector dtype mismatch")
+        if x.n < self.cols_needed:
+            raise ValueError(f"input has {x.n} entries, operator reads up to column {self.cols_needed - 1}")
+        if y.n < self.nrows:
+            raise ValueError(f"output has {y.n} entries, operator has {self.nrows} rows")
+
     def spmv(self, x: DeviceArray, y: DeviceArray):
+        self._shapes(x, y)
         check(lib().gip_dcsrmv(self.dev.ctx, self.ptr, x.ptr, y.ptr))
 
     def spmv_ab(self, x, y, alpha, beta):
+        self._shapes(x, y)
         check(lib().gip_dcsrmv_ab(self.dev.ctx, self.ptr, x.ptr, y.ptr, alpha, beta))
 
     def spmv_vals(self, x, B: "Matrix"):
+        if x.n < self.cols_needed or self.nrows != B.nnz:
+            raise ValueError("spmv_vals: input too short or rows(A) != nnz(B)")
         check(lib().gip_dcsrmv_vals(self.dev.ctx, self.ptr, x.ptr, B.ptr))
 
     def spmv_t(self, x, y, ncols_out):
+        if x.n < self.nrows or y.n < ncols_out or ncols_out < self.cols_needed:
+            raise ValueError("spmv_t: vector lengths do not cover the operator")
         check(lib().gip_dcsrmv_t(self.dev.ctx, self.ptr, x.ptr, y.ptr, ncols_out))
 
     def spmv32(self, x, y):
+        self._shapes(x, y, np.float32)
         check(lib().gip_scsrmv(self.dev.ctx, self.ptr, x.ptr, y.ptr))
 
     def diag_inv(self, out: DeviceArray):
         check(lib().gip_mat_diag_inv(self.ptr, out.ptr))
 
     def destroy(self):
-        if self.ptr:
+        for s in list(getattr(self, "_solvers", [])):
+            s.destroy()
+        if self.ptr and self.dev.ctx:
             lib().gip_mat_destroy(self.ptr)
-            self.ptr = c_void_p()
+        self.ptr = c_void_p()
+
+    _release = destroy
 
     def __del__(self):
         try:
@@ -220,31 +259,45 @@ class Matrix:
             pass
 
 
+def _span(n, *arrays):
+    """Host-side bounds check: every vector covers the n entries a kernel touches."""
+    for a in arrays:
+        if a.n < n or a.dtype != np.float64:
+            raise ValueError(f"vector of {a.n} {a.dtype} entries used over {n} FP64 entries")
+    return n
+
+
 def fill(x: DeviceArray, value, n=None):
-    check(lib().gip_dfill(x.dev.ctx, x.n if n is None else n, x.ptr, value))
+    n = _span(x.n if n is None else n, x)
+    check(lib().gip_dfill(x.dev.ctx, n, x.ptr, value))
 
 
 def axpy(alpha, x, y, n=None):
-    check(lib().gip_daxpy(y.dev.ctx, y.n if n is None else n, alpha, x.ptr, y.ptr))
+    n = _span(y.n if n is None else n, x, y)
+    check(lib().gip_daxpy(y.dev.ctx, n, alpha, x.ptr, y.ptr))
 
 
 def axpby(alpha, x, beta, y, n=None):
-    check(lib().gip_daxpby(y.dev.ctx, y.n if n is None else n, alpha, x.ptr, beta, y.ptr))
+    n = _span(y.n if n is None else n, x, y)
+    check(lib().gip_daxpby(y.dev.ctx, n, alpha, x.ptr, beta, y.ptr))
 
 
 def dot(x, y, n=None):
+    n = _span(x.n if n is None else n, x, y)
     r = c_double()
-    check(lib().gip_ddot(x.dev.ctx, x.n if n is None else n, x.ptr, y.ptr, byref(r)))
+    check(lib().gip_ddot(x.dev.ctx, n, x.ptr, y.ptr, byref(r)))
     return r.value
 
 
 def nrm_inf(x, n=None):
+    n = _span(x.n if n is None else n, x)
     r = c_double()
-    check(lib().gip_dnrm_inf(x.dev.ctx, x.n if n is None else n, x.ptr, byref(r)))
+    check(lib().gip_dnrm_inf(x.dev.ctx, n, x.ptr, byref(r)))
     return r.value
 
 
 def cfl(u, v, w, dxs, n, gamma, rho):
+    n = _span(n, u, v, w, dxs)
     r = c_double()
     check(lib().gip_cfl(u.dev.ctx, n, u.ptr, v.ptr, w.ptr, dxs.ptr, gamma, rho, byref(r)))
     return r.value
@@ -260,12 +313,18 @@ class PcgResult:
 
 class PCG:
     def __init__(self, A: Matrix, n_all: int):
-        self.A, self.dev = A, A.dev
+        self.A, self.dev, self.n_all = A, A.dev, int(n_all)
         self.ptr = c_void_p()
         check(lib().gip_pcg_create(self.dev.ctx, A.ptr, n_all, byref(self.ptr)))
+        if not hasattr(A, "_solvers"):
+            A._solvers = weakref.WeakSet()
+        A._solvers.add(self)
+        self.dev._adopt(self)
 
     def solve(self, b, x, x0, *, maxit=100, tol=0.0, precond=True, precision=PCG_FP64, inner_tol=1e-3,
               poll=16, raise_on_breakdown=True):
+        if b.n < self.A.nrows or x.n < self.n_all or x0.n < self.n_all:
+            raise ValueError("PCG vectors shorter than the solver's layout")
         o = PcgOpts(maxit, tol, 1 if precond else 0, precision, inner_tol, poll)
         st = PcgStats()
         rc = lib().gip_pcg_solve(self.ptr, b.ptr, x.ptr, x0.ptr, byref(o), byref(st))
@@ -274,9 +333,11 @@ class PCG:
         return PcgResult(st.iters, st.corrections, st.relres, st.status)
 
     def destroy(self):
-        if self.ptr:
+        if self.ptr and self.dev.ctx:
             lib().gip_pcg_destroy(self.ptr)
-            self.ptr = c_void_p()
+        self.ptr = c_void_p()
+
+    _release = destroy
 
     def __del__(self):
         try:
@@ -290,6 +351,12 @@ class Event:
         self.dev = dev
         self.ptr = c_void_p()
         check(lib().gip_event_create(dev.ctx, byref(self.ptr)))
+        dev._adopt(self)
+
+    def _release(self):
+        if self.ptr:
+            lib().gip_event_destroy(self.ptr)
+        self.ptr = c_void_p()
 
     def record(self):
         check(lib().gip_event_record(self.dev.ctx, self.ptr))
@@ -301,6 +368,6 @@ class Event:
 
     def __del__(self):
         try:
-            lib().gip_event_destroy(self.ptr)
+            self._release()
         except Exception:
             pass

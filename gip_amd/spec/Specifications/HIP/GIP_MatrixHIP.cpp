@@ -34,6 +34,16 @@ void GIP_MatrixHIP::createDevice(int flags) {
     devFlags = flags;
     gip_check(gip_mat_create(ctx(), num_rows, num_cols, nnz, csrRowIndA, csrColIndA, csrValA, flags, &dmat),
               "gip_mat_create");
+    gip_check(gip_mat_cols_needed(dmat, &colsNeeded), "gip_mat_cols_needed");
+}
+
+void GIP_MatrixHIP::checkShapes(GIP_Vector *x, int outLen) {
+    // the kernels trust the vector lengths: check them on the host first
+    if (x->getSize() < colsNeeded || outLen < num_rows) {
+        std::fprintf(stderr, "gip_hip: operator (%d rows, columns up to %d) applied to vectors of %d -> %d entries\n",
+                     num_rows, colsNeeded - 1, x->getSize(), outLen);
+        std::abort();
+    }
 }
 
 void GIP_MatrixHIP::postConstruct(std::string name, GIP_Topo *topo, GIP_Arch *arch) {
@@ -64,10 +74,12 @@ void GIP_MatrixHIP::postConstruct(const gip_csr &A, GIP_Topo *topo, GIP_Arch *ar
 }
 
 void GIP_MatrixHIP::spmv(GIP_Vector *x, GIP_Vector *y) {
+    checkShapes(x, y->getSize());
     gip_check(gip_dcsrmv(ctx(), dmat, x->getDevicePtr(), y->getDevicePtr()), "spmv");
 }
 
 void GIP_MatrixHIP::spmv(GIP_Vector *x, GIP_Vector *y, double alpha, double beta) {
+    checkShapes(x, y->getSize());
     gip_check(gip_dcsrmv_ab(ctx(), dmat, x->getDevicePtr(), y->getDevicePtr(), alpha, beta), "spmv_ab");
 }
 
@@ -77,6 +89,7 @@ void GIP_MatrixHIP::spmv(GIP_Vector *x, GIP_Matrix *B) {
         std::fprintf(stderr, "gip_hip: spmv into the values of a non-HIP operator\n");
         std::abort();
     }
+    checkShapes(x, Bh->nnz);
     gip_check(gip_dcsrmv_vals(ctx(), dmat, x->getDevicePtr(), Bh->dmat), "spmv_vals");
 }
 

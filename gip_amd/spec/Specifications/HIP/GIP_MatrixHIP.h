@@ -42,6 +42,8 @@ class GIP_MatrixHIP : public GIP_Matrix {
 
    protected:
     void createDevice(int flags);
+    void checkShapes(GIP_Vector *x, int outLen);
+    int colsNeeded = 0;
     gip_ctx *ctx() { return GIP_ArchHIP::of(myArch)->ctx(); }
     gip_mat *dmat = nullptr;
     gip_pcg *pcg = nullptr;

@@ -1,9 +1,17 @@
 #include "GIP_NonLinearHIP.h"
 
+#include <cstdio>
+#include <cstdlib>
+
 double GIP_NonLinearHIP::getCFL(GIP_Topo *topo, GIP_Vector *u, GIP_Vector *v, GIP_Vector *w, GIP_Vector *dxs,
                                 GIP_Vector *, double gamma, double rho) {
     double dt = 0.0;
     gip_ctx *c = GIP_ArchHIP::of(u->getMyArch())->ctx();
+    const int n = topo->getInnerSize();
+    if (u->getSize() < n || v->getSize() < n || w->getSize() < n || dxs->getSize() < n) {
+        std::fprintf(stderr, "gip_hip: getCFL on vectors shorter than the inner range\n");
+        std::abort();
+    }
     gip_check(gip_cfl(c, topo->getInnerSize(), u->getDevicePtr(), v->getDevicePtr(), w->getDevicePtr(),
                       dxs->getDevicePtr(), gamma, rho, &dt),
               "gip_cfl");

@@ -52,25 +52,40 @@ void GIP_VectorHIP::postConstruct(const double *values, int n, GIP_Topo *topo, G
 }
 
 int GIP_VectorHIP::range(enum RUN_DOMAIN d) {
+    int n = 0;
     switch (d) {
-        case _ALL_: return size;
-        case _OWNED_: return myTopo->getOwnedSize();
-        case _INNER_: return myTopo->getInnerSize();
-        default: return 0;
+        case _ALL_: n = size; break;
+        case _OWNED_: n = myTopo->getOwnedSize(); break;
+        case _INNER_: n = myTopo->getInnerSize(); break;
+        default: n = 0;
     }
+    if (n > size) {
+        std::fprintf(stderr, "gip_hip: range of %d entries on a vector of %d\n", n, size);
+        std::abort();
+    }
+    return n;
+}
+
+static int covered(GIP_Vector *x, int n) {
+    // the other operand must cover the range too (host-side bounds check)
+    if (x->getSize() < n) {
+        std::fprintf(stderr, "gip_hip: vector of %d entries used over %d\n", x->getSize(), n);
+        std::abort();
+    }
+    return n;
 }
 
 void GIP_VectorHIP::axpy(GIP_Vector *x, double alpha, enum RUN_DOMAIN rdom) {
-    gip_check(gip_daxpy(ctx(), range(rdom), alpha, x->getDevicePtr(), dvec), "axpy");
+    gip_check(gip_daxpy(ctx(), covered(x, range(rdom)), alpha, x->getDevicePtr(), dvec), "axpy");
 }
 
 void GIP_VectorHIP::axpy(GIP_Vector *x, double alpha, double beta, enum RUN_DOMAIN rdom) {
-    gip_check(gip_daxpby(ctx(), range(rdom), alpha, x->getDevicePtr(), beta, dvec), "axpby");
+    gip_check(gip_daxpby(ctx(), covered(x, range(rdom)), alpha, x->getDevicePtr(), beta, dvec), "axpby");
 }
 
 double GIP_VectorHIP::dot(GIP_Vector *x, GIP_Vector *, enum RUN_DOMAIN rdom) {
     double r = 0.0;
-    gip_check(gip_ddot(ctx(), range(rdom), dvec, x->getDevicePtr(), &r), "dot");
+    gip_check(gip_ddot(ctx(), covered(x, range(rdom)), dvec, x->getDevicePtr(), &r), "dot");
     return r;
 }
 
@@ -81,7 +96,8 @@ double GIP_VectorHIP::norm(GIP_Vector *, enum RUN_DOMAIN rdom) {
 }
 
 void GIP_VectorHIP::copyTo(GIP_Vector *x, enum RUN_DOMAIN rdom) {
-    gip_check(gip_memcpy_d2d(ctx(), x->getDevicePtr(), dvec, sizeof(double) * (size_t)range(rdom)), "copyTo");
+    gip_check(gip_memcpy_d2d(ctx(), x->getDevicePtr(), dvec, sizeof(double) * (size_t)covered(x, range(rdom))),
+              "copyTo");
 }
 
 void GIP_VectorHIP::update() {

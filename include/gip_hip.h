@@ -86,6 +86,8 @@ int gip_mat_create(gip_ctx *ctx, int32_t nrows, int32_t ncols, int32_t nnz, cons
 int gip_mat_destroy(gip_mat *A);
 /* nrows, nnz, chosen format, stored (padded) entries */
 int gip_mat_info(const gip_mat *A, int32_t *nrows, int32_t *nnz, int32_t *format, int64_t *stored);
+/* Largest column index + 1: the length an input vector of a product must have. */
+int gip_mat_cols_needed(const gip_mat *A, int32_t *ncols);
 /* Values in CSR order, device <-> host. */
 int gip_mat_get_values(const gip_mat *A, double *host_vals);
 int gip_mat_set_values(gip_mat *A, const double *host_vals);
