@@ -80,6 +80,35 @@ __device__ inline bool last_block_reduce(double v, double *partials, unsigned in
     return true;
 }
 
+// Two sums published with one ticket: in the last workgroup thread 0 gets
+// both totals (out0, out1).  partials holds 2 * gridDim.x values.
+__device__ inline bool last_block_reduce2(double v0, double v1, double *partials, unsigned int *counter, double *out0,
+                                          double *out1) {
+    __shared__ unsigned int s_ticket2;
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = v0;
+        partials[2 * blockIdx.x + 1] = v1;
+        __threadfence();
+        s_ticket2 = atomicAdd(counter, 1u);
+    }
+    __syncthreads();
+    if (s_ticket2 != gridDim.x - 1) return false;
+    __threadfence();
+    double a0 = 0.0, a1 = 0.0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock) {
+        a0 = a0 + __hip_atomic_load(&partials[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        a1 = a1 + __hip_atomic_load(&partials[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    a0 = block_reduce<RED_SUM>(a0);
+    a1 = block_reduce<RED_SUM>(a1);
+    if (threadIdx.x == 0) {
+        *out0 = a0;
+        *out1 = a1;
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return true;
+}
+
 int launch_reduce(gip_ctx *c, int op, int n, const double *x, const double *y, double *dres);
 
 }  // namespace gip

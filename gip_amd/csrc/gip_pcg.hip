@@ -1,19 +1,26 @@
 // Fused, device-resident Jacobi-PCG of libgip_hip (GIP_LinearSolver.h:45-117).
 //
-// FP64 iteration (the reference recurrences, same operation order):
-//   K1  Ap = A p ; dn = p.Ap                  -> alpha = r1/dn        (last workgroup)
-//   K2  x += alpha p ; r += (-alpha) Ap ; s = Dinv r ; r.s, r.r -> beta = r1'/r1, convergence
-//   K3  p = 1.0 s + beta p
-// All scalars live in device memory; every kernel first reads the solver's
-// state word and returns at once when the solve has finished, so the host
-// can queue many iterations and poll the state only now and then.
+// FP64 iteration k, two kernels, the reference recurrences and roundings:
+//   K1  p_k = 1.0 s + beta p_{k-1}   (k >= 2; formed on the fly for every
+//       gathered column, stored once for the own rows; the first iteration
+//       uses p_1 from the setup), Ap = A p_k, dn = p_k.Ap -> alpha = r1/dn
+//   K2  x += alpha p_k ; r += (-alpha) Ap ; s = Dinv r ; r.s, r.r
+//       -> beta = r1'/r1, convergence, iteration count
+// The direction lives in two buffers used alternately (K1 of iteration k
+// reads p_{k-1} from one and writes p_k to the other), so no workgroup can
+// see a half-updated direction.  Forming p on the fly in K1 computes
+// exactly the expression the reference stores (GIP_LinearSolver.h:113).
+// Every scalar lives in device memory; every kernel first reads the
+// solver's state word and returns when the solve has finished, so the host
+// queues many iterations and polls the state only now and then.
 //
-// MIXED: the same three kernels in FP32 (operator values, vectors, SpMV;
-// dots accumulate in FP64) on the correction equation, inside an FP64
-// defect-correction loop: when the FP32 residual has dropped by inner_tol
-// (or reaches the target) the FP32 correction is added to the FP64 x, the
-// FP64 true residual b - A x replaces the FP32 residual (the search
-// direction is kept), and convergence is decided on that FP64 residual.
+// MIXED: the same two kernels in FP32 (operator values, vectors and SpMV;
+// dot products accumulate in FP64) on the correction equation, inside an
+// FP64 defect-correction loop: when the FP32 residual has dropped by
+// inner_tol (or reaches the target) the FP32 correction is added to the
+// FP64 x, the FP64 true residual b - A x replaces the FP32 residual (the
+// search direction is kept: residual replacement), and convergence is
+// decided on that FP64 residual only.
 #include <cmath>
 #include <new>
 
@@ -25,25 +32,25 @@ namespace gip {
 
 // Device scalar slots of a solver.
 enum {
-    S_RHO = 0,    // r.s of the current iterate
+    S_RHO = 0,       // r.s of the current iterate
     S_ALPHA = 1,
     S_BETA = 2,
-    S_RR = 3,     // r.r
-    S_BB = 4,     // b.b
+    S_RR = 3,        // r.r
+    S_BB = 4,        // b.b
     S_DN = 5,
-    S_TOL2 = 6,   // (tol ||b||)^2, 0 when no tolerance
-    S_TRUE_RR = 7,
-    S_COUNT = 8
+    S_TOL2 = 6,      // (tol ||b||)^2, negative when no tolerance
+    S_TRUE_RR = 7,   // FP64 true residual of the last correction (MIXED)
+    S_RHO_PREV = 8,  // r.s before the last update (MIXED)
+    S_COUNT = 16
 };
 // State word: 0 running, 1 converged, 2 breakdown.
 enum { ST_RUN = 0, ST_CONV = 1, ST_BREAK = 2 };
 
 struct EllView {
-    int fmt;  // GIP_FMT_ELL / GIP_FMT_CSR / GIP_FMT_DIAG
+    int fmt;  // GIP_FMT_ELL / GIP_FMT_CSR
     const int32_t *soff, *col, *rp;
     const double *val;
     const float *val32;
-    const double *diag;
 };
 
 static EllView view(const gip_mat *A) {
@@ -54,13 +61,27 @@ static EllView view(const gip_mat *A) {
     v.rp = A->d_rowptr;
     v.val = A->format == GIP_FMT_ELL ? A->d_ell_val : A->d_val;
     v.val32 = A->format == GIP_FMT_ELL ? A->d_ell_val32 : A->d_val32;
-    v.diag = A->d_diag;
     return v;
 }
 
+// The operand of a product: either a plain vector, or the direction formed
+// on the fly as 1.0 s_j + beta p_j on the solved rows (p_j beyond them).
+template <class T>
+struct Operand {
+    const T *p;
+    const T *s;
+    T beta;
+    int n;
+    bool fused;
+    __device__ inline T at(int j) const {
+        if (!fused || j >= n) return p[j];
+        return T(1) * s[j] + beta * p[j];
+    }
+};
+
 // Row sum in stored order; returns false for an empty row.
 template <int W, class T, class V>
-__device__ inline bool row_dot(const EllView &A, const V *__restrict__ val, const T *__restrict__ x, int row, T &sum) {
+__device__ inline bool row_dot(const EllView &A, const V *__restrict__ val, const Operand<T> &x, int row, T &sum) {
     sum = T(0);
     if (W > 0) {
         const int base = A.soff[row >> 6] + (row & 63);
@@ -72,7 +93,7 @@ __device__ inline bool row_dot(const EllView &A, const V *__restrict__ val, cons
             v[k] = __builtin_nontemporal_load(&val[base + 64 * k]);
         }
 #pragma unroll
-        for (int k = 0; k < W; ++k) sum = sum + x[c[k]] * v[k];
+        for (int k = 0; k < W; ++k) sum = sum + x.at(c[k]) * v[k];
         return true;
     } else if (A.fmt == GIP_FMT_ELL) {
         const int s = row >> 6, base = A.soff[s] + (row & 63), width = (A.soff[s + 1] - A.soff[s]) >> 6;
@@ -80,13 +101,13 @@ __device__ inline bool row_dot(const EllView &A, const V *__restrict__ val, cons
         for (int k = 0; k < width; ++k) {
             const int cc = A.col[base + 64 * k];
             if (cc < 0) break;
-            sum = sum + x[cc] * val[base + 64 * k];
+            sum = sum + x.at(cc) * val[base + 64 * k];
             any = true;
         }
         return any;
     } else {
         const int jb = A.rp[row], je = A.rp[row + 1];
-        for (int j = jb; j < je; ++j) sum = sum + x[A.col[j]] * val[j];
+        for (int j = jb; j < je; ++j) sum = sum + x.at(A.col[j]) * val[j];
         return jb != je;
     }
 }
@@ -104,45 +125,52 @@ struct RowSweep {
     }
 };
 
-#define FOR_ROWS(n, row)                                                   \
-    for (RowSweep rs__(n); rs__.c < rs__.c1; rs__.c += rs__.step)          \
+#define FOR_ROWS(n, row)                                          \
+    for (RowSweep rs__(n); rs__.c < rs__.c1; rs__.c += rs__.step) \
         for (int row = rs__.c * kBlock + (int)threadIdx.x; row < (rs__.c + 1) * kBlock && row < (n); row = (n))
 
 struct PcgDev {
     int n;
-    double *x, *r, *p, *s, *Ap;
+    double *x, *r, *s, *Ap;
     const double *b, *dinv;
-    float *r32, *p32, *s32, *Ap32, *xc32;
+    float *r32, *s32, *Ap32, *xc32;
     const float *dinv32;
-    double *sc;      // S_COUNT scalars
-    int *state;
-    double *part0, *part1;
-    unsigned int *cnt;  // 4 counters
+    double *sc;  // S_COUNT scalars
+    int *state, *iters;
+    double *part;       // reduction partials
+    unsigned int *cnt;  // counters
     int precond;
 };
 
+__device__ inline bool running(const PcgDev &d) { return *(volatile int *)d.state == ST_RUN; }
+
 // ---- setup ---------------------------------------------------------------
 
-// x = x0 (all), Ap = (A x)_i or x0_i on empty rows, r = b + (-1.0) Ap  (:65-70)
+// x = x0 (all), direction buffers = x0 outside the solved rows,
+// Ap = (A x0)_i or x0_i on empty rows, r = b + (-1.0) Ap   (:55-70)
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_setup_resid(EllView A, PcgDev d, const double *__restrict__ x0, int n_all) {
+__global__ __launch_bounds__(kBlock) void k_setup_resid(EllView A, PcgDev d, const double *__restrict__ x0, int n_all,
+                                                        double *__restrict__ p0, double *__restrict__ p1) {
     const int stride = gridDim.x * kBlock;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n_all; i += stride) {
         d.x[i] = x0[i];
-        if (i >= d.n) d.p[i] = x0[i];  // the direction keeps x0 outside the solved rows (:55-56)
+        if (i >= d.n) {
+            p0[i] = x0[i];
+            p1[i] = x0[i];
+        }
     }
-    // A reads x0 directly so no grid-wide ordering with the copy is needed
+    Operand<double> xo{x0, nullptr, 0.0, d.n, false};
     FOR_ROWS(d.n, row) {
         double sum;
-        bool any = row_dot<W, double, double>(A, A.val, x0, row, sum);
+        bool any = row_dot<W, double, double>(A, A.val, xo, row, sum);
         double t = any ? sum : x0[row];
         d.Ap[row] = t;
         d.r[row] = d.b[row] + -1.0 * t;
     }
 }
 
-// p = Dinv r (or r); r1 = r.p, b.b, r.r  (:72-80)
-__global__ __launch_bounds__(kBlock) void k_setup_dir(PcgDev d) {
+// p = Dinv r (or r); r1 = r.p, b.b, r.r  (:72-80); state = converged when ||r|| <= tol ||b||
+__global__ __launch_bounds__(kBlock) void k_setup_dir(PcgDev d, double *__restrict__ p, double tol) {
     double rs = 0.0, bb = 0.0, rr = 0.0;
     FOR_ROWS(d.n, row) {
         double ri = d.r[row];
@@ -153,70 +181,68 @@ __global__ __launch_bounds__(kBlock) void k_setup_dir(PcgDev d) {
         } else {
             pi = ri;
         }
-        d.p[row] = pi;
+        p[row] = pi;
         rs = rs + ri * pi;
         bb = bb + d.b[row] * d.b[row];
         rr = rr + ri * ri;
     }
     rs = block_reduce<RED_SUM>(rs);
-    double out;
-    if (last_block_reduce<RED_SUM>(rs, d.part0, d.cnt + 0, &out)) {
-        if (threadIdx.x == 0) d.sc[S_RHO] = out;
+    double out0, out1;
+    if (last_block_reduce<RED_SUM>(rs, d.part, d.cnt + 0, &out0)) {
+        if (threadIdx.x == 0) d.sc[S_RHO] = out0;
     }
     bb = block_reduce<RED_SUM>(bb);
-    if (last_block_reduce<RED_SUM>(bb, d.part1, d.cnt + 1, &out)) {
-        if (threadIdx.x == 0) d.sc[S_BB] = out;
-    }
     rr = block_reduce<RED_SUM>(rr);
-    if (last_block_reduce<RED_SUM>(rr, d.part0 + kMaxRedBlocks, d.cnt + 2, &out)) {
-        if (threadIdx.x == 0) d.sc[S_RR] = out;
-    }
-}
-
-// state after setup: converged when ||r|| <= tol ||b||
-__global__ void k_setup_state(PcgDev d, double tol) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double tol2 = tol > 0.0 ? tol * tol * d.sc[S_BB] : -1.0;
-        d.sc[S_TOL2] = tol2;
-        *d.state = (tol > 0.0 && d.sc[S_RR] <= tol2) ? ST_CONV : ST_RUN;
+    if (last_block_reduce2(bb, rr, d.part + kMaxRedBlocks, d.cnt + 1, &out0, &out1)) {
+        if (threadIdx.x == 0) {
+            d.sc[S_BB] = out0;
+            d.sc[S_RR] = out1;
+            double tol2 = tol > 0.0 ? tol * tol * out0 : -1.0;
+            d.sc[S_TOL2] = tol2;
+            *d.state = (tol > 0.0 && out1 <= tol2) ? ST_CONV : ST_RUN;
+        }
     }
 }
 
 // ---- FP64 iteration --------------------------------------------------------
 
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_iter_spmv(EllView A, PcgDev d) {
-    if (*(volatile int *)d.state != ST_RUN) return;
+template <int W, bool FUSE>
+__global__ __launch_bounds__(kBlock) void k_iter_spmv(EllView A, PcgDev d, const double *__restrict__ p_old,
+                                                      double *__restrict__ p_new) {
+    if (!running(d)) return;
+    const double beta = FUSE ? d.sc[S_BETA] : 0.0;
+    Operand<double> po{FUSE ? p_old : p_new, d.s, beta, d.n, FUSE};
     double acc = 0.0;
     FOR_ROWS(d.n, row) {
+        double pi = po.at(row);
+        if (FUSE) p_new[row] = pi;
         double sum;
-        if (row_dot<W, double, double>(A, A.val, d.p, row, sum)) d.Ap[row] = sum;
-        acc = acc + d.p[row] * d.Ap[row];
+        if (row_dot<W, double, double>(A, A.val, po, row, sum)) d.Ap[row] = sum;
+        acc = acc + pi * d.Ap[row];
     }
     acc = block_reduce<RED_SUM>(acc);
     double dn;
-    if (last_block_reduce<RED_SUM>(acc, d.part0, d.cnt + 0, &dn)) {
+    if (last_block_reduce<RED_SUM>(acc, d.part, d.cnt + 0, &dn)) {
         if (threadIdx.x == 0) {
-            double alpha = d.sc[S_RHO] / dn;
+            double alpha = d.sc[S_RHO] / dn;  // :93
             d.sc[S_DN] = dn;
             d.sc[S_ALPHA] = alpha;
-            if (dn == 0.0 || !isfinite(alpha)) *d.state = ST_BREAK;
+            if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) *d.state = ST_BREAK;
         }
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d) {
-    if (*(volatile int *)d.state != ST_RUN) return;
+__global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, const double *__restrict__ p) {
+    if (!running(d)) return;
     const double alpha = d.sc[S_ALPHA];
-    if (!isfinite(alpha)) return;  // breakdown flagged by the previous kernel's last workgroup
     const double malpha = -1.0 * alpha;
     double rs = 0.0, rr = 0.0;
     FOR_ROWS(d.n, row) {
-        d.x[row] = d.x[row] + alpha * d.p[row];
-        double ri = d.r[row] + malpha * d.Ap[row];
+        d.x[row] = d.x[row] + alpha * p[row];  // :95
+        double ri = d.r[row] + malpha * d.Ap[row];  // :97
         d.r[row] = ri;
         double si;
-        if (d.precond) {
+        if (d.precond) {  // :99-105
             si = 0.0;
             si = si + ri * d.dinv[row];
         } else {
@@ -227,64 +253,55 @@ __global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d) {
         rr = rr + ri * ri;
     }
     rs = block_reduce<RED_SUM>(rs);
-    double out;
-    if (last_block_reduce<RED_SUM>(rs, d.part0, d.cnt + 0, &out)) {
-        if (threadIdx.x == 0) {
-            double r0 = d.sc[S_RHO];
-            d.sc[S_RHO] = out;
-            d.sc[S_BETA] = out / r0;
-        }
-    }
     rr = block_reduce<RED_SUM>(rr);
-    if (last_block_reduce<RED_SUM>(rr, d.part1, d.cnt + 1, &out)) {
-        if (threadIdx.x == 0) d.sc[S_RR] = out;
-    }
-}
-
-// p = 1.0 s + beta p; the convergence decision of the iteration is taken
-// here (after both reductions of k_iter_update have landed).
-__global__ __launch_bounds__(kBlock) void k_iter_dir(PcgDev d, int *iters) {
-    if (*(volatile int *)d.state != ST_RUN) return;
-    const double beta = d.sc[S_BETA];
-    FOR_ROWS(d.n, row) d.p[row] = 1.0 * d.s[row] + beta * d.p[row];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        *iters += 1;
-        double tol2 = d.sc[S_TOL2];
-        if (tol2 >= 0.0 && d.sc[S_RR] <= tol2) *d.state = ST_CONV;
+    double r1, rrt;
+    if (last_block_reduce2(rs, rr, d.part, d.cnt + 1, &r1, &rrt)) {
+        if (threadIdx.x == 0) {
+            double r0 = d.sc[S_RHO];  // :107-111
+            d.sc[S_RHO] = r1;
+            d.sc[S_BETA] = r1 / r0;
+            d.sc[S_RR] = rrt;
+            *d.iters += 1;
+            double tol2 = d.sc[S_TOL2];
+            if (tol2 >= 0.0 && rrt <= tol2) *d.state = ST_CONV;
+        }
     }
 }
 
 // ---- MIXED iteration (FP32 vectors and operator, FP64 dot accumulation) --------
 
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_m_spmv(EllView A, PcgDev d) {
-    if (*(volatile int *)d.state != ST_RUN) return;
+template <int W, bool FUSE>
+__global__ __launch_bounds__(kBlock) void k_m_spmv(EllView A, PcgDev d, const float *__restrict__ p_old,
+                                                   float *__restrict__ p_new) {
+    if (!running(d)) return;
+    const float beta = FUSE ? (float)d.sc[S_BETA] : 0.0f;
+    Operand<float> po{FUSE ? p_old : p_new, d.s32, beta, d.n, FUSE};
     double acc = 0.0;
     FOR_ROWS(d.n, row) {
+        float pi = po.at(row);
+        if (FUSE) p_new[row] = pi;
         float sum;
-        row_dot<W, float, float>(A, A.val32, d.p32, row, sum);
+        row_dot<W, float, float>(A, A.val32, po, row, sum);
         d.Ap32[row] = sum;
-        acc = acc + (double)d.p32[row] * (double)sum;
+        acc = acc + (double)pi * (double)sum;
     }
     acc = block_reduce<RED_SUM>(acc);
     double dn;
-    if (last_block_reduce<RED_SUM>(acc, d.part0, d.cnt + 0, &dn)) {
+    if (last_block_reduce<RED_SUM>(acc, d.part, d.cnt + 0, &dn)) {
         if (threadIdx.x == 0) {
             double alpha = d.sc[S_RHO] / dn;
             d.sc[S_ALPHA] = alpha;
-            if (dn == 0.0 || !isfinite(alpha)) *d.state = ST_BREAK;
+            if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) *d.state = ST_BREAK;
         }
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d) {
-    if (*(volatile int *)d.state != ST_RUN) return;
-    const double alpha = d.sc[S_ALPHA];
-    if (!isfinite(alpha)) return;
-    const float a32 = (float)alpha;
+__global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, const float *__restrict__ p) {
+    if (!running(d)) return;
+    const float a32 = (float)d.sc[S_ALPHA];
     double rs = 0.0, rr = 0.0;
     FOR_ROWS(d.n, row) {
-        d.xc32[row] = d.xc32[row] + a32 * d.p32[row];
+        d.xc32[row] = d.xc32[row] + a32 * p[row];
         float ri = d.r32[row] - a32 * d.Ap32[row];
         d.r32[row] = ri;
         float si = d.precond ? ri * d.dinv32[row] : ri;
@@ -293,30 +310,21 @@ __global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d) {
         rr = rr + (double)ri * (double)ri;
     }
     rs = block_reduce<RED_SUM>(rs);
-    double out;
-    if (last_block_reduce<RED_SUM>(rs, d.part0, d.cnt + 0, &out)) {
+    rr = block_reduce<RED_SUM>(rr);
+    double r1, rrt;
+    if (last_block_reduce2(rs, rr, d.part, d.cnt + 1, &r1, &rrt)) {
         if (threadIdx.x == 0) {
             double r0 = d.sc[S_RHO];
-            d.sc[S_RHO] = out;
-            d.sc[S_BETA] = out / r0;
+            d.sc[S_RHO_PREV] = r0;
+            d.sc[S_RHO] = r1;
+            d.sc[S_BETA] = r1 / r0;
+            d.sc[S_RR] = rrt;
+            *d.iters += 1;
         }
     }
-    rr = block_reduce<RED_SUM>(rr);
-    if (last_block_reduce<RED_SUM>(rr, d.part1, d.cnt + 1, &out)) {
-        if (threadIdx.x == 0) d.sc[S_RR] = out;
-    }
 }
 
-__global__ __launch_bounds__(kBlock) void k_m_dir(PcgDev d, int *iters) {
-    if (*(volatile int *)d.state != ST_RUN) return;
-    const float beta = (float)d.sc[S_BETA];
-    FOR_ROWS(d.n, row) d.p32[row] = d.s32[row] + beta * d.p32[row];
-    if (blockIdx.x == 0 && threadIdx.x == 0) *iters += 1;
-}
-
-// Defect correction: x += xc, xc = 0, r = b - A x in FP64 (empty rows: x),
-// r32 = r, s32 = Dinv r32, rho = r32.s32, true ||r||^2.
-template <int W>
+// x += xc, xc = 0
 __global__ __launch_bounds__(kBlock) void k_m_fold(PcgDev d) {
     FOR_ROWS(d.n, row) {
         d.x[row] = d.x[row] + (double)d.xc32[row];
@@ -324,12 +332,16 @@ __global__ __launch_bounds__(kBlock) void k_m_fold(PcgDev d) {
     }
 }
 
+// Defect correction: r = b - A x in FP64 (empty rows: x), r32 = r,
+// s32 = Dinv r32, rho = r32.s32; the direction is kept and the next beta
+// uses the replaced residual; convergence on the true ||r||.
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_m_resid(EllView A, PcgDev d) {
+__global__ __launch_bounds__(kBlock) void k_m_resid(EllView A, PcgDev d, int first) {
     double rs = 0.0, rr = 0.0;
+    Operand<double> xo{d.x, nullptr, 0.0, d.n, false};
     FOR_ROWS(d.n, row) {
         double sum;
-        bool any = row_dot<W, double, double>(A, A.val, d.x, row, sum);
+        bool any = row_dot<W, double, double>(A, A.val, xo, row, sum);
         double t = any ? sum : d.x[row];
         double ri = d.b[row] + -1.0 * t;
         d.r[row] = ri;
@@ -341,25 +353,24 @@ __global__ __launch_bounds__(kBlock) void k_m_resid(EllView A, PcgDev d) {
         rr = rr + ri * ri;
     }
     rs = block_reduce<RED_SUM>(rs);
-    double out;
-    if (last_block_reduce<RED_SUM>(rs, d.part0, d.cnt + 0, &out)) {
-        if (threadIdx.x == 0) d.sc[S_RHO] = out;
-    }
     rr = block_reduce<RED_SUM>(rr);
-    if (last_block_reduce<RED_SUM>(rr, d.part1, d.cnt + 1, &out)) {
+    double r1, rrt;
+    if (last_block_reduce2(rs, rr, d.part, d.cnt + 1, &r1, &rrt)) {
         if (threadIdx.x == 0) {
-            d.sc[S_TRUE_RR] = out;
-            d.sc[S_RR] = out;
+            if (!first) d.sc[S_BETA] = r1 / d.sc[S_RHO_PREV];
+            d.sc[S_RHO] = r1;
+            d.sc[S_TRUE_RR] = rrt;
+            d.sc[S_RR] = rrt;
             double tol2 = d.sc[S_TOL2];
-            if (tol2 >= 0.0 && out <= tol2) *d.state = ST_CONV;
+            if (tol2 >= 0.0 && rrt <= tol2) *d.state = ST_CONV;
         }
     }
 }
 
-// p32 = s32 (first direction) and zero the correction
-__global__ __launch_bounds__(kBlock) void k_m_start(PcgDev d) {
+// first direction p32 = s32, correction zero
+__global__ __launch_bounds__(kBlock) void k_m_start(PcgDev d, float *__restrict__ p) {
     FOR_ROWS(d.n, row) {
-        d.p32[row] = d.s32[row];
+        p[row] = d.s32[row];
         d.xc32[row] = 0.0f;
     }
 }
@@ -377,12 +388,13 @@ struct gip_pcg {
     gip_ctx *ctx;
     const gip_mat *A;
     int n, n_all, grid;
-    double *r, *p, *s, *Ap, *dinv;
-    float *r32, *p32, *s32, *Ap32, *xc32, *dinv32;
+    double *r, *p[2], *s, *Ap, *dinv;
+    float *r32, *p32[2], *s32, *Ap32, *xc32, *dinv32;
     double *sc, *part;
     unsigned int *cnt;
-    int *state, *iters;
-    int *h_state;  // pinned: state, iters
+    int *state;  // state, iters
+    int *h_state;  // pinned copy
+    double *h_sc;  // pinned scalars
 };
 
 static int pcg_grid(int n) {
@@ -393,38 +405,40 @@ static int pcg_grid(int n) {
     return g;
 }
 
-#define DISPATCH_W(A, KERNEL, ...)                                                    \
-    do {                                                                              \
-        switch ((A)->format == GIP_FMT_ELL ? (A)->uniform_width : -1) {               \
-            case 5: KERNEL<5><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;        \
-            case 6: KERNEL<6><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;        \
-            case 7: KERNEL<7><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;        \
-            default: KERNEL<0><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;       \
-        }                                                                             \
+// Launch KERNEL<W, ...> with W the operator's uniform ELL width (5..7) or 0.
+#define DISPATCH_W(A, KERNEL, TAIL, ...)                                             \
+    do {                                                                             \
+        switch ((A)->format == GIP_FMT_ELL ? (A)->uniform_width : -1) {              \
+            case 5: KERNEL<5 TAIL><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;  \
+            case 6: KERNEL<6 TAIL><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;  \
+            case 7: KERNEL<7 TAIL><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;  \
+            default: KERNEL<0 TAIL><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break; \
+        }                                                                            \
     } while (0)
+#define NOTAIL
+#define FUSED_TAIL , true
+#define FIRST_TAIL , false
 
-static PcgDev make_dev(gip_pcg *S, const double *b, double *x) {
+static PcgDev make_dev(gip_pcg *S, const double *b, double *x, int precond) {
     PcgDev d;
     d.n = S->n;
     d.x = x;
     d.r = S->r;
-    d.p = S->p;
     d.s = S->s;
     d.Ap = S->Ap;
     d.b = b;
     d.dinv = S->dinv;
     d.r32 = S->r32;
-    d.p32 = S->p32;
     d.s32 = S->s32;
     d.Ap32 = S->Ap32;
     d.xc32 = S->xc32;
     d.dinv32 = S->dinv32;
     d.sc = S->sc;
     d.state = S->state;
-    d.part0 = S->part;
-    d.part1 = S->part + 2 * kMaxRedBlocks;
+    d.iters = S->state + 1;
+    d.part = S->part;
     d.cnt = S->cnt;
-    d.precond = 1;
+    d.precond = precond;
     return d;
 }
 
@@ -443,13 +457,14 @@ int gip_pcg_create(gip_ctx *c, const gip_mat *A, int32_t n_all, gip_pcg **out) {
     S->n_all = n_all;
     S->grid = pcg_grid(A->nrows);
     size_t n = (size_t)std::max(1, A->nrows);
-    size_t na = (size_t)std::max(1, n_all);  // SpMV inputs also cover the ghost columns
+    size_t na = (size_t)std::max(1, n_all);  // directions are SpMV inputs: ghost columns included
     hipError_t e = hipSetDevice(c->device);
     auto dalloc = [&](void **p, size_t bytes) {
         if (e == hipSuccess) e = hipMalloc(p, bytes);
     };
     dalloc((void **)&S->r, n * 8);
-    dalloc((void **)&S->p, na * 8);
+    dalloc((void **)&S->p[0], na * 8);
+    dalloc((void **)&S->p[1], na * 8);
     dalloc((void **)&S->s, n * 8);
     dalloc((void **)&S->Ap, n * 8);
     dalloc((void **)&S->dinv, n * 8);
@@ -458,6 +473,7 @@ int gip_pcg_create(gip_ctx *c, const gip_mat *A, int32_t n_all, gip_pcg **out) {
     dalloc((void **)&S->cnt, 16 * sizeof(unsigned int));
     dalloc((void **)&S->state, 2 * sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_state, 2 * sizeof(int), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_sc, S_COUNT * 8, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMemsetAsync(S->cnt, 0, 16 * sizeof(unsigned int), c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(S->sc, 0, S_COUNT * 8, c->stream);
     if (e != hipSuccess) {
@@ -465,7 +481,6 @@ int gip_pcg_create(gip_ctx *c, const gip_mat *A, int32_t n_all, gip_pcg **out) {
         gip_pcg_destroy(S);
         return rc;
     }
-    S->iters = S->state + 1;
     int rc = A->diag_missing ? GIP_OK : gip_mat_diag_inv(A, S->dinv);
     if (!rc) {
         e = hipStreamSynchronize(c->stream);
@@ -483,11 +498,12 @@ int gip_pcg_destroy(gip_pcg *S) {
     if (!S) return GIP_OK;
     hipSetDevice(S->ctx->device);
     hipStreamSynchronize(S->ctx->stream);
-    void *ptrs[] = {S->r, S->p, S->s, S->Ap, S->dinv, S->r32, S->p32, S->s32, S->Ap32, S->xc32, S->dinv32,
-                    S->sc, S->part, S->cnt, S->state};
+    void *ptrs[] = {S->r,     S->p[0],   S->p[1], S->s,    S->Ap,   S->dinv, S->r32, S->p32[0], S->p32[1],
+                    S->s32,   S->Ap32,   S->xc32, S->dinv32, S->sc, S->part, S->cnt, S->state};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (S->h_state) hipHostFree(S->h_state);
+    if (S->h_sc) hipHostFree(S->h_sc);
     delete S;
     return GIP_OK;
 }
@@ -497,13 +513,15 @@ static int ensure_mixed(gip_pcg *S) {
     const gip_mat *A = S->A;
     if ((A->format == GIP_FMT_ELL && !A->d_ell_val32) || (A->format == GIP_FMT_CSR && !A->d_val32))
         return set_error(GIP_EINVAL, "mixed precision needs an operator created with GIP_MAT_FP32");
-    size_t n = (size_t)std::max(1, S->n);
+    size_t n = (size_t)std::max(1, S->n), na = (size_t)std::max(1, S->n_all);
     float **bufs[] = {&S->r32, &S->s32, &S->Ap32, &S->xc32, &S->dinv32};
     for (float **b : bufs) GIP_HIP_TRY(hipMalloc(b, n * 4));
-    // the FP32 direction is an SpMV input: ghost columns included, and zero there
-    // (the correction leaves the ghost values of x unchanged)
-    GIP_HIP_TRY(hipMalloc(&S->p32, (size_t)std::max(1, S->n_all) * 4));
-    GIP_HIP_TRY(hipMemsetAsync(S->p32, 0, (size_t)std::max(1, S->n_all) * 4, S->ctx->stream));
+    // the FP32 directions are SpMV inputs: ghost columns included, and zero
+    // there (the correction leaves the ghost values of x unchanged)
+    for (int k = 0; k < 2; ++k) {
+        GIP_HIP_TRY(hipMalloc(&S->p32[k], na * 4));
+        GIP_HIP_TRY(hipMemsetAsync(S->p32[k], 0, na * 4, S->ctx->stream));
+    }
     if (S->n) {
         k_to32<<<(S->n + kBlock - 1) / kBlock, kBlock, 0, S->ctx->stream>>>(S->n, S->dinv, S->dinv32);
         GIP_LAUNCH_CHECK("dinv32");
@@ -511,9 +529,11 @@ static int ensure_mixed(gip_pcg *S) {
     return GIP_OK;
 }
 
-static int poll_state(gip_pcg *S) {
-    GIP_HIP_TRY(hipMemcpyAsync(S->h_state, S->state, 2 * sizeof(int), hipMemcpyDeviceToHost, S->ctx->stream));
-    GIP_HIP_TRY(hipStreamSynchronize(S->ctx->stream));
+static int poll(gip_pcg *S, bool scalars) {
+    hipStream_t st = S->ctx->stream;
+    GIP_HIP_TRY(hipMemcpyAsync(S->h_state, S->state, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    if (scalars) GIP_HIP_TRY(hipMemcpyAsync(S->h_sc, S->sc, S_COUNT * 8, hipMemcpyDeviceToHost, st));
+    GIP_HIP_TRY(hipStreamSynchronize(st));
     return GIP_OK;
 }
 
@@ -528,81 +548,67 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
         int rc = ensure_mixed(S);
         if (rc) return rc;
     }
-    PcgDev d = make_dev(S, b, x);
-    d.precond = o->precond ? 1 : 0;
+    PcgDev d = make_dev(S, b, x, o->precond ? 1 : 0);
     EllView v = view(A);
-    const int poll = o->poll > 0 ? o->poll : 16;
+    const int pollk = o->poll > 0 ? o->poll : 16;
     GIP_HIP_TRY(hipMemsetAsync(S->state, 0, 2 * sizeof(int), st));
-    // setup, :55-80
-    DISPATCH_W(A, k_setup_resid, v, d, x0, S->n_all);
-    GIP_LAUNCH_CHECK("pcg setup residual");
-    k_setup_dir<<<S->grid, kBlock, 0, st>>>(d);
-    k_setup_state<<<1, 64, 0, st>>>(d, o->tol);
-    GIP_LAUNCH_CHECK("pcg setup direction");
-    int corrections = 0;
-    double last_rr = 0.0;
-    int rc = poll_state(S);
-    if (rc) return rc;
+    // setup, GIP_LinearSolver.h:55-80
+    DISPATCH_W(A, k_setup_resid, NOTAIL, v, d, x0, S->n_all, S->p[0], S->p[1]);
+    k_setup_dir<<<S->grid, kBlock, 0, st>>>(d, S->p[0], o->tol);
+    GIP_LAUNCH_CHECK("pcg setup");
+    int corrections = 0, rc;
+    if ((rc = poll(S, true))) return rc;
     if (!mixed) {
-        int done = 0;
-        while (done < o->maxit && S->h_state[0] == ST_RUN) {
-            int chunk = std::min(poll, o->maxit - done);
-            for (int k = 0; k < chunk; ++k) {
-                DISPATCH_W(A, k_iter_spmv, v, d);
-                k_iter_update<<<S->grid, kBlock, 0, st>>>(d);
-                k_iter_dir<<<S->grid, kBlock, 0, st>>>(d, S->iters);
+        int k = 0;  // iterations queued
+        while (k < o->maxit && S->h_state[0] == ST_RUN) {
+            int chunk = std::min(pollk, o->maxit - k);
+            for (int j = 0; j < chunk; ++j, ++k) {
+                double *pn = S->p[k & 1];
+                if (k == 0)
+                    DISPATCH_W(A, k_iter_spmv, FIRST_TAIL, v, d, pn, pn);
+                else
+                    DISPATCH_W(A, k_iter_spmv, FUSED_TAIL, v, d, S->p[(k - 1) & 1], pn);
+                k_iter_update<<<S->grid, kBlock, 0, st>>>(d, pn);
             }
             GIP_LAUNCH_CHECK("pcg iteration");
-            done += chunk;
-            if ((rc = poll_state(S))) return rc;
+            if ((rc = poll(S, false))) return rc;
         }
     } else {
-        // r32 = r, s32 = Dinv r32, rho = r32.s32 from the FP64 residual of x0
-        DISPATCH_W(A, k_m_resid, v, d);
-        k_m_start<<<S->grid, kBlock, 0, st>>>(d);
+        // FP32 residual of x0 and the first direction
+        DISPATCH_W(A, k_m_resid, NOTAIL, v, d, 1);
+        k_m_start<<<S->grid, kBlock, 0, st>>>(d, S->p32[0]);
         GIP_LAUNCH_CHECK("mixed setup");
-        double h_sc[S_COUNT];
-        GIP_HIP_TRY(hipMemcpyAsync(h_sc, S->sc, sizeof h_sc, hipMemcpyDeviceToHost, st));
-        if ((rc = poll_state(S))) return rc;
-        last_rr = h_sc[S_TRUE_RR];
-        const double inner2 = (o->inner_tol > 0.0 ? o->inner_tol : 1e-3) * (o->inner_tol > 0.0 ? o->inner_tol : 1e-3);
-        int done = 0;
-        while (done < o->maxit && S->h_state[0] == ST_RUN) {
-            int chunk = std::min(poll, o->maxit - done);
-            for (int k = 0; k < chunk; ++k) {
-                DISPATCH_W(A, k_m_spmv, v, d);
-                k_m_update<<<S->grid, kBlock, 0, st>>>(d);
-                k_m_dir<<<S->grid, kBlock, 0, st>>>(d, S->iters);
+        if ((rc = poll(S, true))) return rc;
+        double last_rr = S->h_sc[S_TRUE_RR];
+        const double it = o->inner_tol > 0.0 ? o->inner_tol : 1e-3;
+        const double inner2 = it * it;
+        int k = 0;
+        while (k < o->maxit && S->h_state[0] == ST_RUN) {
+            int chunk = std::min(pollk, o->maxit - k);
+            for (int j = 0; j < chunk; ++j, ++k) {
+                float *pn = S->p32[k & 1];
+                if (k == 0)
+                    DISPATCH_W(A, k_m_spmv, FIRST_TAIL, v, d, pn, pn);
+                else
+                    DISPATCH_W(A, k_m_spmv, FUSED_TAIL, v, d, S->p32[(k - 1) & 1], pn);
+                k_m_update<<<S->grid, kBlock, 0, st>>>(d, pn);
             }
             GIP_LAUNCH_CHECK("mixed iteration");
-            done += chunk;
-            GIP_HIP_TRY(hipMemcpyAsync(h_sc, S->sc, sizeof h_sc, hipMemcpyDeviceToHost, st));
-            if ((rc = poll_state(S))) return rc;
+            if ((rc = poll(S, true))) return rc;
             if (S->h_state[0] != ST_RUN) break;
-            const double rr32 = h_sc[S_RR];
-            const double tol2 = h_sc[S_TOL2];
+            const double rr32 = S->h_sc[S_RR], tol2 = S->h_sc[S_TOL2];
             const bool near_target = tol2 >= 0.0 && rr32 <= tol2;
-            if (rr32 <= inner2 * last_rr || near_target || done >= o->maxit) {
-                k_m_fold<0><<<S->grid, kBlock, 0, st>>>(d);
-                DISPATCH_W(A, k_m_resid, v, d);
+            if (rr32 <= inner2 * last_rr || near_target || k >= o->maxit) {
+                k_m_fold<<<S->grid, kBlock, 0, st>>>(d);
+                DISPATCH_W(A, k_m_resid, NOTAIL, v, d, 0);
                 GIP_LAUNCH_CHECK("mixed correction");
                 ++corrections;
-                GIP_HIP_TRY(hipMemcpyAsync(h_sc, S->sc, sizeof h_sc, hipMemcpyDeviceToHost, st));
-                if ((rc = poll_state(S))) return rc;
-                last_rr = h_sc[S_TRUE_RR];
+                if ((rc = poll(S, true))) return rc;
+                last_rr = S->h_sc[S_TRUE_RR];
             }
         }
-        if (S->h_state[0] == ST_RUN) {
-            // fold any correction left (maxit reached between corrections)
-            k_m_fold<0><<<S->grid, kBlock, 0, st>>>(d);
-            DISPATCH_W(A, k_m_resid, v, d);
-            GIP_LAUNCH_CHECK("mixed final correction");
-            ++corrections;
-        }
     }
-    double h_sc[S_COUNT];
-    GIP_HIP_TRY(hipMemcpyAsync(h_sc, S->sc, sizeof h_sc, hipMemcpyDeviceToHost, st));
-    if ((rc = poll_state(S))) return rc;
+    if ((rc = poll(S, true))) return rc;
     int status = GIP_OK;
     if (S->h_state[0] == ST_BREAK)
         status = GIP_EBREAKDOWN;
@@ -611,7 +617,8 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
     if (stats) {
         stats->iters = S->h_state[1];
         stats->corrections = corrections;
-        stats->relres = h_sc[S_BB] > 0.0 ? std::sqrt(h_sc[S_RR] / h_sc[S_BB]) : std::sqrt(h_sc[S_RR]);
+        const double bb = S->h_sc[S_BB], rr = S->h_sc[S_RR];
+        stats->relres = bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
         stats->status = status;
     }
     return status == GIP_EBREAKDOWN ? set_error(GIP_EBREAKDOWN, "PCG breakdown (p.Ap == 0 or non-finite step)") : GIP_OK;
