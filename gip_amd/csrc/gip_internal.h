@@ -49,6 +49,7 @@ struct gip_mat {
     int32_t nrows, ncols, nnz;
     int32_t cols_needed;  // largest column index + 1
     int32_t format;
+    int32_t lanes;  // CSR: lanes per row (1, 2, 4, 8)
     // CSR (kept for every format: value get/set, transposed products, diagonal)
     int32_t *d_rowptr;
     int32_t *d_col;

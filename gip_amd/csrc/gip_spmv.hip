@@ -86,6 +86,41 @@ __global__ __launch_bounds__(kBlock) void k_csr(int nrows, const int32_t *__rest
         y[out] = a * sum + b * y[out];
 }
 
+// CSR with R lanes per row (64/R rows per wavefront): the lanes load a
+// row's entries cooperatively (coalesced) and form the products; the
+// group's first lane then adds the products in stored order, so the sum
+// is the same as the thread-per-row kernel's.
+template <int MODE, int R>
+__global__ __launch_bounds__(kBlock) void k_csr_vec(int nrows, const int32_t *__restrict__ rp,
+                                                   const int32_t *__restrict__ col, const double *__restrict__ val,
+                                                   const double *__restrict__ x, double *__restrict__ y, double a,
+                                                   double b, const int32_t *__restrict__ scatter) {
+    const int gid = blockIdx.x * kBlock + threadIdx.x;
+    const int row = gid / R, lane = threadIdx.x % R, lead = threadIdx.x & ~(R - 1) & 63;
+    const bool live = row < nrows;
+    const int jb = live ? rp[row] : 0, je = live ? rp[row + 1] : 0;
+    double sum = 0.0;
+    // every lane of the wavefront runs the same number of rounds
+    int rounds = (je - jb + R - 1) / R;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) rounds = max(rounds, __shfl_xor(rounds, off, 64));
+    for (int t = 0; t < rounds; ++t) {
+        const int j = jb + t * R + lane;
+        const double prod = j < je ? x[col[j]] * val[j] : 0.0;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const double pq = __shfl(prod, lead + q, 64);
+            if (jb + t * R + q < je) sum = sum + pq;
+        }
+    }
+    if (!live || lane != 0 || jb == je) return;
+    const int out = scatter ? scatter[row] : row;
+    if (MODE == 0)
+        y[out] = sum;
+    else
+        y[out] = a * sum + b * y[out];
+}
+
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_diag(int nrows, const double *__restrict__ d, const double *__restrict__ x,
                                                 double *__restrict__ y, double a, double b,
@@ -206,6 +241,23 @@ int spmv_launch(gip_ctx *c, const gip_mat *A, const double *x, double *y, double
             k_diag<0><<<g, kBlock, 0, st>>>(A->nrows, A->d_diag, x, y, a, b, scatter);
         else
             k_diag<1><<<g, kBlock, 0, st>>>(A->nrows, A->d_diag, x, y, a, b, scatter);
+    } else if (A->lanes > 1) {
+        const int gv = (int)(((int64_t)A->nrows * A->lanes + kBlock - 1) / kBlock);
+#define GIP_VEC_CASE(Rc)                                                                                            \
+    case Rc:                                                                                                        \
+        if (mode == 0)                                                                                              \
+            k_csr_vec<0, Rc><<<gv, kBlock, 0, st>>>(A->nrows, A->d_rowptr, A->d_col, A->d_val, x, y, a, b, scatter); \
+        else                                                                                                        \
+            k_csr_vec<1, Rc><<<gv, kBlock, 0, st>>>(A->nrows, A->d_rowptr, A->d_col, A->d_val, x, y, a, b, scatter); \
+        break;
+        switch (A->lanes) {
+            GIP_VEC_CASE(2)
+            GIP_VEC_CASE(4)
+            GIP_VEC_CASE(8)
+            default:
+                return set_error(GIP_EINVAL, "unsupported lanes per row");
+        }
+#undef GIP_VEC_CASE
     } else {
         if (mode == 0)
             k_csr<0><<<g, kBlock, 0, st>>>(A->nrows, A->d_rowptr, A->d_col, A->d_val, x, y, a, b, scatter);
@@ -258,6 +310,7 @@ int gip_mat_create(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, const 
     A->ncols = ncols;
     A->nnz = nnz;
     A->cols_needed = maxc + 1;
+    A->lanes = 1;
     const int32_t want = flags & 0xff;
     const bool fp32 = (flags & GIP_MAT_FP32) != 0;
 
@@ -379,6 +432,78 @@ int gip_mat_info(const gip_mat *A, int32_t *nrows, int32_t *nnz, int32_t *format
     if (nnz) *nnz = A->nnz;
     if (format) *format = A->format;
     if (stored) *stored = A->ell_stored;
+    return GIP_OK;
+}
+
+int gip_mat_autotune(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, const int32_t *rowptr, const int32_t *col,
+                     const double *val, int32_t flags, const double *x, double *y, int32_t reps, gip_mat **out,
+                     gip_tune_report *rep) {
+    if (!c || !out || !x || !y || reps < 1) return set_error(GIP_EINVAL, "gip_mat_autotune: bad argument");
+    *out = nullptr;
+    struct Cand { int fmt, lanes; };
+    const Cand cands[5] = {{GIP_FMT_ELL, 1}, {GIP_FMT_CSR, 1}, {GIP_FMT_CSR, 2}, {GIP_FMT_CSR, 4}, {GIP_FMT_CSR, 8}};
+    gip_tune_report r{};
+    r.chosen_format = -1;
+    float best = 0.0f;
+    gip_mat *keep = nullptr, *csr = nullptr;
+    hipEvent_t e0, e1;
+    GIP_HIP_TRY(hipEventCreate(&e0));
+    GIP_HIP_TRY(hipEventCreate(&e1));
+    int rc = GIP_OK;
+    for (int k = 0; k < 5 && !rc; ++k) {
+        r.us[k] = -1.0f;
+        gip_mat *A = nullptr;
+        if (cands[k].fmt == GIP_FMT_ELL) {
+            rc = gip_mat_create(c, nrows, ncols, nnz, rowptr, col, val, GIP_FMT_AUTO | (flags & ~0xff), &A);
+            if (rc) break;
+            if (A->format != GIP_FMT_ELL) {  // too much padding for ELL: not a candidate
+                gip_mat_destroy(A);
+                continue;
+            }
+        } else {
+            if (!csr) rc = gip_mat_create(c, nrows, ncols, nnz, rowptr, col, val, GIP_FMT_CSR | (flags & ~0xff), &csr);
+            if (rc) break;
+            A = csr;
+            A->lanes = cands[k].lanes;
+        }
+        for (int w = 0; w < 2 && !rc; ++w) rc = spmv_launch(c, A, x, y, 1.0, 0.0, 0, nullptr);
+        hipEventRecord(e0, c->stream);
+        for (int w = 0; w < reps && !rc; ++w) rc = spmv_launch(c, A, x, y, 1.0, 0.0, 0, nullptr);
+        hipEventRecord(e1, c->stream);
+        float ms = 0.0f;
+        if (!rc && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) {
+            r.us[k] = 1e3f * ms / reps;
+            if (r.chosen_format < 0 || r.us[k] < best) {
+                best = r.us[k];
+                r.chosen_format = cands[k].fmt;
+                r.chosen_lanes = cands[k].lanes;
+                if (keep && keep != csr) gip_mat_destroy(keep);
+                keep = A;
+            } else if (A != csr) {
+                gip_mat_destroy(A);
+            }
+        } else if (A != csr) {
+            gip_mat_destroy(A);
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (csr && keep != csr) gip_mat_destroy(csr);
+    if (rc || !keep) {
+        if (keep) gip_mat_destroy(keep);
+        return rc ? rc : set_error(GIP_EHIP, "gip_mat_autotune: no candidate could be timed");
+    }
+    if (keep == csr) keep->lanes = r.chosen_lanes;
+    if (rep) *rep = r;
+    *out = keep;
+    return GIP_OK;
+}
+
+int gip_mat_set_lanes(gip_mat *A, int32_t lanes) {
+    if (!A || !(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8))
+        return set_error(GIP_EINVAL, "gip_mat_set_lanes: lanes must be 1, 2, 4 or 8");
+    if (A->format != GIP_FMT_CSR && lanes != 1) return set_error(GIP_EINVAL, "gip_mat_set_lanes: CSR operators only");
+    A->lanes = lanes;
     return GIP_OK;
 }
 

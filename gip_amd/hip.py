@@ -29,6 +29,10 @@ class PcgStats(ctypes.Structure):
     _fields_ = [("iters", c_int32), ("corrections", c_int32), ("relres", c_double), ("status", c_int32)]
 
 
+class TuneReport(ctypes.Structure):
+    _fields_ = [("chosen_format", c_int32), ("chosen_lanes", c_int32), ("us", c_float * 5)]
+
+
 # exported entry points and their argument types (kept in step with include/gip_hip.h)
 SIGNATURES = {
     "gip_last_error": ([], ctypes.c_char_p),
@@ -51,6 +55,9 @@ SIGNATURES = {
     "gip_mat_destroy": ([c_void_p], c_int),
     "gip_mat_info": ([c_void_p, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32), POINTER(c_int64)], c_int),
     "gip_mat_cols_needed": ([c_void_p, POINTER(c_int32)], c_int),
+    "gip_mat_set_lanes": ([c_void_p, c_int32], c_int),
+    "gip_mat_autotune": ([c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_int32, c_void_p,
+                          c_void_p, c_int32, POINTER(c_void_p), c_void_p], c_int),
     "gip_mat_get_values": ([c_void_p, c_void_p], c_int),
     "gip_mat_set_values": ([c_void_p, c_void_p], c_int),
     "gip_mat_diag_inv": ([c_void_p, c_void_p], c_int),
@@ -188,6 +195,28 @@ class Matrix:
     @classmethod
     def from_csr(cls, dev, csr, **kw):
         return cls(dev, csr.rowptr, csr.col, csr.val, nrows=csr.nrows, ncols=csr.ncols, **kw)
+
+    @classmethod
+    def autotune(cls, dev, csr, reps=5, fp32=False):
+        """Fastest of the candidate layouts (gip_mat_autotune); returns (Matrix, report dict)."""
+        rowptr = np.ascontiguousarray(csr.rowptr, dtype=np.int32)
+        col = np.ascontiguousarray(csr.col, dtype=np.int32)
+        val = np.ascontiguousarray(csr.val, dtype=np.float64)
+        need = int(col.max()) + 1 if col.size else 1
+        x = dev.to_device(np.ones(max(need, 1)))
+        y = dev.array(max(csr.nrows, 1))
+        self = cls.__new__(cls)
+        self.dev, self.nrows, self.nnz = dev, int(csr.nrows), int(val.size)
+        self.ptr = c_void_p()
+        rep = TuneReport()
+        check(lib().gip_mat_autotune(dev.ctx, self.nrows, int(csr.ncols), self.nnz, rowptr.ctypes.data,
+                                     col.ctypes.data, val.ctypes.data, MAT_FP32 if fp32 else 0, x.ptr, y.ptr, reps,
+                                     byref(self.ptr), ctypes.cast(byref(rep), c_void_p)))
+        dev._adopt(self)
+        return self, {"format": rep.chosen_format, "lanes": rep.chosen_lanes, "us": list(rep.us)}
+
+    def set_lanes(self, lanes):
+        check(lib().gip_mat_set_lanes(self.ptr, lanes))
 
     def info(self):
         nr, nz, fm, st = c_int32(), c_int32(), c_int32(), c_int64()

@@ -88,6 +88,23 @@ int gip_mat_destroy(gip_mat *A);
 int gip_mat_info(const gip_mat *A, int32_t *nrows, int32_t *nnz, int32_t *format, int64_t *stored);
 /* Largest column index + 1: the length an input vector of a product must have. */
 int gip_mat_cols_needed(const gip_mat *A, int32_t *ncols);
+/* CSR operators: lanes per row (1, 2, 4, 8 = 64, 32, 16, 8 rows per
+ * wavefront); the lanes load a row cooperatively and the sum keeps the
+ * stored order. */
+int gip_mat_set_lanes(gip_mat *A, int32_t lanes);
+/* In-project SpMV autotuner: builds the operator in each candidate layout
+ * (sliced ELL, 64 rows per wavefront; CSR with 1, 2, 4 or 8 lanes per row),
+ * times `reps` products y = A x on the device (x, y: device vectors covering
+ * the operator) and returns the fastest; the rest are freed.  `report`
+ * (optional) gets each candidate's mean time in microseconds (-1: not a
+ * candidate) in that order. */
+typedef struct gip_tune_report {
+    int32_t chosen_format, chosen_lanes;
+    float us[5];
+} gip_tune_report;
+int gip_mat_autotune(gip_ctx *ctx, int32_t nrows, int32_t ncols, int32_t nnz, const int32_t *rowptr, const int32_t *col,
+                     const double *val, int32_t flags, const double *x, double *y, int32_t reps, gip_mat **A,
+                     gip_tune_report *report);
 /* Values in CSR order, device <-> host. */
 int gip_mat_get_values(const gip_mat *A, double *host_vals);
 int gip_mat_set_values(gip_mat *A, const double *host_vals);

@@ -78,6 +78,39 @@ def test_spmv_forced_layouts(dev, cav46, fmt):
         assert info["format"] == fmt
 
 
+@pytest.mark.parametrize("lanes", [2, 4, 8])
+def test_spmv_lanes_per_row_bitwise(dev, cav46, lanes):
+    """CSR with several lanes per row keeps the stored summation order."""
+    for name in ("L", "Ec", "SO", "Mres"):
+        A = cav46.op(name)
+        rng = np.random.default_rng(lanes)
+        x = rng.standard_normal(_ncols(cav46, A))
+        y0 = rng.standard_normal(A.nrows)
+        M = hip.Matrix.from_csr(dev, A, fmt=hip.FMT_CSR)
+        M.set_lanes(lanes)
+        dy = dev.to_device(y0)
+        M.spmv_ab(dev.to_device(x), dy, 0.25, 2.0)
+        want = y0.copy()
+        pyoracle.spmv_ab(pyoracle.Csr.of(A), x, want, 0.25, 2.0)
+        assert np.array_equal(dy.download(), want), (name, lanes)
+
+
+def test_autotuner_picks_a_timed_candidate(dev, cav46):
+    for name in ("L", "Ec"):
+        A = cav46.op(name)
+        M, rep = hip.Matrix.autotune(dev, A, reps=3)
+        timed = [u for u in rep["us"] if u > 0]
+        idx = 0 if rep["format"] == hip.FMT_ELL else {1: 1, 2: 2, 4: 3, 8: 4}[rep["lanes"]]
+        assert timed and min(timed) == pytest.approx(rep["us"][idx], rel=1e-6)
+        x = np.random.default_rng(0).standard_normal(_ncols(cav46, A))
+        dy = dev.to_device(np.zeros(A.nrows))
+        M.spmv(dev.to_device(x), dy)
+        want = np.zeros(A.nrows)
+        pyoracle.spmv(pyoracle.Csr.of(A), x, want)
+        assert np.array_equal(dy.download(), want)
+        print(f"[autotune {name}] {rep}")
+
+
 def test_layout_choice(dev, cav46):
     assert hip.Matrix.from_csr(dev, cav46.op("L")).info()["format"] == hip.FMT_ELL
     A = cav46.op("L")
