@@ -9,7 +9,8 @@ import pytest
 from gip_amd import hip
 
 ROOT = pathlib.Path(__file__).resolve().parents[1]
-HEADERS = {"gip_hip.h": "libgip_hip.so", "gip_cavity.h": "libgip_cavity.so", "gip_dns.h": "libgip_dns.so"}
+HEADERS = {"gip_hip.h": "libgip_hip.so", "gip_cavity.h": "libgip_cavity.so", "gip_dns.h": "libgip_dns.so",
+           "gip_batch.h": "libgip_hip.so"}
 
 
 def declared(header):

@@ -1,0 +1,105 @@
+"""Batched cavities (include/gip_batch.h) against the oracle run instance by
+instance on the same operators.
+
+Bar: with the FP64 solver every instance's per-step time, dt, massr and
+u, v, w, p fields agree with the oracle to 1e-10 relative (only reduction
+orders differ) and PCG iteration counts to +-1; graph replay equals eager
+execution bitwise; the mixed-precision batch matches the FP64 oracle to the
+solver tolerance.
+"""
+import numpy as np
+import pytest
+
+from gip_amd import batch, cavity, hip
+from oracle import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = hip.Device(0)
+    yield d
+    d.close()
+
+
+def _cases(tmp_path_factory, sizes, gammas):
+    cavs, dirs = [], []
+    for k, (n, g) in enumerate(zip(sizes, gammas)):
+        c = cavity.generate(n, gamma=g)
+        d = tmp_path_factory.mktemp(f"b{k}")
+        c.write(d, f"c{k}")
+        cavs.append(c)
+        dirs.append(d)
+    return cavs, dirs
+
+
+SIZES = [9, 12, 7, 16]
+GAMMAS = [1e-2, 1 / 400, 1 / 3200, 1e-3]
+
+
+def _rel(a, b):
+    return abs(a - b) / abs(b) if b else abs(a)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_fp64_matches_oracle_per_instance(dev, tmp_path_factory, graph):
+    cavs, dirs = _cases(tmp_path_factory, SIZES, GAMMAS)
+    b = batch.Batch(dev, cavs, GAMMAS, pcg_tol=1e-9, pcg_maxit=2000, precision=hip.PCG_FP64, poll=8,
+                    use_graph=graph)
+    orcs = [pyoracle.Dns(d, f"c{k}", gamma=g, pcg_maxit=2000, pcg_tol=1e-9) for k, (d, g) in
+            enumerate(zip(dirs, GAMMAS))]
+    for step in range(5):
+        b.step()
+        for k, o in enumerate(orcs):
+            w = o.step()
+            g = b.info(k)
+            print(f"[batch step {step + 1} inst {k}] massr {g['massr']:.6e}/{w['massr']:.6e} "
+                  f"pcg {g['pcg_iters']}/{w['pcg_iters']}", flush=True)
+            assert g["pcg_status"] == 0
+            for key in ("time", "dt", "massr"):
+                assert _rel(g[key], w[key]) <= 1e-10, (step, k, key, g[key], w[key])
+            assert abs(g["pcg_iters"] - w["pcg_iters"]) <= 1
+    for k, o in enumerate(orcs):
+        for f in ("u", "v", "w", "p"):
+            a, r = b.field(k, f), o.field(f)
+            assert np.linalg.norm(a - r) <= 1e-10 * np.linalg.norm(r), (k, f)
+
+
+def test_batch_graph_replay_equals_eager(dev, tmp_path_factory):
+    cavs, _ = _cases(tmp_path_factory, SIZES, GAMMAS)
+    runs = []
+    for graph in (False, True):
+        b = batch.Batch(dev, cavs, GAMMAS, pcg_tol=1e-8, precision=hip.PCG_MIXED, poll=4, use_graph=graph)
+        out = []
+        for _ in range(4):
+            b.step()
+            dev.sync()
+            out.append([b.info(k) for k in range(len(cavs))])
+        runs.append((out, [b.field(k, "p") for k in range(len(cavs))]))
+        b.close()
+    assert runs[0][0] == runs[1][0]
+    for a, c in zip(runs[0][1], runs[1][1]):
+        assert np.array_equal(a, c)
+
+
+def test_batch_mixed_precision_close_to_fp64_oracle(dev, tmp_path_factory):
+    cavs, dirs = _cases(tmp_path_factory, SIZES, GAMMAS)
+    b = batch.Batch(dev, cavs, GAMMAS, pcg_tol=1e-10, pcg_maxit=5000, precision=hip.PCG_MIXED, poll=16,
+                    autotune=True)
+    orcs = [pyoracle.Dns(d, f"c{k}", gamma=g, pcg_maxit=5000, pcg_tol=1e-10) for k, (d, g) in
+            enumerate(zip(dirs, GAMMAS))]
+    for _ in range(5):
+        b.step()
+        for k, o in enumerate(orcs):
+            w = o.step()
+            g = b.info(k)
+            assert g["pcg_status"] == 0
+            assert _rel(g["massr"], w["massr"]) <= 1e-6
+    for k, o in enumerate(orcs):
+        for f in ("u", "v", "w", "p"):
+            a, r = b.field(k, f), o.field(f)
+            err = np.linalg.norm(a - r) / np.linalg.norm(r)
+            print(f"[batch mixed inst {k} {f}] rel {err:.2e}")
+            assert err <= 1e-7, (k, f, err)
+    print("[batch layouts]", b.layouts())
