@@ -52,7 +52,7 @@ def build_cavity():
           "-Wl,-rpath,$ORIGIN/..", "-o", BIN / "gip_cavity_tool"])
 
 
-HIP_SOURCES = ["gip_runtime.hip", "gip_blas1.hip", "gip_spmv.hip", "gip_pcg.hip", "gip_batch.hip"]
+HIP_SOURCES = ["gip_runtime.hip", "gip_blas1.hip", "gip_spmv.hip", "gip_pcg.hip", "gip_batch.hip", "gip_adjoint.hip"]
 
 
 def build_hip():

@@ -10,7 +10,7 @@ from gip_amd import hip
 
 ROOT = pathlib.Path(__file__).resolve().parents[1]
 HEADERS = {"gip_hip.h": "libgip_hip.so", "gip_cavity.h": "libgip_cavity.so", "gip_dns.h": "libgip_dns.so",
-           "gip_batch.h": "libgip_hip.so"}
+           "gip_batch.h": "libgip_hip.so", "gip_adjoint.h": "libgip_hip.so"}
 
 
 def declared(header):
