@@ -72,7 +72,7 @@ def pcg_leg(hip, dev, A, b_dev, n, n_all, precision, steps, warmup, maxit, bnorm
     times, res = [], None
     for k in range(warmup + steps):
         e0.record()
-        res = solver.solve(b_dev, x, z, maxit=maxit, tol=1e-10, precision=precision, inner_tol=1e-3, poll=32)
+        res = solver.solve(b_dev, x, z, maxit=maxit, tol=1e-10, precision=precision, inner_tol=0.1, poll=32)
         e1.record()
         ms = e0.elapsed_ms(e1)
         if k >= warmup:

@@ -37,7 +37,7 @@ class Batch:
     """Many cavities (gip_amd.cavity.Cavity, every operator built) advanced together."""
 
     def __init__(self, dev: hip.Device, cavities, gammas, *, rho=1.0, pcg_tol=1e-8, pcg_maxit=5000,
-                 precision=hip.PCG_MIXED, inner_tol=1e-3, poll=16, use_graph=True, autotune=False):
+                 precision=hip.PCG_MIXED, inner_tol=0.1, poll=16, use_graph=True, autotune=False):
         self.dev = dev
         self.cavities = list(cavities)
         n = len(self.cavities)

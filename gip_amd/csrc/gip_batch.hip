@@ -417,7 +417,8 @@ __global__ __launch_bounds__(kBlock) void k_bpm_resid(EllView A, BPcg d, Seg S, 
         double v[2] = {(double)r32 * (double)s32, ri * ri}, out[2];
         if (seg_reduce<2, RED_SUM>(v, c, inst, S, d.part, d.cnt + 2 * d.B, out) && threadIdx.x == 0) {
             double *q = bsc(d, inst);
-            q[B_BETA] = out[0] / q[B_RHO_PREV];
+            // keep the direction unless the FP32 recursion drifted >10x from the true residual
+            q[B_BETA] = out[1] > 100.0 * q[B_RR] ? 0.0 : out[0] / q[B_RHO_PREV];
             q[B_RHO] = out[0];
             q[B_RR] = out[1];
             q[B_LAST_RR] = out[1];
@@ -614,7 +615,7 @@ BPcg pcg_view(gip_batch *b) {
     d.part = b->part;
     d.cnt = b->cnt;
     d.maxit = b->o.pcg_maxit;
-    const double it = b->o.inner_tol > 0 ? b->o.inner_tol : 1e-3;
+    const double it = b->o.inner_tol > 0 ? b->o.inner_tol : 0.1;
     d.inner2 = it * it;
     return d;
 }

@@ -275,7 +275,9 @@ __global__ __launch_bounds__(kBlock) void k_m_resid(EllView A, PcgDev d, int fir
     double r1, rrt;
     if (last_block_reduce2(rs, rr, d.part, d.cnt + 1, &r1, &rrt)) {
         if (threadIdx.x == 0) {
-            if (!first) d.sc[S_BETA] = r1 / d.sc[S_RHO_PREV];
+            // keep the direction (reliable update) unless the FP32 recursion
+            // had drifted from the true residual by more than 10x: then restart
+            if (!first) d.sc[S_BETA] = rrt > 100.0 * d.sc[S_RR] ? 0.0 : r1 / d.sc[S_RHO_PREV];
             d.sc[S_RHO] = r1;
             d.sc[S_TRUE_RR] = rrt;
             d.sc[S_RR] = rrt;
@@ -498,7 +500,7 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
         GIP_LAUNCH_CHECK("mixed setup");
         if ((rc = poll(S, true))) return rc;
         double last_rr = S->h_sc[S_TRUE_RR];
-        const double it = o->inner_tol > 0.0 ? o->inner_tol : 1e-3;
+        const double it = o->inner_tol > 0.0 ? o->inner_tol : 0.1;
         const double inner2 = it * it;
         int k = 0;
         while (k < o->maxit && S->h_state[0] == ST_RUN) {

@@ -51,7 +51,7 @@ class Simulation:
     """GIP_DNS<GIP_MatrixHIP, GIP_VectorHIP, GIP_ArchHIP, GIP_NonLinearHIP> on one device."""
 
     def __init__(self, *, root=None, name=None, cavity=None, gamma=1e-3, rho=1.0, pcg_maxit=100, pcg_tol=0.0,
-                 pcg_mode="reference", pcg_inner_tol=1e-3, print_mode=0, device=0):
+                 pcg_mode="reference", pcg_inner_tol=0.1, print_mode=0, device=0):
         cfg = Config(gamma, rho, pcg_maxit, pcg_tol, MODES[pcg_mode], pcg_inner_tol, print_mode, device)
         self.h = c_void_p()
         if cavity is not None:

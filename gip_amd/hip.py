@@ -350,7 +350,7 @@ class PCG:
         A._solvers.add(self)
         self.dev._adopt(self)
 
-    def solve(self, b, x, x0, *, maxit=100, tol=0.0, precond=True, precision=PCG_FP64, inner_tol=1e-3,
+    def solve(self, b, x, x0, *, maxit=100, tol=0.0, precond=True, precision=PCG_FP64, inner_tol=0.1,
               poll=16, raise_on_breakdown=True):
         if b.n < self.A.nrows or x.n < self.n_all or x0.n < self.n_all:
             raise ValueError("PCG vectors shorter than the solver's layout")

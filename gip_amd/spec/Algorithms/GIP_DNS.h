@@ -93,7 +93,7 @@ void GIP_DNS<Matrix, Vector, Node, NonLinear>::configure(const GIP_Parameters &p
     solverParam.setInt("maxIt", p.hasInt("pcg_maxIt") ? p.getInt("pcg_maxIt") : 100);
     solverParam.setBool("precond", p.hasBool("pcg_precond") ? p.getBool("pcg_precond") : true);
     solverParam.setString("mode", p.hasString("pcg_mode") ? p.getString("pcg_mode") : std::string("reference"));
-    solverParam.setDouble("innerTol", p.hasDouble("pcg_innerTol") ? p.getDouble("pcg_innerTol") : 1e-3);
+    solverParam.setDouble("innerTol", p.hasDouble("pcg_innerTol") ? p.getDouble("pcg_innerTol") : 0.1);
 }
 
 template <class Matrix, class Vector, class Node, class NonLinear>

@@ -56,7 +56,7 @@ class GIP_LinearSolver {
     int maxIt = 0;
     bool precond = true;
     std::string mode = "reference";
-    double innerTol = 1e-3;
+    double innerTol = 0.1;
     GIP_Topo *myTopo = nullptr;
     GIP_Arch *myNode = nullptr;
     int lastIters = 0, lastStat = 0, lastCorr = 0;
@@ -69,7 +69,7 @@ void GIP_LinearSolver<Matrix, Vector>::setUp(Matrix *A, GIP_Parameters *param) {
     maxIt = param->getInt("maxIt");
     precond = param->hasBool("precond") ? param->getBool("precond") : true;
     mode = param->hasString("mode") ? param->getString("mode") : std::string("reference");
-    innerTol = param->hasDouble("innerTol") ? param->getDouble("innerTol") : 1e-3;
+    innerTol = param->hasDouble("innerTol") ? param->getDouble("innerTol") : 0.1;
     dA = A;
     myTopo = A->getMyTopo();
     if (precond) diagonal.setUpDiagonal(A);

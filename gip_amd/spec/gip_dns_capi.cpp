@@ -38,7 +38,7 @@ static GIP_Parameters params_of(const gip_dns_config *c) {
     p.setDouble("pcg_tol", c->pcg_tol);
     p.setBool("pcg_precond", true);
     p.setString("pcg_mode", c->pcg_mode == 1 ? "fused" : c->pcg_mode == 2 ? "mixed" : "reference");
-    p.setDouble("pcg_innerTol", c->pcg_inner_tol > 0 ? c->pcg_inner_tol : 1e-3);
+    p.setDouble("pcg_innerTol", c->pcg_inner_tol > 0 ? c->pcg_inner_tol : 0.1);
     p.setInt("print", c->print);
     p.setInt("device", c->device);
     p.setInt("steps", 0);
