@@ -167,8 +167,8 @@ def main():
     log(f"spmv fp64: {spmv_ms:.4f} ms, {spmv_gbs:.0f} GB/s algorithmic ({alg / 1e9:.3f} GB)")
     del x, y
 
-    fp64 = pcg_leg(hip, dev, A, b_dev, N, NA, hip.PCG_FP64, args.steps, args.warmup, 200000, bnorm)
-    mixed = pcg_leg(hip, dev, A, b_dev, N, NA, hip.PCG_MIXED, args.steps, args.warmup, 200000, bnorm)
+    fp64 = pcg_leg(hip, dev, A, b_dev, N, NA, hip.PCG_FP64, args.steps, args.warmup, 20000, bnorm)
+    mixed = pcg_leg(hip, dev, A, b_dev, N, NA, hip.PCG_MIXED, args.steps, args.warmup, 20000, bnorm)
     log(f"fp64 pcg {fp64['ms']:.1f} ms ({fp64['iters']} it, true {fp64['true_relres']:.2e}); "
         f"mixed {mixed['ms']:.1f} ms ({mixed['iters']} it, true {mixed['true_relres']:.2e}); "
         f"speedup {fp64['ms'] / mixed['ms']:.2f}x")
