@@ -29,15 +29,19 @@ namespace gip {
 
 __global__ __launch_bounds__(kBlock) void k_energy(int n, const double *__restrict__ vol, const double *__restrict__ u,
                                                    const double *__restrict__ v, const double *__restrict__ w,
-                                                   double *__restrict__ part, unsigned int *__restrict__ cnt,
-                                                   double *__restrict__ out) {
+                                                   double *__restrict__ part) {
     double acc = 0.0;
     const int stride = gridDim.x * kBlock;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
         acc = acc + vol[i] * (u[i] * u[i] + v[i] * v[i] + w[i] * w[i]);
     acc = block_reduce<RED_SUM>(acc);
-    double tot;
-    if (last_block_reduce<RED_SUM>(acc, part, cnt, &tot) && threadIdx.x == 0) *out = 0.5 * tot;
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// one workgroup: J = 0.5 * (sum of the partials in index order)
+__global__ __launch_bounds__(kBlock) void k_energy_fin(const double *__restrict__ part, int np, double *__restrict__ out) {
+    const double tot = sum_partials<RED_SUM>(part, np);
+    if (threadIdx.x == 0) *out = 0.5 * tot;
 }
 
 // seed: y_i = V_i x_i on the inner cells
@@ -102,7 +106,6 @@ struct gip_adjoint {
     double *u = nullptr, *v = nullptr, *w = nullptr, *ax0 = nullptr, *ay0 = nullptr, *az0 = nullptr, *mf = nullptr,
            *p0 = nullptr, *p = nullptr, *ax = nullptr, *ay = nullptr, *az = nullptr, *rhs = nullptr, *gx = nullptr,
            *gy = nullptr, *gz = nullptr, *scal = nullptr, *part = nullptr;
-    unsigned int *cnt = nullptr;
     // adjoint state and work
     double *bu = nullptr, *bv = nullptr, *bw = nullptr, *bax0 = nullptr, *bay0 = nullptr, *baz0 = nullptr,
            *bmf = nullptr, *bgx = nullptr, *bgy = nullptr, *bgz = nullptr, *bpe = nullptr, *bp = nullptr,
@@ -320,12 +323,6 @@ int gip_adjoint_create(gip_ctx *c, const gip_cavity *cav, const gip_adjoint_opts
     if (!rc) rc = alloc(a, &a->tmp, std::max(std::max(NA, F), (size_t)a->NNZC));
     if (!rc) rc = alloc(a, &a->scal, 8);
     if (!rc) rc = alloc(a, &a->part, kMaxRedBlocks);
-    if (!rc) {
-        hipError_t e = hipMalloc(&a->cnt, 4 * sizeof(unsigned int));
-        if (e == hipSuccess) a->allocs.push_back(a->cnt);
-        if (e == hipSuccess) e = hipMemsetAsync(a->cnt, 0, 4 * sizeof(unsigned int), c->stream);
-        if (e != hipSuccess) rc = hip_fail(e, "adjoint counters");
-    }
     // boundary faces: rows of SO without entries
     std::vector<int32_t> bmask(F, 0);
     for (size_t fi = 0; fi < F; ++fi) bmask[fi] = cav->SO.rowptr[fi + 1] == cav->SO.rowptr[fi];
@@ -388,7 +385,8 @@ int gip_adjoint_forward(gip_adjoint *a, const double *lid, const double *u0, con
         if (!a->frozen && n + 1 < a->o.steps) TRY(gip_cfl(c, N, a->u, a->v, a->w, a->d_dxs, a->o.gamma, a->o.rho, &dt));
     }
     a->frozen = true;
-    k_energy<<<reduce_grid(N), kBlock, 0, c->stream>>>(N, a->d_vol, a->u, a->v, a->w, a->part, a->cnt, a->scal);
+    k_energy<<<reduce_grid(N), kBlock, 0, c->stream>>>(N, a->d_vol, a->u, a->v, a->w, a->part);
+    k_energy_fin<<<1, kBlock, 0, c->stream>>>(a->part, reduce_grid(N), a->scal);
     GIP_LAUNCH_CHECK("objective");
     double h = 0.0;
     TRY(gip_memcpy_d2h(c, &h, a->scal, sizeof(double)));

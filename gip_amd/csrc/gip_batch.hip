@@ -36,35 +36,32 @@ __device__ inline double fac_of(const double *fac, int inst, int code) {
     return code == FAC_ONE ? 1.0 : code == FAC_ZERO ? 0.0 : fac[inst * F_COUNT + code];
 }
 
-// Reduce K per-thread values over chunk c of instance inst; in the workgroup
-// that completes the instance, thread 0 receives the instance totals
-// (partials combined in chunk order) and true is returned to every thread.
+// Segmented reductions run as two launches: the chunk kernel stores each
+// chunk's workgroup totals (seg_store), then a finalize kernel with one
+// workgroup per instance combines that instance's chunk totals in chunk
+// order (seg_total) and applies the per-instance update.  The kernel
+// boundary orders the stores before the loads: no atomics, no device-scope
+// fences.
 template <int K, int OP>
-__device__ inline bool seg_reduce(double (&v)[K], int c, int inst, const Seg &S, double *part, unsigned int *cnt,
-                                  double (&out)[K]) {
-    __shared__ unsigned int s_t;
+__device__ inline void seg_store(double (&v)[K], int c, double *part) {
 #pragma unroll
     for (int k = 0; k < K; ++k) v[k] = block_reduce<OP>(v[k]);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < K; ++k) part[(size_t)c * K + k] = v[k];
-        __threadfence();
-        s_t = atomicAdd(&cnt[inst], 1u);
     }
-    __syncthreads();
-    if (s_t != (unsigned int)S.inst_nch[inst] - 1u) return false;
-    __threadfence();
+}
+
+// Instance totals of the chunk totals (valid in thread 0).
+template <int K, int OP>
+__device__ inline void seg_total(const double *part, int inst, const Seg &S, double (&out)[K]) {
     const int c0 = S.inst_c0[inst], nch = S.inst_nch[inst];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         double acc = RedOp<OP>::identity();
-        for (int j = threadIdx.x; j < nch; j += kBlock)
-            acc = RedOp<OP>::combine(acc, __hip_atomic_load(&part[(size_t)(c0 + j) * K + k], __ATOMIC_RELAXED,
-                                                            __HIP_MEMORY_SCOPE_AGENT));
+        for (int j = threadIdx.x; j < nch; j += kBlock) acc = RedOp<OP>::combine(acc, part[(size_t)(c0 + j) * K + k]);
         out[k] = block_reduce<OP>(acc);
     }
-    if (threadIdx.x == 0) __hip_atomic_store(&cnt[inst], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
 }
 
 #define FOR_CHUNKS(S, c) for (int c = blockIdx.x; c < (S).nchunk; c += gridDim.x)
@@ -125,27 +122,30 @@ __global__ __launch_bounds__(kBlock) void k_b_copy(int n, double *__restrict__ d
 }
 
 // massr_i = ||temp||_inf over instance i's inner rows; then time_i += dt_i (GIP_DNS.h:398-401)
-__global__ __launch_bounds__(kBlock) void k_b_massr(const double *__restrict__ temp, Seg S, double *part,
-                                                    unsigned int *cnt, double *__restrict__ massr,
-                                                    double *__restrict__ time, const double *__restrict__ dt,
-                                                    double *__restrict__ dt_used) {
+__global__ __launch_bounds__(kBlock) void k_b_massr(const double *__restrict__ temp, Seg S, double *part) {
     FOR_CHUNKS(S, c) {
-        const int inst = S.chunk_inst[c];
-        double v[1] = {fabs(temp[c * kChunk + threadIdx.x])}, out[1];
-        if (seg_reduce<1, RED_MAXABS>(v, c, inst, S, part, cnt, out) && threadIdx.x == 0) {
-            massr[inst] = out[0];
-            dt_used[inst] = dt[inst];
-            time[inst] += dt[inst];
-        }
+        double v[1] = {fabs(temp[c * kChunk + threadIdx.x])};
+        seg_store<1, RED_MAXABS>(v, c, part);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_b_massr_fin(const double *part, Seg S, double *__restrict__ massr,
+                                                        double *__restrict__ time, const double *__restrict__ dt,
+                                                        double *__restrict__ dt_used) {
+    const int inst = blockIdx.x;
+    double out[1];
+    seg_total<1, RED_MAXABS>(part, inst, S, out);
+    if (threadIdx.x == 0) {
+        massr[inst] = out[0];
+        dt_used[inst] = dt[inst];
+        time[inst] += dt[inst];
     }
 }
 
 // dt_i = CFL over instance i's inner rows (GIP_NonLinearMulticore.cpp:3-55), sigma_i = 1/dt_i
 __global__ __launch_bounds__(kBlock) void k_b_cfl(const double *__restrict__ u, const double *__restrict__ v,
                                                   const double *__restrict__ w, const double *__restrict__ dxs, Seg S,
-                                                  const double *__restrict__ gamma, double rho, double *part,
-                                                  unsigned int *cnt, double *__restrict__ dt,
-                                                  double *__restrict__ sigma) {
+                                                  const double *__restrict__ gamma, double rho, double *part) {
     FOR_CHUNKS(S, c) {
         const int inst = S.chunk_inst[c], i = c * kChunk + threadIdx.x;
         const double g = gamma[inst];
@@ -161,12 +161,20 @@ __global__ __launch_bounds__(kBlock) void k_b_cfl(const double *__restrict__ u, 
             double cc = 0.2 * dx * dx * rho / g;
             if (acc > cc) acc = cc;
         }
-        double vv[1] = {acc}, out[1];
-        if (seg_reduce<1, RED_MIN>(vv, c, inst, S, part, cnt, out) && threadIdx.x == 0) {
-            double d = 0.8 * out[0] > 1e-1 ? 1e-1 : 0.8 * out[0];
-            dt[inst] = d;
-            sigma[inst] = 1.0 / d;
-        }
+        double vv[1] = {acc};
+        seg_store<1, RED_MIN>(vv, c, part);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_b_cfl_fin(const double *part, Seg S, double *__restrict__ dt,
+                                                      double *__restrict__ sigma) {
+    const int inst = blockIdx.x;
+    double out[1];
+    seg_total<1, RED_MIN>(part, inst, S, out);
+    if (threadIdx.x == 0) {
+        const double d = 0.8 * out[0] > 1e-1 ? 1e-1 : 0.8 * out[0];  // GIP_NonLinearMulticore.cpp:47-50
+        dt[inst] = d;
+        sigma[inst] = 1.0 / d;
     }
 }
 
@@ -180,8 +188,7 @@ struct BPcg {
     const float *dinv32;
     double *sc;  // B * B_COUNT
     int *state, *iters, *active;
-    double *part;
-    unsigned int *cnt;  // B counters per reduction site
+    double *part;  // chunk totals
     int maxit;
     double inner2;
 };
@@ -232,9 +239,9 @@ __global__ __launch_bounds__(kBlock) void k_bp_setup_resid(EllView A, BPcg d, Se
 
 // s = Dinv r (the first direction, formed by the first fused iteration with
 // beta = 0); rho = r.s, b.b, r.r; state (GIP_LinearSolver.h:72-80)
-__global__ __launch_bounds__(kBlock) void k_bp_setup_dir(BPcg d, Seg S, double tol, int precond) {
+__global__ __launch_bounds__(kBlock) void k_bp_setup_dir(BPcg d, Seg S, int precond) {
     FOR_CHUNKS(S, c) {
-        const int row = c * kChunk + threadIdx.x, inst = S.chunk_inst[c];
+        const int row = c * kChunk + threadIdx.x;
         const double ri = d.r[row];
         double si;
         if (precond) {
@@ -244,16 +251,23 @@ __global__ __launch_bounds__(kBlock) void k_bp_setup_dir(BPcg d, Seg S, double t
             si = ri;
         }
         d.s[row] = si;
-        double v[3] = {ri * si, d.b[row] * d.b[row], ri * ri}, out[3];
-        if (seg_reduce<3, RED_SUM>(v, c, inst, S, d.part, d.cnt, out) && threadIdx.x == 0) {
-            double *q = bsc(d, inst);
-            q[B_RHO] = out[0];
-            q[B_BB] = out[1];
-            q[B_RR] = out[2];
-            q[B_TOL2] = tol * tol * out[1];
-            q[B_BETA] = 0.0;
-            if (out[2] <= q[B_TOL2]) leave(d, inst, BS_CONV);
-        }
+        double v[3] = {ri * si, d.b[row] * d.b[row], ri * ri};
+        seg_store<3, RED_SUM>(v, c, d.part);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bp_setup_fin(BPcg d, Seg S, double tol) {
+    const int inst = blockIdx.x;
+    double out[3];
+    seg_total<3, RED_SUM>(d.part, inst, S, out);
+    if (threadIdx.x == 0) {
+        double *q = bsc(d, inst);
+        q[B_RHO] = out[0];
+        q[B_BB] = out[1];
+        q[B_RR] = out[2];
+        q[B_TOL2] = tol * tol * out[1];
+        q[B_BETA] = 0.0;
+        if (out[2] <= q[B_TOL2]) leave(d, inst, BS_CONV);
     }
 }
 
@@ -269,13 +283,22 @@ __global__ __launch_bounds__(kBlock) void k_bp_spmv(EllView A, BPcg d, Seg S, co
         p_new[row] = pi;
         double sum;
         if (row_dot<W, double, double>(A, A.val, po, row, sum)) d.Ap[row] = sum;
-        double v[1] = {pi * d.Ap[row]}, out[1];
-        if (seg_reduce<1, RED_SUM>(v, c, inst, S, d.part, d.cnt, out) && threadIdx.x == 0) {
-            double *q = bsc(d, inst);
-            const double alpha = q[B_RHO] / out[0];
-            q[B_ALPHA] = alpha;
-            if (out[0] == 0.0 || !isfinite(out[0]) || !isfinite(alpha)) leave(d, inst, BS_BREAK);
-        }
+        double v[1] = {pi * d.Ap[row]};
+        seg_store<1, RED_SUM>(v, c, d.part);
+    }
+}
+
+// alpha = rho / p.Ap of the running instances (FP64 and MIXED)
+__global__ __launch_bounds__(kBlock) void k_bp_spmv_fin(BPcg d, Seg S) {
+    const int inst = blockIdx.x;
+    if (d.state[inst] != BS_RUN) return;
+    double out[1];
+    seg_total<1, RED_SUM>(d.part, inst, S, out);
+    if (threadIdx.x == 0) {
+        double *q = bsc(d, inst);
+        const double alpha = q[B_RHO] / out[0];
+        q[B_ALPHA] = alpha;
+        if (out[0] == 0.0 || !isfinite(out[0]) || !isfinite(alpha)) leave(d, inst, BS_BREAK);
     }
 }
 
@@ -297,43 +320,59 @@ __global__ __launch_bounds__(kBlock) void k_bp_update(BPcg d, Seg S, const doubl
             si = ri;
         }
         d.s[row] = si;
-        double v[2] = {ri * si, ri * ri}, out[2];
-        if (seg_reduce<2, RED_SUM>(v, c, inst, S, d.part, d.cnt + d.B, out) && threadIdx.x == 0) {
-            const double r0 = q[B_RHO];
-            q[B_RHO] = out[0];
-            q[B_BETA] = out[0] / r0;
-            q[B_RR] = out[1];
-            const int it = ++d.iters[inst];
-            if (out[1] <= q[B_TOL2])
-                leave(d, inst, BS_CONV);
-            else if (it >= d.maxit)
-                leave(d, inst, BS_MAXED);
-        }
+        double v[2] = {ri * si, ri * ri};
+        seg_store<2, RED_SUM>(v, c, d.part);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bp_update_fin(BPcg d, Seg S) {
+    const int inst = blockIdx.x;
+    if (d.state[inst] != BS_RUN) return;
+    double out[2];
+    seg_total<2, RED_SUM>(d.part, inst, S, out);
+    if (threadIdx.x == 0) {
+        double *q = bsc(d, inst);
+        const double r0 = q[B_RHO];
+        q[B_RHO] = out[0];
+        q[B_BETA] = out[0] / r0;
+        q[B_RR] = out[1];
+        const int it = ++d.iters[inst];
+        if (out[1] <= q[B_TOL2])
+            leave(d, inst, BS_CONV);
+        else if (it >= d.maxit)
+            leave(d, inst, BS_MAXED);
     }
 }
 
 // mixed: FP32 copies of the FP64 residual of x0; rho, b.b, true r.r
-__global__ __launch_bounds__(kBlock) void k_bpm_setup(BPcg d, Seg S, double tol, int precond) {
+__global__ __launch_bounds__(kBlock) void k_bpm_setup(BPcg d, Seg S, int precond) {
     FOR_CHUNKS(S, c) {
-        const int row = c * kChunk + threadIdx.x, inst = S.chunk_inst[c];
+        const int row = c * kChunk + threadIdx.x;
         const double ri = d.r[row];
         const float r32 = (float)ri;
         const float s32 = precond ? r32 * d.dinv32[row] : r32;
         d.r32[row] = r32;
         d.s32[row] = s32;
         d.xc32[row] = 0.0f;
-        double v[3] = {(double)r32 * (double)s32, d.b[row] * d.b[row], ri * ri}, out[3];
-        if (seg_reduce<3, RED_SUM>(v, c, inst, S, d.part, d.cnt, out) && threadIdx.x == 0) {
-            double *q = bsc(d, inst);
-            q[B_RHO] = out[0];
-            q[B_BB] = out[1];
-            q[B_RR] = out[2];
-            q[B_LAST_RR] = out[2];
-            q[B_TOL2] = tol * tol * out[1];
-            q[B_BETA] = 0.0;
-            q[B_NEED] = 0.0;
-            if (out[2] <= q[B_TOL2]) leave(d, inst, BS_CONV);
-        }
+        double v[3] = {(double)r32 * (double)s32, d.b[row] * d.b[row], ri * ri};
+        seg_store<3, RED_SUM>(v, c, d.part);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bpm_setup_fin(BPcg d, Seg S, double tol) {
+    const int inst = blockIdx.x;
+    double out[3];
+    seg_total<3, RED_SUM>(d.part, inst, S, out);
+    if (threadIdx.x == 0) {
+        double *q = bsc(d, inst);
+        q[B_RHO] = out[0];
+        q[B_BB] = out[1];
+        q[B_RR] = out[2];
+        q[B_LAST_RR] = out[2];
+        q[B_TOL2] = tol * tol * out[1];
+        q[B_BETA] = 0.0;
+        q[B_NEED] = 0.0;
+        if (out[2] <= q[B_TOL2]) leave(d, inst, BS_CONV);
     }
 }
 
@@ -350,13 +389,8 @@ __global__ __launch_bounds__(kBlock) void k_bpm_spmv(EllView A, BPcg d, Seg S, c
         float sum;
         row_dot<W, float, float>(A, A.val32, po, row, sum);
         d.Ap32[row] = sum;
-        double v[1] = {(double)pi * (double)sum}, out[1];
-        if (seg_reduce<1, RED_SUM>(v, c, inst, S, d.part, d.cnt, out) && threadIdx.x == 0) {
-            double *q = bsc(d, inst);
-            const double alpha = q[B_RHO] / out[0];
-            q[B_ALPHA] = alpha;
-            if (out[0] == 0.0 || !isfinite(out[0]) || !isfinite(alpha)) leave(d, inst, BS_BREAK);
-        }
+        double v[1] = {(double)pi * (double)sum};
+        seg_store<1, RED_SUM>(v, c, d.part);
     }
 }
 
@@ -372,16 +406,25 @@ __global__ __launch_bounds__(kBlock) void k_bpm_update(BPcg d, Seg S, const floa
         d.r32[row] = ri;
         const float si = precond ? ri * d.dinv32[row] : ri;
         d.s32[row] = si;
-        double v[2] = {(double)ri * (double)si, (double)ri * (double)ri}, out[2];
-        if (seg_reduce<2, RED_SUM>(v, c, inst, S, d.part, d.cnt + d.B, out) && threadIdx.x == 0) {
-            const double r0 = q[B_RHO];
-            q[B_RHO_PREV] = r0;
-            q[B_RHO] = out[0];
-            q[B_BETA] = out[0] / r0;
-            q[B_RR] = out[1];
-            const int it = ++d.iters[inst];
-            if (out[1] <= d.inner2 * q[B_LAST_RR] || out[1] <= q[B_TOL2] || it >= d.maxit) q[B_NEED] = 1.0;
-        }
+        double v[2] = {(double)ri * (double)si, (double)ri * (double)ri};
+        seg_store<2, RED_SUM>(v, c, d.part);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bpm_update_fin(BPcg d, Seg S) {
+    const int inst = blockIdx.x;
+    if (d.state[inst] != BS_RUN) return;
+    double out[2];
+    seg_total<2, RED_SUM>(d.part, inst, S, out);
+    if (threadIdx.x == 0) {
+        double *q = bsc(d, inst);
+        const double r0 = q[B_RHO];
+        q[B_RHO_PREV] = r0;
+        q[B_RHO] = out[0];
+        q[B_BETA] = out[0] / r0;
+        q[B_RR] = out[1];
+        const int it = ++d.iters[inst];
+        if (out[1] <= d.inner2 * q[B_LAST_RR] || out[1] <= q[B_TOL2] || it >= d.maxit) q[B_NEED] = 1.0;
     }
 }
 
@@ -414,20 +457,28 @@ __global__ __launch_bounds__(kBlock) void k_bpm_resid(EllView A, BPcg d, Seg S, 
         const float s32 = precond ? r32 * d.dinv32[row] : r32;
         d.r32[row] = r32;
         d.s32[row] = s32;
-        double v[2] = {(double)r32 * (double)s32, ri * ri}, out[2];
-        if (seg_reduce<2, RED_SUM>(v, c, inst, S, d.part, d.cnt + 2 * d.B, out) && threadIdx.x == 0) {
-            double *q = bsc(d, inst);
-            // keep the direction unless the FP32 recursion drifted >10x from the true residual
-            q[B_BETA] = out[1] > 100.0 * q[B_RR] ? 0.0 : out[0] / q[B_RHO_PREV];
-            q[B_RHO] = out[0];
-            q[B_RR] = out[1];
-            q[B_LAST_RR] = out[1];
-            q[B_NEED] = 0.0;
-            if (out[1] <= q[B_TOL2])
-                leave(d, inst, BS_CONV);
-            else if (d.iters[inst] >= d.maxit)
-                leave(d, inst, BS_MAXED);
-        }
+        double v[2] = {(double)r32 * (double)s32, ri * ri};
+        seg_store<2, RED_SUM>(v, c, d.part);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bpm_resid_fin(BPcg d, Seg S) {
+    const int inst = blockIdx.x;
+    if (d.state[inst] != BS_RUN || bsc(d, inst)[B_NEED] == 0.0) return;
+    double out[2];
+    seg_total<2, RED_SUM>(d.part, inst, S, out);
+    if (threadIdx.x == 0) {
+        double *q = bsc(d, inst);
+        // keep the direction unless the FP32 recursion drifted >10x from the true residual
+        q[B_BETA] = out[1] > 100.0 * q[B_RR] ? 0.0 : out[0] / q[B_RHO_PREV];
+        q[B_RHO] = out[0];
+        q[B_RR] = out[1];
+        q[B_LAST_RR] = out[1];
+        q[B_NEED] = 0.0;
+        if (out[1] <= q[B_TOL2])
+            leave(d, inst, BS_CONV);
+        else if (d.iters[inst] >= d.maxit)
+            leave(d, inst, BS_MAXED);
     }
 }
 
@@ -479,7 +530,6 @@ struct gip_batch {
     float *r32 = nullptr, *s32 = nullptr, *Ap32 = nullptr, *xc32 = nullptr, *dinv32 = nullptr, *pq[2] = {};
     double *sc = nullptr, *part = nullptr;
     int *state = nullptr, *iters = nullptr, *active = nullptr, *h_active = nullptr;
-    unsigned int *cnt = nullptr;
     bool started = false;
     hipGraphExec_t g_pre = nullptr, g_iter = nullptr, g_post = nullptr;
     int grid_in = 0, grid_f = 0, grid_all = 0;
@@ -613,7 +663,6 @@ BPcg pcg_view(gip_batch *b) {
     d.iters = b->iters;
     d.active = b->active;
     d.part = b->part;
-    d.cnt = b->cnt;
     d.maxit = b->o.pcg_maxit;
     const double it = b->o.inner_tol > 0 ? b->o.inner_tol : 0.1;
     d.inner2 = it * it;
@@ -652,10 +701,13 @@ int enqueue_pre(gip_batch *b) {
     k_bp_init<<<(b->B + 63) / 64, 64, 0, st>>>(d);
     BDISPATCH(L, b->grid_in, k_bp_setup_resid, view_of(L), d, b->seg_in, b->p0, b->pp[0], b->pp[1],
               mixed ? b->pq[0] : nullptr, mixed ? b->pq[1] : nullptr);
-    if (mixed)
-        k_bpm_setup<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, b->o.pcg_tol, 1);
-    else
-        k_bp_setup_dir<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, b->o.pcg_tol, 1);
+    if (mixed) {
+        k_bpm_setup<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, 1);
+        k_bpm_setup_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in, b->o.pcg_tol);
+    } else {
+        k_bp_setup_dir<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, 1);
+        k_bp_setup_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in, b->o.pcg_tol);
+    }
     (void)ALLB;
     GIP_LAUNCH_CHECK("batch pcg setup");
     return GIP_OK;
@@ -672,15 +724,20 @@ int enqueue_iter(gip_batch *b) {
         const int cur = k & 1, old = cur ^ 1;
         if (mixed) {
             BDISPATCH(L, b->grid_in, k_bpm_spmv, view_of(L), d, b->seg_in, b->pq[old], b->pq[cur]);
+            k_bp_spmv_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in);
             k_bpm_update<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, b->pq[cur], 1);
+            k_bpm_update_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in);
         } else {
             BDISPATCH(L, b->grid_in, k_bp_spmv, view_of(L), d, b->seg_in, b->pp[old], b->pp[cur]);
+            k_bp_spmv_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in);
             k_bp_update<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, b->pp[cur], 1);
+            k_bp_update_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in);
         }
     }
     if (mixed) {
         k_bpm_fold<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in);
         BDISPATCH(L, b->grid_in, k_bpm_resid, view_of(L), d, b->seg_in, 1);
+        k_bpm_resid_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in);
     }
     GIP_LAUNCH_CHECK("batch pcg iterations");
     return GIP_OK;
@@ -710,10 +767,10 @@ int enqueue_post(gip_batch *b) {
     if (!rc) rc = spmv_launch(b->c, b->op[OP_SFZ], b->w, b->mf, b->rho, 1.0, 1, nullptr);
     if (!rc) rc = spmv_launch(b->c, b->op[OP_MRES], b->mf, b->temp, 1.0, 0.0, 0, nullptr);
     if (rc) return rc;
-    k_b_massr<<<b->grid_in, kBlock, 0, st>>>(b->temp, b->seg_in, b->part, b->cnt + 3 * b->B, b->massr, b->time, b->dt,
-                                             b->dt_used);
-    k_b_cfl<<<b->grid_in, kBlock, 0, st>>>(b->u, b->v, b->w, b->dxs, b->seg_in, b->gamma, b->rho, b->part,
-                                           b->cnt + 4 * b->B, b->dt, b->sigma);
+    k_b_massr<<<b->grid_in, kBlock, 0, st>>>(b->temp, b->seg_in, b->part);
+    k_b_massr_fin<<<b->B, kBlock, 0, st>>>(b->part, b->seg_in, b->massr, b->time, b->dt, b->dt_used);
+    k_b_cfl<<<b->grid_in, kBlock, 0, st>>>(b->u, b->v, b->w, b->dxs, b->seg_in, b->gamma, b->rho, b->part);
+    k_b_cfl_fin<<<b->B, kBlock, 0, st>>>(b->part, b->seg_in, b->dt, b->sigma);
     GIP_LAUNCH_CHECK("batch fluxes and CFL");
     return GIP_OK;
 }
@@ -831,7 +888,6 @@ int gip_batch_create(gip_ctx *c, int32_t ninst, const gip_cavity *const *cavs, c
     if (!rc) rc = dalloc_t(b, &b->state, (size_t)B);
     if (!rc) rc = dalloc_t(b, &b->iters, (size_t)B);
     if (!rc) rc = dalloc_t(b, &b->active, 1);
-    if (!rc) rc = dalloc_t(b, &b->cnt, (size_t)B * 8);
     if (!rc) {
         hipError_t e = hipHostMalloc((void **)&b->h_active, sizeof(int), hipHostMallocDefault);
         if (e != hipSuccess) rc = hip_fail(e, "hipHostMalloc");
@@ -913,8 +969,8 @@ int gip_batch_step(gip_batch *b) {
     hipStream_t st = b->c->stream;
     int rc;
     if (!b->started) {  // initial CFL (GIP_DNS.h:101-102)
-        k_b_cfl<<<b->grid_in, kBlock, 0, st>>>(b->u, b->v, b->w, b->dxs, b->seg_in, b->gamma, b->rho, b->part,
-                                               b->cnt + 4 * b->B, b->dt, b->sigma);
+        k_b_cfl<<<b->grid_in, kBlock, 0, st>>>(b->u, b->v, b->w, b->dxs, b->seg_in, b->gamma, b->rho, b->part);
+        k_b_cfl_fin<<<b->B, kBlock, 0, st>>>(b->part, b->seg_in, b->dt, b->sigma);
         GIP_LAUNCH_CHECK("batch initial CFL");
         b->started = true;
     }

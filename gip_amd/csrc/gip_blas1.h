@@ -109,6 +109,21 @@ __device__ inline bool last_block_reduce2(double v0, double v1, double *partials
     return true;
 }
 
+// Combine n partials in index order (thread t takes t, t+256, ..., then the
+// fixed tree): the same value in every workgroup that calls it.  Used in
+// kernel prologues to read the previous kernel's per-workgroup partials
+// without atomics or cross-XCD fences (the kernel boundary orders them).
+template <int OP>
+__device__ inline double sum_partials(const double *__restrict__ part, int n) {
+    __shared__ double s_all;
+    double acc = RedOp<OP>::identity();
+    for (int i = threadIdx.x; i < n; i += kBlock) acc = RedOp<OP>::combine(acc, part[i]);
+    acc = block_reduce<OP>(acc);
+    if (threadIdx.x == 0) s_all = acc;
+    __syncthreads();
+    return s_all;
+}
+
 int launch_reduce(gip_ctx *c, int op, int n, const double *x, const double *y, double *dres);
 
 }  // namespace gip

@@ -24,17 +24,26 @@ __global__ __launch_bounds__(kBlock) void k_axpby(int n, double a, const double 
     if (i < n) y[i] = a * x[i] + b * y[i];
 }
 
+// Reductions run as two launches: every workgroup stores its partial, then
+// one workgroup combines the partials in index order.  The kernel boundary
+// orders the two, so no atomics or device-wide fences are needed (a fence
+// per workgroup costs an L2 write-back on every XCD).
 template <int OP>
 __global__ __launch_bounds__(kBlock) void k_reduce(int n, const double *__restrict__ x, const double *__restrict__ y,
-                                                   double *__restrict__ partials, unsigned int *__restrict__ counter,
-                                                   double *__restrict__ result) {
+                                                   double *__restrict__ partials) {
     double acc = RedOp<OP>::identity();
     const int stride = gridDim.x * kBlock;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) acc = RedOp<OP>::combine(acc, RedOp<OP>::term(x, y, i));
     acc = block_reduce<OP>(acc);
-    if (last_block_reduce<OP>(acc, partials, counter, &acc)) {
-        if (threadIdx.x == 0) *result = acc;
-    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+// one workgroup: combine `np` partials; CFL = 1 applies GIP_NonLinearMulticore.cpp:47-50
+template <int OP, int CFL>
+__global__ __launch_bounds__(kBlock) void k_finalize(const double *__restrict__ partials, int np,
+                                                    double *__restrict__ result) {
+    const double acc = sum_partials<OP>(partials, np);
+    if (threadIdx.x == 0) *result = CFL ? (0.8 * acc > 1e-1 ? 1e-1 : 0.8 * acc) : acc;
 }
 
 struct CflArgs {
@@ -42,8 +51,7 @@ struct CflArgs {
     double gamma, rho;
 };
 
-__global__ __launch_bounds__(kBlock) void k_cfl(int n, CflArgs a, double *__restrict__ partials,
-                                                unsigned int *__restrict__ counter, double *__restrict__ result) {
+__global__ __launch_bounds__(kBlock) void k_cfl(int n, CflArgs a, double *__restrict__ partials) {
     // GIP_NonLinearMulticore.cpp:14-43: the running minimum starts at 1e20 and
     // takes each admissible convective and diffusive limit.
     double acc = 1e20;
@@ -62,9 +70,7 @@ __global__ __launch_bounds__(kBlock) void k_cfl(int n, CflArgs a, double *__rest
         }
     }
     acc = block_reduce<RED_MIN>(acc);
-    if (last_block_reduce<RED_MIN>(acc, partials, counter, &acc)) {
-        if (threadIdx.x == 0) *result = 0.8 * acc > 1e-1 ? 1e-1 : 0.8 * acc;  // :47-50
-    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 }
 
 static inline int blocks_for(int n) { return (n + kBlock - 1) / kBlock; }
@@ -72,13 +78,14 @@ static inline int blocks_for(int n) { return (n + kBlock - 1) / kBlock; }
 int launch_reduce(gip_ctx *c, int op, int n, const double *x, const double *y, double *dres) {
     int g = reduce_grid(n);
     double *part = c->d_partials;
-    unsigned int *cnt = c->d_counter + op;
     switch (op) {
         case RED_SUM:
-            k_reduce<RED_SUM><<<g, kBlock, 0, c->stream>>>(n, x, y, part, cnt, dres);
+            k_reduce<RED_SUM><<<g, kBlock, 0, c->stream>>>(n, x, y, part);
+            k_finalize<RED_SUM, 0><<<1, kBlock, 0, c->stream>>>(part, g, dres);
             break;
         case RED_MAXABS:
-            k_reduce<RED_MAXABS><<<g, kBlock, 0, c->stream>>>(n, x, y, part, cnt, dres);
+            k_reduce<RED_MAXABS><<<g, kBlock, 0, c->stream>>>(n, x, y, part);
+            k_finalize<RED_MAXABS, 0><<<1, kBlock, 0, c->stream>>>(part, g, dres);
             break;
         default:
             return set_error(GIP_EINVAL, "unknown reduction");
@@ -153,7 +160,8 @@ int gip_cfl(gip_ctx *c, int32_t n, const double *u, const double *v, const doubl
     if (!c || n < 0 || (n && (!u || !v || !w || !dxs)) || !dt) return set_error(GIP_EINVAL, "gip_cfl: bad argument");
     CflArgs a{u, v, w, dxs, gamma, rho};
     int g = reduce_grid(n);
-    k_cfl<<<g, kBlock, 0, c->stream>>>(n, a, c->d_partials, c->d_counter + RED_MIN, c->d_scalars + kSlotResult);
+    k_cfl<<<g, kBlock, 0, c->stream>>>(n, a, c->d_partials);
+    k_finalize<RED_MIN, 1><<<1, kBlock, 0, c->stream>>>(c->d_partials, g, c->d_scalars + kSlotResult);
     GIP_LAUNCH_CHECK("gip_cfl");
     return fetch_result(c, c->d_scalars + kSlotResult, dt);
 }

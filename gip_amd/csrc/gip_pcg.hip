@@ -1,26 +1,38 @@
 // Fused, device-resident Jacobi-PCG of libgip_hip (GIP_LinearSolver.h:45-117).
 //
 // FP64 iteration k, two kernels, the reference recurrences and roundings:
-//   K1  p_k = 1.0 s + beta p_{k-1}   (k >= 2; formed on the fly for every
+//   K1  rho_k = sum of the r.s partials, beta = rho_k / rho_{k-1};
+//       p_k = 1.0 s + beta p_{k-1}   (k >= 1; formed on the fly for every
 //       gathered column, stored once for the own rows; the first iteration
-//       uses p_1 from the setup), Ap = A p_k, dn = p_k.Ap -> alpha = r1/dn
-//   K2  x += alpha p_k ; r += (-alpha) Ap ; s = Dinv r ; r.s, r.r
-//       -> beta = r1'/r1, convergence, iteration count
+//       uses p_0 from the setup), Ap = A p_k, partials of p_k.Ap
+//   K2  dn = sum of the p.Ap partials, alpha = rho_k / dn;
+//       x += alpha p_k ; r += (-alpha) Ap ; s = Dinv r ; partials of r.s, r.r
+// Cross-workgroup communication is fence-free: every workgroup stores its
+// partial sums with plain stores, and every workgroup of the NEXT kernel
+// re-reduces all partials in its prologue in the same fixed order, so all
+// workgroups compute bit-identical scalars.  The kernel boundary orders the
+// stores before the loads; no atomics, no device-scope fences (a fence per
+// workgroup costs an L2 write-back on every XCD).  Scalars that must survive
+// a kernel (rho of the previous iteration, the state word) are written by
+// workgroup 0 only, into slots nobody reads in the same kernel (rho uses two
+// slots by iteration parity).
 // The direction lives in two buffers used alternately (K1 of iteration k
 // reads p_{k-1} from one and writes p_k to the other), so no workgroup can
 // see a half-updated direction.  Forming p on the fly in K1 computes
 // exactly the expression the reference stores (GIP_LinearSolver.h:113).
-// Every scalar lives in device memory; every kernel first reads the
-// solver's state word and returns when the solve has finished, so the host
-// queues many iterations and polls the state only now and then.
+// Every kernel first reads the solver's state word and returns when the
+// solve has finished, so the host queues many iterations and polls the
+// state only now and then.  Convergence ||r|| <= tol ||b|| is tested in the
+// prologue of K1 (and by a one-workgroup kernel after the last iteration).
 //
 // MIXED: the same two kernels in FP32 (operator values, vectors and SpMV;
 // dot products accumulate in FP64) on the correction equation, inside an
 // FP64 defect-correction loop: when the FP32 residual has dropped by
 // inner_tol (or reaches the target) the FP32 correction is added to the
 // FP64 x, the FP64 true residual b - A x replaces the FP32 residual (the
-// search direction is kept: residual replacement), and convergence is
-// decided on that FP64 residual only.
+// search direction is kept: reliable-update residual replacement, restarted
+// with beta = 0 when the recursive residual had drifted more than 10x from
+// the true one), and convergence is decided on that FP64 residual only.
 #include <cmath>
 #include <new>
 
@@ -33,34 +45,35 @@ namespace gip {
 
 // Device scalar slots of a solver.
 enum {
-    S_RHO = 0,       // r.s of the current iterate
-    S_ALPHA = 1,
-    S_BETA = 2,
-    S_RR = 3,        // r.r
-    S_BB = 4,        // b.b
-    S_DN = 5,
-    S_TOL2 = 6,      // (tol ||b||)^2, negative when no tolerance
-    S_TRUE_RR = 7,   // FP64 true residual of the last correction (MIXED)
-    S_RHO_PREV = 8,  // r.s before the last update (MIXED)
+    S_RHO0 = 0,     // rho of even iterations
+    S_RHO1 = 1,     // rho of odd iterations
+    S_RR = 3,       // r.r (FP64: last update; MIXED: last status or true residual)
+    S_BB = 4,       // b.b
+    S_TOL2 = 6,     // (tol ||b||)^2, negative when no tolerance
+    S_TRUE_RR = 7,  // FP64 true residual of the last correction (MIXED)
     S_COUNT = 16
 };
 // State word: 0 running, 1 converged, 2 breakdown.
 enum { ST_RUN = 0, ST_CONV = 1, ST_BREAK = 2 };
+// Partial-sum arrays, kMaxRedBlocks doubles each.
+enum { P_DN = 0, P_RS = 1, P_RR = 2, P_BB = 3, P_RS2 = 4, P_RR2 = 5, P_COUNT = 6 };
 
 struct PcgDev {
-    int n;
+    int n, G;  // solved rows, workgroups of every iteration kernel
     double *x, *r, *s, *Ap;
     const double *b, *dinv;
     float *r32, *s32, *Ap32, *xc32;
     const float *dinv32;
     double *sc;  // S_COUNT scalars
     int *state, *iters;
-    double *part;       // reduction partials
-    unsigned int *cnt;  // counters
+    double *part;  // P_COUNT x kMaxRedBlocks partials
     int precond;
+    __device__ double *P(int k) const { return part + k * kMaxRedBlocks; }
 };
 
 __device__ inline bool running(const PcgDev &d) { return *(volatile int *)d.state == ST_RUN; }
+__device__ inline bool leader() { return blockIdx.x == 0 && threadIdx.x == 0; }
+__device__ inline double rho_slot(const PcgDev &d, int k) { return d.sc[S_RHO0 + (k & 1)]; }
 
 // ---- setup ---------------------------------------------------------------
 
@@ -87,8 +100,8 @@ __global__ __launch_bounds__(kBlock) void k_setup_resid(EllView A, PcgDev d, con
     }
 }
 
-// p = Dinv r (or r); r1 = r.p, b.b, r.r  (:72-80); state = converged when ||r|| <= tol ||b||
-__global__ __launch_bounds__(kBlock) void k_setup_dir(PcgDev d, double *__restrict__ p, double tol) {
+// p = Dinv r (or r); partials of r.p, b.b, r.r  (:72-80)
+__global__ __launch_bounds__(kBlock) void k_setup_dir(PcgDev d, double *__restrict__ p) {
     double rs = 0.0, bb = 0.0, rr = 0.0;
     FOR_ROWS(d.n, row) {
         double ri = d.r[row];
@@ -105,20 +118,24 @@ __global__ __launch_bounds__(kBlock) void k_setup_dir(PcgDev d, double *__restri
         rr = rr + ri * ri;
     }
     rs = block_reduce<RED_SUM>(rs);
-    double out0, out1;
-    if (last_block_reduce<RED_SUM>(rs, d.part, d.cnt + 0, &out0)) {
-        if (threadIdx.x == 0) d.sc[S_RHO] = out0;
-    }
+    if (threadIdx.x == 0) d.P(P_RS)[blockIdx.x] = rs;
     bb = block_reduce<RED_SUM>(bb);
+    if (threadIdx.x == 0) d.P(P_BB)[blockIdx.x] = bb;
     rr = block_reduce<RED_SUM>(rr);
-    if (last_block_reduce2(bb, rr, d.part + kMaxRedBlocks, d.cnt + 1, &out0, &out1)) {
-        if (threadIdx.x == 0) {
-            d.sc[S_BB] = out0;
-            d.sc[S_RR] = out1;
-            double tol2 = tol > 0.0 ? tol * tol * out0 : -1.0;
-            d.sc[S_TOL2] = tol2;
-            *d.state = (tol > 0.0 && out1 <= tol2) ? ST_CONV : ST_RUN;
-        }
+    if (threadIdx.x == 0) d.P(P_RR)[blockIdx.x] = rr;
+}
+
+// one workgroup: b.b, r.r, the tolerance, the state word and the iteration count
+__global__ __launch_bounds__(kBlock) void k_setup_fin(PcgDev d, double tol) {
+    const double bb = sum_partials<RED_SUM>(d.P(P_BB), d.G);
+    const double rr = sum_partials<RED_SUM>(d.P(P_RR), d.G);
+    if (threadIdx.x == 0) {
+        const double tol2 = tol > 0.0 ? tol * tol * bb : -1.0;
+        d.sc[S_BB] = bb;
+        d.sc[S_RR] = rr;
+        d.sc[S_TOL2] = tol2;
+        *d.state = (tol > 0.0 && rr <= tol2) ? ST_CONV : ST_RUN;
+        *d.iters = 0;
     }
 }
 
@@ -126,9 +143,19 @@ __global__ __launch_bounds__(kBlock) void k_setup_dir(PcgDev d, double *__restri
 
 template <int W, bool FUSE>
 __global__ __launch_bounds__(kBlock) void k_iter_spmv(EllView A, PcgDev d, const double *__restrict__ p_old,
-                                                      double *__restrict__ p_new) {
+                                                      double *__restrict__ p_new, int k) {
     if (!running(d)) return;
-    const double beta = FUSE ? d.sc[S_BETA] : 0.0;
+    const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);  // :107
+    const double tol2 = d.sc[S_TOL2];
+    if (tol2 >= 0.0) {
+        const double rr = sum_partials<RED_SUM>(d.P(P_RR), d.G);
+        if (rr <= tol2) {  // every workgroup reaches the same decision
+            if (leader()) *d.state = ST_CONV;
+            return;
+        }
+    }
+    const double beta = FUSE ? rho / rho_slot(d, k - 1) : 0.0;  // :109
+    if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
     Operand<double> po{FUSE ? p_old : p_new, d.s, beta, d.n, FUSE};
     double acc = 0.0;
     FOR_ROWS(d.n, row) {
@@ -139,24 +166,21 @@ __global__ __launch_bounds__(kBlock) void k_iter_spmv(EllView A, PcgDev d, const
         acc = acc + pi * d.Ap[row];
     }
     acc = block_reduce<RED_SUM>(acc);
-    double dn;
-    if (last_block_reduce<RED_SUM>(acc, d.part, d.cnt + 0, &dn)) {
-        if (threadIdx.x == 0) {
-            double alpha = d.sc[S_RHO] / dn;  // :93
-            d.sc[S_DN] = dn;
-            d.sc[S_ALPHA] = alpha;
-            if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) *d.state = ST_BREAK;
-        }
-    }
+    if (threadIdx.x == 0) d.P(P_DN)[blockIdx.x] = acc;
 }
 
-__global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, const double *__restrict__ p) {
+__global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, const double *__restrict__ p, int k) {
     if (!running(d)) return;
-    const double alpha = d.sc[S_ALPHA];
+    const double dn = sum_partials<RED_SUM>(d.P(P_DN), d.G);
+    const double alpha = rho_slot(d, k) / dn;  // :93
+    if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) {
+        if (leader()) *d.state = ST_BREAK;
+        return;
+    }
     const double malpha = -1.0 * alpha;
     double rs = 0.0, rr = 0.0;
     FOR_ROWS(d.n, row) {
-        d.x[row] = d.x[row] + alpha * p[row];  // :95
+        d.x[row] = d.x[row] + alpha * p[row];       // :95
         double ri = d.r[row] + malpha * d.Ap[row];  // :97
         d.r[row] = ri;
         double si;
@@ -171,18 +195,19 @@ __global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, const double *
         rr = rr + ri * ri;
     }
     rs = block_reduce<RED_SUM>(rs);
+    if (threadIdx.x == 0) d.P(P_RS)[blockIdx.x] = rs;
     rr = block_reduce<RED_SUM>(rr);
-    double r1, rrt;
-    if (last_block_reduce2(rs, rr, d.part, d.cnt + 1, &r1, &rrt)) {
-        if (threadIdx.x == 0) {
-            double r0 = d.sc[S_RHO];  // :107-111
-            d.sc[S_RHO] = r1;
-            d.sc[S_BETA] = r1 / r0;
-            d.sc[S_RR] = rrt;
-            *d.iters += 1;
-            double tol2 = d.sc[S_TOL2];
-            if (tol2 >= 0.0 && rrt <= tol2) *d.state = ST_CONV;
-        }
+    if (threadIdx.x == 0) d.P(P_RR)[blockIdx.x] = rr;
+    if (leader()) *d.iters = k + 1;
+}
+
+// one workgroup after the last iteration: r.r and the final convergence test
+__global__ __launch_bounds__(kBlock) void k_fin(PcgDev d) {
+    const double rr = sum_partials<RED_SUM>(d.P(P_RR), d.G);
+    if (threadIdx.x == 0) {
+        d.sc[S_RR] = rr;
+        const double tol2 = d.sc[S_TOL2];
+        if (*d.state == ST_RUN && tol2 >= 0.0 && rr <= tol2) *d.state = ST_CONV;
     }
 }
 
@@ -190,9 +215,12 @@ __global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, const double *
 
 template <int W, bool FUSE>
 __global__ __launch_bounds__(kBlock) void k_m_spmv(EllView A, PcgDev d, const float *__restrict__ p_old,
-                                                   float *__restrict__ p_new) {
+                                                   float *__restrict__ p_new, int k) {
     if (!running(d)) return;
-    const float beta = FUSE ? (float)d.sc[S_BETA] : 0.0f;
+    const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);
+    // after a restart the previous rho slot holds +inf: beta = 0
+    const float beta = FUSE ? (float)(rho / rho_slot(d, k - 1)) : 0.0f;
+    if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
     Operand<float> po{FUSE ? p_old : p_new, d.s32, beta, d.n, FUSE};
     double acc = 0.0;
     FOR_ROWS(d.n, row) {
@@ -204,19 +232,18 @@ __global__ __launch_bounds__(kBlock) void k_m_spmv(EllView A, PcgDev d, const fl
         acc = acc + (double)pi * (double)sum;
     }
     acc = block_reduce<RED_SUM>(acc);
-    double dn;
-    if (last_block_reduce<RED_SUM>(acc, d.part, d.cnt + 0, &dn)) {
-        if (threadIdx.x == 0) {
-            double alpha = d.sc[S_RHO] / dn;
-            d.sc[S_ALPHA] = alpha;
-            if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) *d.state = ST_BREAK;
-        }
-    }
+    if (threadIdx.x == 0) d.P(P_DN)[blockIdx.x] = acc;
 }
 
-__global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, const float *__restrict__ p) {
+__global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, const float *__restrict__ p, int k) {
     if (!running(d)) return;
-    const float a32 = (float)d.sc[S_ALPHA];
+    const double dn = sum_partials<RED_SUM>(d.P(P_DN), d.G);
+    const double alpha = rho_slot(d, k) / dn;
+    if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) {
+        if (leader()) *d.state = ST_BREAK;
+        return;
+    }
+    const float a32 = (float)alpha;
     double rs = 0.0, rr = 0.0;
     FOR_ROWS(d.n, row) {
         d.xc32[row] = d.xc32[row] + a32 * p[row];
@@ -228,18 +255,16 @@ __global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, const float *__re
         rr = rr + (double)ri * (double)ri;
     }
     rs = block_reduce<RED_SUM>(rs);
+    if (threadIdx.x == 0) d.P(P_RS)[blockIdx.x] = rs;
     rr = block_reduce<RED_SUM>(rr);
-    double r1, rrt;
-    if (last_block_reduce2(rs, rr, d.part, d.cnt + 1, &r1, &rrt)) {
-        if (threadIdx.x == 0) {
-            double r0 = d.sc[S_RHO];
-            d.sc[S_RHO_PREV] = r0;
-            d.sc[S_RHO] = r1;
-            d.sc[S_BETA] = r1 / r0;
-            d.sc[S_RR] = rrt;
-            *d.iters += 1;
-        }
-    }
+    if (threadIdx.x == 0) d.P(P_RR)[blockIdx.x] = rr;
+    if (leader()) *d.iters = k + 1;
+}
+
+// one workgroup: the FP32 recursive r.r for the host's correction test
+__global__ __launch_bounds__(kBlock) void k_m_status(PcgDev d) {
+    const double rr = sum_partials<RED_SUM>(d.P(P_RR), d.G);
+    if (threadIdx.x == 0) d.sc[S_RR] = rr;
 }
 
 // x += xc, xc = 0
@@ -251,10 +276,9 @@ __global__ __launch_bounds__(kBlock) void k_m_fold(PcgDev d) {
 }
 
 // Defect correction: r = b - A x in FP64 (empty rows: x), r32 = r,
-// s32 = Dinv r32, rho = r32.s32; the direction is kept and the next beta
-// uses the replaced residual; convergence on the true ||r||.
+// s32 = Dinv r32; partials of r32.s32 and of the true r.r.
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_m_resid(EllView A, PcgDev d, int first) {
+__global__ __launch_bounds__(kBlock) void k_m_resid(EllView A, PcgDev d) {
     double rs = 0.0, rr = 0.0;
     Operand<double> xo{d.x, nullptr, 0.0, d.n, false};
     FOR_ROWS(d.n, row) {
@@ -271,19 +295,29 @@ __global__ __launch_bounds__(kBlock) void k_m_resid(EllView A, PcgDev d, int fir
         rr = rr + ri * ri;
     }
     rs = block_reduce<RED_SUM>(rs);
+    if (threadIdx.x == 0) d.P(P_RS2)[blockIdx.x] = rs;
     rr = block_reduce<RED_SUM>(rr);
-    double r1, rrt;
-    if (last_block_reduce2(rs, rr, d.part, d.cnt + 1, &r1, &rrt)) {
-        if (threadIdx.x == 0) {
-            // keep the direction (reliable update) unless the FP32 recursion
-            // had drifted from the true residual by more than 10x: then restart
-            if (!first) d.sc[S_BETA] = rrt > 100.0 * d.sc[S_RR] ? 0.0 : r1 / d.sc[S_RHO_PREV];
-            d.sc[S_RHO] = r1;
-            d.sc[S_TRUE_RR] = rrt;
-            d.sc[S_RR] = rrt;
-            double tol2 = d.sc[S_TOL2];
-            if (tol2 >= 0.0 && rrt <= tol2) *d.state = ST_CONV;
-        }
+    if (threadIdx.x == 0) d.P(P_RR2)[blockIdx.x] = rr;
+}
+
+// One workgroup after k_m_resid: the replaced residual becomes the
+// recursion's (its partials are copied over the FP32 ones); the direction is
+// kept unless the recursive r.r had drifted below a hundredth of the true
+// one, then the previous rho slot is set to +inf so the next beta is 0.
+__global__ __launch_bounds__(kBlock) void k_m_corr_fin(PcgDev d, int k, int first) {
+    const double rrt = sum_partials<RED_SUM>(d.P(P_RR2), d.G);
+    const double rrec = first ? 0.0 : sum_partials<RED_SUM>(d.P(P_RR), d.G);
+    __syncthreads();
+    for (int i = threadIdx.x; i < d.G; i += kBlock) {
+        d.P(P_RS)[i] = d.P(P_RS2)[i];
+        d.P(P_RR)[i] = d.P(P_RR2)[i];
+    }
+    if (threadIdx.x == 0) {
+        if (!first && rrt > 100.0 * rrec) d.sc[S_RHO0 + ((k - 1) & 1)] = INFINITY;
+        d.sc[S_TRUE_RR] = rrt;
+        d.sc[S_RR] = rrt;
+        const double tol2 = d.sc[S_TOL2];
+        if (*d.state == ST_RUN && tol2 >= 0.0 && rrt <= tol2) *d.state = ST_CONV;
     }
 }
 
@@ -311,12 +345,12 @@ struct gip_pcg {
     double *r, *p[2], *s, *Ap, *dinv;
     float *r32, *p32[2], *s32, *Ap32, *xc32, *dinv32;
     double *sc, *part;
-    unsigned int *cnt;
-    int *state;  // state, iters
+    int *state;    // state, iters
     int *h_state;  // pinned copy
     double *h_sc;  // pinned scalars
 };
 
+// Multiple of 8 (one share per XCD), at most kMaxRedBlocks (one partial each).
 static int pcg_grid(int n) {
     int nchunk = (n + kBlock - 1) / kBlock;
     int g = ((nchunk + 7) / 8) * 8;
@@ -342,6 +376,7 @@ static int pcg_grid(int n) {
 static PcgDev make_dev(gip_pcg *S, const double *b, double *x, int precond) {
     PcgDev d;
     d.n = S->n;
+    d.G = S->grid;
     d.x = x;
     d.r = S->r;
     d.s = S->s;
@@ -357,7 +392,6 @@ static PcgDev make_dev(gip_pcg *S, const double *b, double *x, int precond) {
     d.state = S->state;
     d.iters = S->state + 1;
     d.part = S->part;
-    d.cnt = S->cnt;
     d.precond = precond;
     return d;
 }
@@ -389,13 +423,12 @@ int gip_pcg_create(gip_ctx *c, const gip_mat *A, int32_t n_all, gip_pcg **out) {
     dalloc((void **)&S->Ap, n * 8);
     dalloc((void **)&S->dinv, n * 8);
     dalloc((void **)&S->sc, S_COUNT * 8);
-    dalloc((void **)&S->part, 4 * kMaxRedBlocks * 8);
-    dalloc((void **)&S->cnt, 16 * sizeof(unsigned int));
+    dalloc((void **)&S->part, (size_t)P_COUNT * kMaxRedBlocks * 8);
     dalloc((void **)&S->state, 2 * sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_state, 2 * sizeof(int), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_sc, S_COUNT * 8, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipMemsetAsync(S->cnt, 0, 16 * sizeof(unsigned int), c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(S->sc, 0, S_COUNT * 8, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(S->state, 0, 2 * sizeof(int), c->stream);
     if (e != hipSuccess) {
         int rc = hip_fail(e, "gip_pcg_create");
         gip_pcg_destroy(S);
@@ -418,8 +451,8 @@ int gip_pcg_destroy(gip_pcg *S) {
     if (!S) return GIP_OK;
     hipSetDevice(S->ctx->device);
     hipStreamSynchronize(S->ctx->stream);
-    void *ptrs[] = {S->r,     S->p[0],   S->p[1], S->s,    S->Ap,   S->dinv, S->r32, S->p32[0], S->p32[1],
-                    S->s32,   S->Ap32,   S->xc32, S->dinv32, S->sc, S->part, S->cnt, S->state};
+    void *ptrs[] = {S->r,   S->p[0], S->p[1], S->s,      S->Ap, S->dinv, S->r32,  S->p32[0],
+                    S->p32[1], S->s32, S->Ap32, S->xc32, S->dinv32, S->sc, S->part, S->state};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (S->h_state) hipHostFree(S->h_state);
@@ -471,10 +504,10 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
     PcgDev d = make_dev(S, b, x, o->precond ? 1 : 0);
     EllView v = view(A);
     const int pollk = o->poll > 0 ? o->poll : 16;
-    GIP_HIP_TRY(hipMemsetAsync(S->state, 0, 2 * sizeof(int), st));
     // setup, GIP_LinearSolver.h:55-80
     DISPATCH_W(A, k_setup_resid, NOTAIL, v, d, x0, S->n_all, S->p[0], S->p[1]);
-    k_setup_dir<<<S->grid, kBlock, 0, st>>>(d, S->p[0], o->tol);
+    k_setup_dir<<<S->grid, kBlock, 0, st>>>(d, S->p[0]);
+    k_setup_fin<<<1, kBlock, 0, st>>>(d, o->tol);
     GIP_LAUNCH_CHECK("pcg setup");
     int corrections = 0, rc;
     if ((rc = poll(S, true))) return rc;
@@ -485,17 +518,20 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
             for (int j = 0; j < chunk; ++j, ++k) {
                 double *pn = S->p[k & 1];
                 if (k == 0)
-                    DISPATCH_W(A, k_iter_spmv, FIRST_TAIL, v, d, pn, pn);
+                    DISPATCH_W(A, k_iter_spmv, FIRST_TAIL, v, d, pn, pn, k);
                 else
-                    DISPATCH_W(A, k_iter_spmv, FUSED_TAIL, v, d, S->p[(k - 1) & 1], pn);
-                k_iter_update<<<S->grid, kBlock, 0, st>>>(d, pn);
+                    DISPATCH_W(A, k_iter_spmv, FUSED_TAIL, v, d, S->p[(k - 1) & 1], pn, k);
+                k_iter_update<<<S->grid, kBlock, 0, st>>>(d, pn, k);
             }
             GIP_LAUNCH_CHECK("pcg iteration");
             if ((rc = poll(S, false))) return rc;
         }
+        k_fin<<<1, kBlock, 0, st>>>(d);
+        GIP_LAUNCH_CHECK("pcg final test");
     } else {
         // FP32 residual of x0 and the first direction
-        DISPATCH_W(A, k_m_resid, NOTAIL, v, d, 1);
+        DISPATCH_W(A, k_m_resid, NOTAIL, v, d);
+        k_m_corr_fin<<<1, kBlock, 0, st>>>(d, 0, 1);
         k_m_start<<<S->grid, kBlock, 0, st>>>(d, S->p32[0]);
         GIP_LAUNCH_CHECK("mixed setup");
         if ((rc = poll(S, true))) return rc;
@@ -508,11 +544,12 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
             for (int j = 0; j < chunk; ++j, ++k) {
                 float *pn = S->p32[k & 1];
                 if (k == 0)
-                    DISPATCH_W(A, k_m_spmv, FIRST_TAIL, v, d, pn, pn);
+                    DISPATCH_W(A, k_m_spmv, FIRST_TAIL, v, d, pn, pn, k);
                 else
-                    DISPATCH_W(A, k_m_spmv, FUSED_TAIL, v, d, S->p32[(k - 1) & 1], pn);
-                k_m_update<<<S->grid, kBlock, 0, st>>>(d, pn);
+                    DISPATCH_W(A, k_m_spmv, FUSED_TAIL, v, d, S->p32[(k - 1) & 1], pn, k);
+                k_m_update<<<S->grid, kBlock, 0, st>>>(d, pn, k);
             }
+            k_m_status<<<1, kBlock, 0, st>>>(d);
             GIP_LAUNCH_CHECK("mixed iteration");
             if ((rc = poll(S, true))) return rc;
             if (S->h_state[0] != ST_RUN) break;
@@ -520,7 +557,8 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
             const bool near_target = tol2 >= 0.0 && rr32 <= tol2;
             if (rr32 <= inner2 * last_rr || near_target || k >= o->maxit) {
                 k_m_fold<<<S->grid, kBlock, 0, st>>>(d);
-                DISPATCH_W(A, k_m_resid, NOTAIL, v, d, 0);
+                DISPATCH_W(A, k_m_resid, NOTAIL, v, d);
+                k_m_corr_fin<<<1, kBlock, 0, st>>>(d, k, 0);
                 GIP_LAUNCH_CHECK("mixed correction");
                 ++corrections;
                 if ((rc = poll(S, true))) return rc;
