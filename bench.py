@@ -14,6 +14,14 @@ FP64 vs mixed), and the reference CPU path's PCG on the same system
 (`cpu_baseline`).  Inputs are resident in HBM before the timed region.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--ns-steps 20]
+
+Other suites (one JSON line each, not the default):
+    --suite batch     64 uneven cavities (n ~ U{32..96}, Re log-uniform in
+                      [100, 3200]) advanced 50 steps as one batch vs one by one
+    --suite adjoint   64^3, Re=400, 200 steps, checkpoints every 20: forward +
+                      backward time and 16 central finite-difference checks
+    --probe spmv      a few launches of the FP64 SpMV and of a plain copy of
+                      known size, for rocprofv3 --pmc traffic runs
 """
 from __future__ import annotations
 
@@ -97,8 +105,26 @@ def ns_leg(dns, cav, mode, steps, warmup):
         its.append(s["pcg_iters"])
     sim.sync()
     dt = time.perf_counter() - t0
+    fields = {k: sim.field(k) for k in ("u", "v", "w", "p")}
     sim.close()
-    return {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps, "pcg_iters_mean": float(np.mean(its))}
+    return {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps, "pcg_iters_mean": float(np.mean(its))}, fields
+
+
+def spmv32_bytes(nnz, rows, cols):
+    """Algorithmic bytes of one FP32-value SpMV (int32 indices)."""
+    return 8 * nnz + 4 * (rows + 1) + 4 * rows + 4 * cols
+
+
+def measured_traffic(n):
+    """HBM bytes per launch of the FP64 SpMV at n^3 from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+    f = ROOT / "profiles" / f"spmv_ell_fp64_{n}_traffic.txt"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(float(line.split(":", 1)[1]))
+    return None
 
 
 def cpu_baseline(n, iters, seed, fp64_iters):
@@ -134,11 +160,19 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=60)
     ap.add_argument("--skip-ns", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--suite", choices=("pcg", "batch", "adjoint"), default="pcg")
+    ap.add_argument("--probe", choices=("spmv",), default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs one partition on one device "
               f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
         sys.exit(2)
+    if args.probe:
+        return probe_spmv(args)
+    if args.suite == "batch":
+        return batch_suite(args)
+    if args.suite == "adjoint":
+        return adjoint_suite(args)
 
     from gip_amd import cavity, dns, hip
 
@@ -169,6 +203,13 @@ def main():
 
     fp64 = pcg_leg(hip, dev, A, b_dev, N, NA, hip.PCG_FP64, args.steps, args.warmup, 20000, bnorm)
     mixed = pcg_leg(hip, dev, A, b_dev, N, NA, hip.PCG_MIXED, args.steps, args.warmup, 20000, bnorm)
+    it_fp64 = alg + 88 * N  # FP64 PCG iteration: SpMV + x, r, p, Ap, Dinv, s streams
+    it_mixed = spmv32_bytes(info["nnz"], N, NA) + 44 * N
+    for leg, byts in ((fp64, it_fp64), (mixed, it_mixed)):
+        ms_it = leg["ms"] / max(1, leg["iters"])
+        leg["ms_per_iter"] = ms_it
+        leg["iter_alg_bytes"] = byts
+        leg["iter_gbs"] = byts / (ms_it * 1e-3) / 1e9
     log(f"fp64 pcg {fp64['ms']:.1f} ms ({fp64['iters']} it, true {fp64['true_relres']:.2e}); "
         f"mixed {mixed['ms']:.1f} ms ({mixed['iters']} it, true {mixed['true_relres']:.2e}); "
         f"speedup {fp64['ms'] / mixed['ms']:.2f}x")
@@ -181,9 +222,16 @@ def main():
         t0 = time.perf_counter()
         full = cavity.generate(ns_n, gamma=1e-3)
         log(f"generated full {ns_n}^3 cavity operators in {time.perf_counter() - t0:.1f} s")
-        ns = {"n": ns_n, "re": 1000, "steps": args.ns_steps,
-              "fp64": ns_leg(dns, full, "fused", args.ns_steps, 1),
-              "mixed": ns_leg(dns, full, "mixed", args.ns_steps, 1)}
+        r64, f64 = ns_leg(dns, full, "fused", args.ns_steps, 1)
+        r32, f32 = ns_leg(dns, full, "mixed", args.ns_steps, 1)
+        # error of the mixed-precision run against the FP64 run, per field
+        # (max abs difference over the inner cells / max abs FP64 value)
+        N = full.n_inner
+        err = {k: float(np.abs(f32[k][:N] - f64[k][:N]).max() / max(1e-300, np.abs(f64[k][:N]).max()))
+               for k in f64}
+        ns = {"n": ns_n, "re": 1000, "steps": args.ns_steps, "fp64": r64, "mixed": r32,
+              "mixed_vs_fp64_rel_err": err}
+        del f64, f32
         log(f"ns steps/s fp64 {ns['fp64']['steps_per_s']:.3f}, mixed {ns['mixed']['steps_per_s']:.3f}")
         del full
 
@@ -204,7 +252,7 @@ def main():
         "data": "synthetic: generated 256^3 uniform cavity Laplacian, seeded N(0,1) zero-mean RHS",
         "config": {"workload": f"{n}^3 cavity Poisson, Jacobi-PCG to 1e-10 FP64 relative true residual "
                                f"(mixed precision), one partition", "n": n, "nnz": info["nnz"],
-                   "layout": {1: "csr", 2: "sliced-ell", 3: "diag"}.get(info["format"], "?")},
+                   "layout": hip.FORMAT_NAMES.get(info["format"], "?")},
         "pcg_fp64_ms": round(fp64["ms"], 3),
         "pcg_mixed_ms": round(mixed["ms"], 3),
         "mixed_speedup": round(fp64["ms"] / mixed["ms"], 3),
@@ -213,11 +261,125 @@ def main():
         "spmv_fp64": {"ms": spmv_ms, "gbs": spmv_gbs, "algorithmic_bytes": alg},
         "ns_steps": ns,
         "roofline": {"bound": "hbm", "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n),
+                     "algorithmic_bytes": alg,
                      "kernel": "k_ell<0,7> (FP64 sliced-ELL SpMV, 256^3 Laplacian)"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def probe_spmv(args):
+    """Five FP64 SpMVs on the n^3 Laplacian and five copies of 2^27 doubles
+    (1 GiB read + 1 GiB written each): the launches a rocprofv3 --pmc run
+    counts, the copy calibrating FETCH_SIZE/WRITE_SIZE against known bytes."""
+    from gip_amd import cavity, hip
+    cav = cavity.generate(args.n, gamma=1e-3, ops=cavity.OP_BITS["L"])
+    L = cav.op("L")
+    dev = hip.Device(0)
+    A = hip.Matrix.from_csr(dev, L)
+    info = A.info()
+    x = dev.to_device(np.random.default_rng(1).standard_normal(cav.n_all))
+    y = dev.array(cav.n_all)
+    for _ in range(5):
+        A.spmv(x, y)
+    nc = 1 << 27
+    a, b = dev.array(nc), dev.array(nc)
+    hip.fill(a, 1.0)
+    for _ in range(5):
+        hip.copy(a, b)
+    dev.sync()
+    print(json.dumps({"probe": "spmv", "n": args.n, "spmv_algorithmic_bytes": spmv_bytes(info["nnz"], cav.n_inner,
+                                                                                          cav.n_all),
+                      "copy_bytes_read": 8 * nc, "copy_bytes_written": 8 * nc}), flush=True)
+
+
+def batch_suite(args):
+    """64 cavities of uneven size and Reynolds number, 50 steps: the batched
+    integrator (one block-diagonal system, segmented kernels, hipGraph) vs
+    the same instances stepped one after another with the single-cavity
+    path.  Both solve the pressure to 1e-8 with the mixed-precision PCG."""
+    from gip_amd import batch as gbatch
+    from gip_amd import cavity, dns, hip
+    rng = np.random.default_rng(args.seed)
+    B, steps = 64, 50
+    sizes = [int(v) for v in rng.integers(32, 97, B)]
+    re = np.exp(rng.uniform(np.log(100.0), np.log(3200.0), B))
+    t0 = time.perf_counter()
+    cavs = [cavity.generate(n, gamma=1.0 / r) for n, r in zip(sizes, re)]
+    cells = sum(c.n_inner for c in cavs)
+    log(f"generated {B} cavities ({cells} cells) in {time.perf_counter() - t0:.1f} s")
+    dev = hip.Device(0)
+    bt = gbatch.Batch(dev, cavs, 1.0 / re, pcg_tol=1e-8, pcg_maxit=20000, precision=hip.PCG_MIXED, poll=16,
+                      use_graph=True, autotune=True)
+    bt.step()  # warm-up: graph capture
+    dev.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps - 1):
+        bt.step()
+    dev.sync()
+    t_batch = time.perf_counter() - t0
+    bt_u = [bt.field(i, "u") for i in range(B)]
+    bt_info = [bt.info(i) for i in range(B)]
+    bt.close()
+    log(f"batch: {steps - 1} steps in {t_batch:.2f} s")
+    t_seq, max_diff = 0.0, 0.0
+    for i, (c, r) in enumerate(zip(cavs, re)):
+        sim = dns.Simulation(cavity=c, gamma=1.0 / r, pcg_mode="mixed", pcg_tol=1e-8, pcg_maxit=20000)
+        sim.step()
+        sim.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps - 1):
+            sim.step()
+        sim.sync()
+        t_seq += time.perf_counter() - t0
+        u = sim.field("u")
+        ref = max(1e-30, float(np.abs(u).max()))
+        max_diff = max(max_diff, float(np.abs(u - bt_u[i]).max()) / ref)
+        sim.close()
+    log(f"sequential: {t_seq:.2f} s; max relative u difference batch vs sequential {max_diff:.2e}")
+    inst_steps = B * (steps - 1)
+    print(json.dumps({
+        "metric": "batched cavities: instance-steps/s (64 uneven cavities, PCG to 1e-8, mixed precision)",
+        "value": inst_steps / t_batch, "unit": "instance-steps/s", "n_gpus": 1, "steps": steps - 1, "warmup": 1,
+        "higher_is_better": True, "sequential_instance_steps_per_s": inst_steps / t_seq,
+        "speedup_vs_sequential": t_seq / t_batch, "max_rel_u_diff_vs_sequential": max_diff,
+        "cells_total": cells, "sizes": sizes, "re": [round(float(v), 1) for v in re],
+        "pcg_iters_last_step": [i["pcg_iters"] for i in bt_info],
+        "data": "synthetic: generated uniform cavities, lid u=1, from rest"}), flush=True)
+
+
+def adjoint_suite(args):
+    """64^3 cavity, Re=400, 200 steps with checkpoints every 20 steps:
+    J = 1/2 sum V (u^2+v^2+w^2) at the end; gradient with respect to the
+    64x64 lid values (1 + 0.05 N(0,1)) and the initial velocity, checked by
+    central differences (eps 1e-6) along 16 random directions, dt frozen."""
+    from gip_amd import adjoint as gadj
+    from gip_amd import cavity, hip
+    n, steps, every = 64, 200, 20
+    cav = cavity.generate(n, gamma=1.0 / 400.0)
+    dev = hip.Device(0)
+    adj = gadj.Adjoint(dev, cav, steps=steps, checkpoint_every=every, gamma=1.0 / 400.0, pcg_tol=1e-13)
+    rng = np.random.default_rng(args.seed)
+    lid = 1.0 + 0.05 * rng.standard_normal(n * n)
+    u0, v0, w0 = (np.zeros(cav.n_inner) for _ in range(3))
+    adj.forward(lid, u0, v0, w0)  # freezes dt
+    t0 = time.perf_counter()
+    adj.forward(lid, u0, v0, w0)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    adj.backward()
+    t_bwd = time.perf_counter() - t0
+    J, ad, fd, rel = gadj.fd_check(adj, lid, u0, v0, w0, ndir=16, eps=1e-6, seed=args.seed)
+    adj.close()
+    log(f"adjoint: forward {t_fwd:.2f} s, backward {t_bwd:.2f} s, max FD rel err {rel.max():.2e}")
+    print(json.dumps({
+        "metric": "adjoint of 200 cavity steps (64^3, Re=400): backward/forward time, FD agreement",
+        "value": t_bwd / t_fwd, "unit": "backward/forward", "n_gpus": 1, "higher_is_better": False,
+        "forward_s": t_fwd, "backward_s": t_bwd, "J": J, "fd_max_rel_err": float(rel.max()),
+        "fd_rel_err": [float(v) for v in rel], "directional_adjoint": [float(v) for v in ad],
+        "directional_fd": [float(v) for v in fd], "checkpoint_every": every, "steps": steps,
+        "data": "synthetic: generated 64^3 uniform cavity, lid 1 + 0.05 N(0,1), from rest"}), flush=True)
 
 
 if __name__ == "__main__":

@@ -611,24 +611,16 @@ int make_seg(gip_batch *b, const std::vector<int64_t> &off, const std::vector<in
     return GIP_OK;
 }
 
-EllView view_of(const gip_mat *A) {
-    EllView v;
-    v.fmt = A->format;
-    v.soff = A->d_slice_off;
-    v.col = A->format == GIP_FMT_ELL ? A->d_ell_col : A->d_col;
-    v.rp = A->d_rowptr;
-    v.val = A->format == GIP_FMT_ELL ? A->d_ell_val : A->d_val;
-    v.val32 = A->format == GIP_FMT_ELL ? A->d_ell_val32 : A->d_val32;
-    return v;
-}
-
-int width_of(const gip_mat *A) { return A->format == GIP_FMT_ELL ? A->uniform_width : 0; }
+// the shared operator view and kernel width (gip_rows.h)
+EllView view_of(const gip_mat *A) { return view(A); }
 
 #define BDISPATCH(A, GRID, KERNEL, ...)                                                          \
     do {                                                                                         \
-        switch (width_of(A)) {                                                                   \
+        switch (kernel_width(A)) {                                                               \
             case 6: KERNEL<6><<<GRID, kBlock, 0, st>>>(__VA_ARGS__); break;                      \
             case 7: KERNEL<7><<<GRID, kBlock, 0, st>>>(__VA_ARGS__); break;                      \
+            case -7: KERNEL<-7><<<GRID, kBlock, 0, st>>>(__VA_ARGS__); break;                    \
+            case -8: KERNEL<-8><<<GRID, kBlock, 0, st>>>(__VA_ARGS__); break;                    \
             default: KERNEL<0><<<GRID, kBlock, 0, st>>>(__VA_ARGS__); break;                     \
         }                                                                                        \
     } while (0)

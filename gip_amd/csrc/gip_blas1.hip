@@ -24,6 +24,11 @@ __global__ __launch_bounds__(kBlock) void k_axpby(int n, double a, const double 
     if (i < n) y[i] = a * x[i] + b * y[i];
 }
 
+__global__ __launch_bounds__(kBlock) void k_copy(int n, const double *__restrict__ x, double *__restrict__ y) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) y[i] = x[i];
+}
+
 // Reductions run as two launches: every workgroup stores its partial, then
 // one workgroup combines the partials in index order.  The kernel boundary
 // orders the two, so no atomics or device-wide fences are needed (a fence
@@ -128,6 +133,14 @@ int gip_daxpby(gip_ctx *c, int32_t n, double alpha, const double *x, double beta
     if (!n) return GIP_OK;
     k_axpby<<<blocks_for(n), kBlock, 0, c->stream>>>(n, alpha, x, beta, y);
     GIP_LAUNCH_CHECK("gip_daxpby");
+    return GIP_OK;
+}
+
+int gip_dcopy(gip_ctx *c, int32_t n, const double *x, double *y) {
+    if (!c || n < 0 || (n && (!x || !y))) return set_error(GIP_EINVAL, "gip_dcopy: bad argument");
+    if (!n) return GIP_OK;
+    k_copy<<<blocks_for(n), kBlock, 0, c->stream>>>(n, x, y);
+    GIP_LAUNCH_CHECK("gip_dcopy");
     return GIP_OK;
 }
 

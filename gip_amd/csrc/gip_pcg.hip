@@ -359,15 +359,18 @@ static int pcg_grid(int n) {
     return g;
 }
 
-// Launch KERNEL<W, ...> with W the operator's uniform ELL width (5..7) or 0.
-#define DISPATCH_W(A, KERNEL, TAIL, ...)                                             \
-    do {                                                                             \
-        switch ((A)->format == GIP_FMT_ELL ? (A)->uniform_width : -1) {              \
-            case 5: KERNEL<5 TAIL><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;  \
-            case 6: KERNEL<6 TAIL><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;  \
-            case 7: KERNEL<7 TAIL><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;  \
-            default: KERNEL<0 TAIL><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break; \
-        }                                                                            \
+// Launch KERNEL<W, ...> with W the operator's kernel width: a uniform ELL
+// width (5..7), a stencil width (-7, -8), or 0 for the generic loops.
+#define DISPATCH_W(A, KERNEL, TAIL, ...)                                              \
+    do {                                                                              \
+        switch (kernel_width(A)) {                                                    \
+            case 5: KERNEL<5 TAIL><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;   \
+            case 6: KERNEL<6 TAIL><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;   \
+            case 7: KERNEL<7 TAIL><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;   \
+            case -7: KERNEL<-7 TAIL><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break; \
+            case -8: KERNEL<-8 TAIL><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break; \
+            default: KERNEL<0 TAIL><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;  \
+        }                                                                             \
     } while (0)
 #define NOTAIL
 #define FUSED_TAIL , true
@@ -401,8 +404,8 @@ extern "C" {
 int gip_pcg_create(gip_ctx *c, const gip_mat *A, int32_t n_all, gip_pcg **out) {
     if (!c || !A || !out || n_all < A->nrows) return set_error(GIP_EINVAL, "gip_pcg_create: bad argument");
     if (n_all < A->cols_needed) return set_error(GIP_EINVAL, "gip_pcg_create: vectors shorter than the operator's columns");
-    if (A->format != GIP_FMT_ELL && A->format != GIP_FMT_CSR)
-        return set_error(GIP_EINVAL, "gip_pcg_create: operator must be ELL or CSR");
+    if (A->format != GIP_FMT_ELL && A->format != GIP_FMT_CSR && A->format != GIP_FMT_STENCIL)
+        return set_error(GIP_EINVAL, "gip_pcg_create: operator must be ELL, stencil or CSR");
     gip_pcg *S = new (std::nothrow) gip_pcg();
     if (!S) return set_error(GIP_ENOMEM, "gip_pcg_create: allocation failed");
     S->ctx = c;
@@ -464,7 +467,8 @@ int gip_pcg_destroy(gip_pcg *S) {
 static int ensure_mixed(gip_pcg *S) {
     if (S->r32) return GIP_OK;
     const gip_mat *A = S->A;
-    if ((A->format == GIP_FMT_ELL && !A->d_ell_val32) || (A->format == GIP_FMT_CSR && !A->d_val32))
+    if (((A->format == GIP_FMT_ELL || A->format == GIP_FMT_STENCIL) && !A->d_ell_val32) ||
+        (A->format == GIP_FMT_CSR && !A->d_val32))
         return set_error(GIP_EINVAL, "mixed precision needs an operator created with GIP_MAT_FP32");
     size_t n = (size_t)std::max(1, S->n), na = (size_t)std::max(1, S->n_all);
     float **bufs[] = {&S->r32, &S->s32, &S->Ap32, &S->xc32, &S->dinv32};

@@ -10,21 +10,42 @@
 namespace gip {
 
 struct EllView {
-    int fmt;  // GIP_FMT_ELL / GIP_FMT_CSR
+    int fmt;  // GIP_FMT_ELL / GIP_FMT_CSR / GIP_FMT_STENCIL
     const int32_t *soff, *col, *rp;
     const double *val;
     const float *val32;
+    // stencil layout
+    const int32_t *st_off, *st_tsoff, *st_flag, *st_tcol;
+    const uint16_t *st_tinfo;
+    int st_w, tail_base, cmax;
 };
 
 static EllView view(const gip_mat *A) {
-    EllView v;
+    EllView v{};
+    const bool packed = A->format == GIP_FMT_ELL || A->format == GIP_FMT_STENCIL;
     v.fmt = A->format;
     v.soff = A->d_slice_off;
     v.col = A->format == GIP_FMT_ELL ? A->d_ell_col : A->d_col;
     v.rp = A->d_rowptr;
-    v.val = A->format == GIP_FMT_ELL ? A->d_ell_val : A->d_val;
-    v.val32 = A->format == GIP_FMT_ELL ? A->d_ell_val32 : A->d_val32;
+    v.val = packed ? A->d_ell_val : A->d_val;
+    v.val32 = packed ? A->d_ell_val32 : A->d_val32;
+    v.st_off = A->d_st_off;
+    v.st_tsoff = A->d_st_tsoff;
+    v.st_flag = A->d_st_flag;
+    v.st_tcol = A->d_st_tcol;
+    v.st_tinfo = A->d_st_tinfo;
+    v.st_w = A->st_w;
+    v.tail_base = (int)A->st_tail_base;
+    v.cmax = A->cols_needed > 0 ? A->cols_needed - 1 : 0;
     return v;
+}
+
+// Kernel template width of an operator: W > 0 uniform-width ELL, W < 0 the
+// stencil layout with -W slots, 0 the generic loops.
+static inline int kernel_width(const gip_mat *A) {
+    if (A->format == GIP_FMT_ELL) return A->uniform_width;
+    if (A->format == GIP_FMT_STENCIL) return -A->st_w;
+    return 0;
 }
 
 // The operand of a product: either a plain vector, or the direction formed
@@ -42,11 +63,51 @@ struct Operand {
     }
 };
 
+// Stencil-layout row sum: the slots in order (column row + offset, clamped
+// into the vector for slots the row lacks, whose value is 0), then the
+// row's tail in stored order.  The wavefront's 64 rows are one slice, so
+// the slice's offsets and header are wavefront-uniform (scalar loads).
+template <int SW, class T, class V>
+__device__ inline bool stencil_row(const EllView &A, const V *__restrict__ val, const Operand<T> &x, int row, T &sum) {
+    const int w = SW > 0 ? SW : A.st_w;
+    const int s = __builtin_amdgcn_readfirstlane(row >> 6);
+    const int32_t *off = A.st_off + (size_t)s * w;
+    const V *vs = val + (size_t)s * w * 64 + (row & 63);
+    if (SW > 0) {
+        int c[SW > 0 ? SW : 1];
+        V v[SW > 0 ? SW : 1];
+#pragma unroll
+        for (int k = 0; k < SW; ++k) {
+            const int cc = row + off[k];
+            c[k] = cc < 0 ? 0 : (cc > A.cmax ? A.cmax : cc);
+            v[k] = __builtin_nontemporal_load(&vs[64 * k]);
+        }
+#pragma unroll
+        for (int k = 0; k < SW; ++k) sum = sum + x.at(c[k]) * v[k];
+    } else {
+        for (int k = 0; k < w; ++k) {
+            const int cc = row + off[k];
+            sum = sum + x.at(cc < 0 ? 0 : (cc > A.cmax ? A.cmax : cc)) * vs[64 * k];
+        }
+    }
+    if (A.st_flag[s]) {
+        const unsigned info = A.st_tinfo[row];
+        if (info & 0x8000u) return false;  // empty row: left untouched
+        const int t = A.st_tsoff[s] + (int)(info & 255u), nt = (int)((info >> 8) & 127u);
+        for (int e = 0; e < nt; ++e) sum = sum + x.at(A.st_tcol[t + e]) * val[A.tail_base + t + e];
+    }
+    return true;
+}
+
 // Row sum in stored order; returns false for an empty row.
 template <int W, class T, class V>
 __device__ inline bool row_dot(const EllView &A, const V *__restrict__ val, const Operand<T> &x, int row, T &sum) {
     sum = T(0);
-    if (W > 0) {
+    if (W < 0) {
+        return stencil_row<(W < 0 ? -W : 0), T, V>(A, val, x, row, sum);
+    } else if (W == 0 && A.fmt == GIP_FMT_STENCIL) {
+        return stencil_row<0, T, V>(A, val, x, row, sum);
+    } else if (W > 0) {
         const int base = A.soff[row >> 6] + (row & 63);
         int c[W > 0 ? W : 1];
         V v[W > 0 ? W : 1];

@@ -10,12 +10,18 @@
 // of row 64 s + l is stored at slice_off[s] + 64 k + l, so the 64 lanes of a
 // wavefront read 64 consecutive values and indices per entry.  Rows shorter
 // than the slice width are padded with column -1.
+//
+// Stencil: the rows of a slice share per-slot column offsets (see
+// gip_layout.h), so a slot stores only values; the wavefront reads 64
+// consecutive values per slot and gathers x at row + offset.
 #include <algorithm>
 #include <cstring>
 #include <new>
 #include <vector>
 
 #include "gip_internal.h"
+#include "gip_layout.h"
+#include "gip_rows.h"
 #include "gip_spmv.h"
 
 namespace gip {
@@ -66,6 +72,33 @@ __global__ __launch_bounds__(kBlock) void k_ell(int nrows, const int32_t *__rest
         y[out] = sum;
     else
         y[out] = a * sum + b * y[out];
+}
+
+// Stencil-layout product (W < 0: -W slots unrolled; 0: runtime width).
+template <int MODE, int W>
+__global__ __launch_bounds__(kBlock) void k_stencil(EllView A, int nrows, const double *__restrict__ x,
+                                                   double *__restrict__ y, double a, double b,
+                                                   const int32_t *__restrict__ scatter) {
+    const int row = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (row >= nrows) return;
+    Operand<double> xo{x, nullptr, 0.0, 0, false};
+    double sum;
+    if (!row_dot<W, double, double>(A, A.val, xo, row, sum)) return;
+    const int out = scatter ? scatter[row] : row;
+    if (MODE == 0)
+        y[out] = sum;
+    else
+        y[out] = a * sum + b * y[out];
+}
+
+__global__ __launch_bounds__(kBlock) void k_stencil32(EllView A, int nrows, const float *__restrict__ x,
+                                                     float *__restrict__ y) {
+    const int row = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (row >= nrows) return;
+    Operand<float> xo{x, nullptr, 0.0f, 0, false};
+    float sum;
+    row_dot<0, float, float>(A, A.val32, xo, row, sum);
+    y[row] = sum;
 }
 
 template <int MODE>
@@ -196,8 +229,9 @@ static int upload(gip_ctx *c, T **dptr, const T *host, size_t count) {
 }
 
 static void free_mat_arrays(gip_mat *A) {
-    void *ptrs[] = {A->d_rowptr, A->d_col,     A->d_val,  A->d_val32, A->d_slice_off, A->d_ell_col,
-                    A->d_ell_val, A->d_ell_val32, A->d_csr2ell, A->d_diag, A->d_tsrc,  A->d_tdst};
+    void *ptrs[] = {A->d_rowptr,   A->d_col,     A->d_val,      A->d_val32,    A->d_slice_off, A->d_ell_col,
+                    A->d_ell_val,  A->d_ell_val32, A->d_csr2ell, A->d_diag,     A->d_tsrc,      A->d_tdst,
+                    A->d_st_off,   A->d_st_tsoff, A->d_st_flag,  A->d_st_tinfo, A->d_st_tcol};
     for (void *p : ptrs)
         if (p) hipFree(p);
 }
@@ -236,6 +270,29 @@ int spmv_launch(gip_ctx *c, const gip_mat *A, const double *x, double *y, double
                                                      b, scatter);
         }
 #undef GIP_ELL_CASE
+    } else if (A->format == GIP_FMT_STENCIL) {
+        const EllView v = view(A);
+#define GIP_ST_CASE(Wc)                                                                   \
+    case Wc:                                                                              \
+        if (mode == 0)                                                                    \
+            k_stencil<0, Wc><<<g, kBlock, 0, st>>>(v, A->nrows, x, y, a, b, scatter);     \
+        else                                                                              \
+            k_stencil<1, Wc><<<g, kBlock, 0, st>>>(v, A->nrows, x, y, a, b, scatter);     \
+        break;
+        switch (-A->st_w) {
+            GIP_ST_CASE(-2)
+            GIP_ST_CASE(-3)
+            GIP_ST_CASE(-4)
+            GIP_ST_CASE(-6)
+            GIP_ST_CASE(-7)
+            GIP_ST_CASE(-8)
+            default:
+                if (mode == 0)
+                    k_stencil<0, 0><<<g, kBlock, 0, st>>>(v, A->nrows, x, y, a, b, scatter);
+                else
+                    k_stencil<1, 0><<<g, kBlock, 0, st>>>(v, A->nrows, x, y, a, b, scatter);
+        }
+#undef GIP_ST_CASE
     } else if (A->format == GIP_FMT_DIAG) {
         if (mode == 0)
             k_diag<0><<<g, kBlock, 0, st>>>(A->nrows, A->d_diag, x, y, a, b, scatter);
@@ -273,11 +330,14 @@ int spmv32_launch(gip_ctx *c, const gip_mat *A, const float *x, float *y) {
     if (A->format == GIP_FMT_ELL) {
         if (!A->d_ell_val32) return set_error(GIP_EINVAL, "operator has no FP32 values");
         k_ell32<<<nblocks(A->nrows), kBlock, 0, c->stream>>>(A->nrows, A->d_slice_off, A->d_ell_col, A->d_ell_val32, x, y);
+    } else if (A->format == GIP_FMT_STENCIL) {
+        if (!A->d_ell_val32) return set_error(GIP_EINVAL, "operator has no FP32 values");
+        k_stencil32<<<nblocks(A->nrows), kBlock, 0, c->stream>>>(view(A), A->nrows, x, y);
     } else if (A->format == GIP_FMT_CSR) {
         if (!A->d_val32) return set_error(GIP_EINVAL, "operator has no FP32 values");
         k_csr32<<<nblocks(A->nrows), kBlock, 0, c->stream>>>(A->nrows, A->d_rowptr, A->d_col, A->d_val32, x, y);
     } else {
-        return set_error(GIP_EINVAL, "FP32 SpMV needs an ELL or CSR operator");
+        return set_error(GIP_EINVAL, "FP32 SpMV needs an ELL, stencil or CSR operator");
     }
     GIP_LAUNCH_CHECK("spmv32 kernel");
     return GIP_OK;
@@ -346,11 +406,23 @@ int gip_mat_create(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, const 
         if (stored > INT32_MAX) uniform = false;
         soff[(size_t)s + 1] = (int32_t)std::min<int64_t>(stored, INT32_MAX);
     }
+    // stencil layout: planned when automatic or asked for; chosen automatically
+    // when it streams clearly fewer bytes than ELL and CSR would
+    StencilPlan plan;
+    bool stencil_ok = false;
+    if ((want == GIP_FMT_AUTO && !(is_diag && nrows > 0)) || want == GIP_FMT_STENCIL)
+        stencil_ok = plan_stencil(nrows, nnz, rowptr, col, plan);
     int32_t fmt = want;
+    if (fmt == GIP_FMT_STENCIL && !stencil_ok) fmt = GIP_FMT_AUTO;
     if (fmt == GIP_FMT_AUTO) {
+        const bool ell_ok = stored <= INT32_MAX && stored <= (int64_t)(1.2 * nnz) + kSlice * 8;
+        const int64_t csr_bytes = 12 * (int64_t)nnz + 8 * (int64_t)nrows;
+        const int64_t ell_bytes = ell_ok ? 12 * stored + 4 * (int64_t)ns : csr_bytes;
         if (is_diag && nrows > 0)
             fmt = GIP_FMT_DIAG;
-        else if (stored <= INT32_MAX && stored <= (int64_t)(1.2 * nnz) + kSlice * 8)
+        else if (stencil_ok && (double)stencil_bytes(plan, nrows) < 0.85 * (double)std::min(ell_bytes, csr_bytes))
+            fmt = GIP_FMT_STENCIL;
+        else if (ell_ok)
             fmt = GIP_FMT_ELL;
         else
             fmt = GIP_FMT_CSR;
@@ -384,14 +456,44 @@ int gip_mat_create(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, const 
         if (!rc) rc = upload(c, &A->d_ell_col, ecol.data(), ecol.size());
         if (!rc) rc = upload(c, &A->d_ell_val, evalv.data(), evalv.size());
         if (!rc) rc = upload(c, &A->d_csr2ell, c2e.data(), c2e.size());
+        if (!rc) {
+            // the host arrays above are uploaded asynchronously: wait before they go
+            hipError_t e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) rc = hip_fail(e, "ELL upload");
+        }
         if (!rc && fp32) {
-            GIP_HIP_TRY(hipStreamSynchronize(c->stream));
             hipError_t e = hipMalloc(&A->d_ell_val32, sizeof(float) * (size_t)std::max<int64_t>(stored, 1));
             if (e != hipSuccess) rc = hip_fail(e, "hipMalloc fp32 values");
             else if (stored) k_to_f32<<<nblocks(stored), kBlock, 0, c->stream>>>(stored, A->d_ell_val, A->d_ell_val32);
         }
         // CSR columns for transposes
         if (!rc) rc = upload(c, &A->d_col, col, (size_t)nnz);
+    } else if (!rc && fmt == GIP_FMT_STENCIL) {
+        A->nslices = plan.ns;
+        A->ell_stored = plan.stored;
+        A->st_w = plan.w;
+        A->st_tail_base = plan.tail_base;
+        A->st_tail_nnz = (int32_t)plan.tcol.size();
+        std::vector<double> sval((size_t)plan.stored, 0.0);
+        for (int32_t j = 0; j < nnz; ++j) sval[(size_t)plan.pos[(size_t)j]] = val[j];
+        rc = upload(c, &A->d_ell_val, sval.data(), sval.size());
+        if (!rc) rc = upload(c, &A->d_csr2ell, plan.pos.data(), plan.pos.size());
+        if (!rc) rc = upload(c, &A->d_st_off, plan.off.data(), plan.off.size());
+        if (!rc) rc = upload(c, &A->d_st_tsoff, plan.tsoff.data(), plan.tsoff.size());
+        if (!rc) rc = upload(c, &A->d_st_flag, plan.flag.data(), plan.flag.size());
+        if (!rc) rc = upload(c, &A->d_st_tinfo, plan.tinfo.data(), plan.tinfo.size());
+        if (!rc) rc = upload(c, &A->d_st_tcol, plan.tcol.data(), plan.tcol.size());
+        if (!rc) rc = upload(c, &A->d_col, col, (size_t)nnz);
+        if (!rc) {
+            // the host arrays above are uploaded asynchronously: wait before they go
+            hipError_t e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) rc = hip_fail(e, "stencil upload");
+        }
+        if (!rc && fp32) {
+            hipError_t e = hipMalloc(&A->d_ell_val32, sizeof(float) * (size_t)plan.stored);
+            if (e != hipSuccess) rc = hip_fail(e, "hipMalloc fp32 values");
+            else k_to_f32<<<nblocks(plan.stored), kBlock, 0, c->stream>>>(plan.stored, A->d_ell_val, A->d_ell_val32);
+        }
     } else if (!rc) {
         A->ell_stored = nnz;
         rc = upload(c, &A->d_col, col, (size_t)nnz);
@@ -441,7 +543,8 @@ int gip_mat_autotune(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, cons
     if (!c || !out || !x || !y || reps < 1) return set_error(GIP_EINVAL, "gip_mat_autotune: bad argument");
     *out = nullptr;
     struct Cand { int fmt, lanes; };
-    const Cand cands[5] = {{GIP_FMT_ELL, 1}, {GIP_FMT_CSR, 1}, {GIP_FMT_CSR, 2}, {GIP_FMT_CSR, 4}, {GIP_FMT_CSR, 8}};
+    const Cand cands[6] = {{GIP_FMT_ELL, 1}, {GIP_FMT_CSR, 1}, {GIP_FMT_CSR, 2},
+                           {GIP_FMT_CSR, 4}, {GIP_FMT_CSR, 8}, {GIP_FMT_STENCIL, 1}};
     gip_tune_report r{};
     r.chosen_format = -1;
     float best = 0.0f;
@@ -450,13 +553,15 @@ int gip_mat_autotune(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, cons
     GIP_HIP_TRY(hipEventCreate(&e0));
     GIP_HIP_TRY(hipEventCreate(&e1));
     int rc = GIP_OK;
-    for (int k = 0; k < 5 && !rc; ++k) {
+    for (int k = 0; k < 6 && !rc; ++k) {
         r.us[k] = -1.0f;
         gip_mat *A = nullptr;
-        if (cands[k].fmt == GIP_FMT_ELL) {
-            rc = gip_mat_create(c, nrows, ncols, nnz, rowptr, col, val, GIP_FMT_AUTO | (flags & ~0xff), &A);
+        if (cands[k].fmt == GIP_FMT_ELL || cands[k].fmt == GIP_FMT_STENCIL) {
+            rc = gip_mat_create(c, nrows, ncols, nnz, rowptr, col, val, cands[k].fmt | (flags & ~0xff), &A);
             if (rc) break;
-            if (A->format != GIP_FMT_ELL) {  // too much padding for ELL: not a candidate
+            // not a candidate: the layout does not fit, or ELL would pad heavily
+            if (A->format != cands[k].fmt ||
+                (A->format == GIP_FMT_ELL && A->ell_stored > (int64_t)(1.2 * nnz) + kSlice * 8)) {
                 gip_mat_destroy(A);
                 continue;
             }
@@ -517,7 +622,7 @@ int gip_mat_cols_needed(const gip_mat *A, int32_t *ncols) {
 static int values_csr_order(const gip_mat *A, double *dst_dev) {
     gip_ctx *c = A->ctx;
     if (A->nnz == 0) return GIP_OK;
-    if (A->format == GIP_FMT_ELL) {
+    if (A->format == GIP_FMT_ELL || A->format == GIP_FMT_STENCIL) {
         k_gather_vals<<<nblocks(A->nnz), kBlock, 0, c->stream>>>(A->nnz, A->d_ell_val, A->d_csr2ell, dst_dev);
         GIP_LAUNCH_CHECK("gather values");
     } else if (A->format == GIP_FMT_DIAG) {
@@ -548,7 +653,7 @@ int gip_mat_set_values(gip_mat *A, const double *host) {
     if (!A || (A->nnz && !host)) return set_error(GIP_EINVAL, "gip_mat_set_values: bad argument");
     if (!A->nnz) return GIP_OK;
     gip_ctx *c = A->ctx;
-    if (A->format == GIP_FMT_ELL) {
+    if (A->format == GIP_FMT_ELL || A->format == GIP_FMT_STENCIL) {
         double *tmp = nullptr;
         GIP_HIP_TRY(hipMalloc(&tmp, sizeof(double) * A->nnz));
         hipError_t e = hipMemcpyAsync(tmp, host, sizeof(double) * A->nnz, hipMemcpyHostToDevice, c->stream);
@@ -593,7 +698,7 @@ int gip_dcsrmv_vals(gip_ctx *c, const gip_mat *A, const double *x, gip_mat *B) {
     if (!c || !A || !B || (A->nrows && !x)) return set_error(GIP_EINVAL, "gip_dcsrmv_vals: bad argument");
     if (A->nrows != B->nnz) return set_error(GIP_EINVAL, "gip_dcsrmv_vals: rows of A must equal nnz of B");
     int rc;
-    if (B->format == GIP_FMT_ELL)
+    if (B->format == GIP_FMT_ELL || B->format == GIP_FMT_STENCIL)
         rc = spmv_launch(c, A, x, B->d_ell_val, 1.0, 0.0, 0, B->d_csr2ell);
     else if (B->format == GIP_FMT_DIAG)
         rc = spmv_launch(c, A, x, B->d_diag, 1.0, 0.0, 0, nullptr);
@@ -617,7 +722,9 @@ __global__ __launch_bounds__(kBlock) void k_fill0(int n, double *__restrict__ y)
 }
 
 static double *value_store(const gip_mat *A) {
-    return A->format == GIP_FMT_ELL ? A->d_ell_val : A->format == GIP_FMT_DIAG ? A->d_diag : A->d_val;
+    return (A->format == GIP_FMT_ELL || A->format == GIP_FMT_STENCIL) ? A->d_ell_val
+           : A->format == GIP_FMT_DIAG                                 ? A->d_diag
+                                                                       : A->d_val;
 }
 
 __global__ __launch_bounds__(kBlock) void k_refresh(int n, const double *__restrict__ src, const int32_t *__restrict__ sidx,
@@ -634,7 +741,7 @@ static int build_transpose(gip_ctx *c, gip_mat *A) {
     hipError_t e = hipMemcpyAsync(rp.data(), A->d_rowptr, sizeof(int32_t) * rp.size(), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess && A->nnz)
         e = hipMemcpyAsync(cl.data(), A->d_col, sizeof(int32_t) * cl.size(), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess && A->format == GIP_FMT_ELL && A->nnz) {
+    if (e == hipSuccess && (A->format == GIP_FMT_ELL || A->format == GIP_FMT_STENCIL) && A->nnz) {
         c2e.resize((size_t)A->nnz);
         e = hipMemcpyAsync(c2e.data(), A->d_csr2ell, sizeof(int32_t) * c2e.size(), hipMemcpyDeviceToHost, c->stream);
     }
@@ -657,7 +764,7 @@ static int build_transpose(gip_ctx *c, gip_mat *A) {
     int rc = gip_mat_create(c, nc, A->nrows, A->nnz, trp.data(), tcl.data(), zeros.data(), GIP_FMT_AUTO, &T);
     if (rc) return rc;
     std::vector<int32_t> tdst((size_t)A->nnz);
-    if (T->format == GIP_FMT_ELL && A->nnz) {
+    if ((T->format == GIP_FMT_ELL || T->format == GIP_FMT_STENCIL) && A->nnz) {
         e = hipMemcpy(tdst.data(), T->d_csr2ell, sizeof(int32_t) * tdst.size(), hipMemcpyDeviceToHost);
         if (e != hipSuccess) {
             gip_mat_destroy(T);

@@ -14,7 +14,8 @@ import numpy as np
 
 from ._lib import POINTER, byref, c_double, c_float, c_int, c_int32, c_int64, c_size_t, c_void_p, load
 
-FMT_AUTO, FMT_CSR, FMT_ELL, FMT_DIAG = 0, 1, 2, 3
+FMT_AUTO, FMT_CSR, FMT_ELL, FMT_DIAG, FMT_STENCIL = 0, 1, 2, 3, 4
+FORMAT_NAMES = {FMT_CSR: "csr", FMT_ELL: "sliced-ell", FMT_DIAG: "diag", FMT_STENCIL: "stencil"}
 MAT_FP32 = 1 << 8
 PCG_FP64, PCG_MIXED = 0, 1
 OK, EINVAL, EHIP, ENOMEM, EBREAKDOWN, ENOCONV = 0, -1, -2, -3, -4, -5
@@ -30,7 +31,7 @@ class PcgStats(ctypes.Structure):
 
 
 class TuneReport(ctypes.Structure):
-    _fields_ = [("chosen_format", c_int32), ("chosen_lanes", c_int32), ("us", c_float * 5)]
+    _fields_ = [("chosen_format", c_int32), ("chosen_lanes", c_int32), ("us", c_float * 6)]
 
 
 # exported entry points and their argument types (kept in step with include/gip_hip.h)
@@ -69,6 +70,7 @@ SIGNATURES = {
     "gip_dfill": ([c_void_p, c_int32, c_void_p, c_double], c_int),
     "gip_daxpy": ([c_void_p, c_int32, c_double, c_void_p, c_void_p], c_int),
     "gip_daxpby": ([c_void_p, c_int32, c_double, c_void_p, c_double, c_void_p], c_int),
+    "gip_dcopy": ([c_void_p, c_int32, c_void_p, c_void_p], c_int),
     "gip_ddot": ([c_void_p, c_int32, c_void_p, c_void_p, POINTER(c_double)], c_int),
     "gip_dnrm_inf": ([c_void_p, c_int32, c_void_p, POINTER(c_double)], c_int),
     "gip_cfl": ([c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double,
@@ -309,6 +311,12 @@ def axpy(alpha, x, y, n=None):
 def axpby(alpha, x, beta, y, n=None):
     n = _span(y.n if n is None else n, x, y)
     check(lib().gip_daxpby(y.dev.ctx, n, alpha, x.ptr, beta, y.ptr))
+
+
+def copy(x, y, n=None):
+    """y[0:n] = x[0:n] (GIP_VectorMulticore.cpp:167-181)."""
+    n = _span(y.n if n is None else n, x, y)
+    check(lib().gip_dcopy(y.dev.ctx, n, x.ptr, y.ptr))
 
 
 def dot(x, y, n=None):

@@ -75,11 +75,18 @@ int gip_event_destroy(gip_event *ev);
 /* ---- sparse operators ----------------------------------------------------
  * Device copy of a CSR operator (GIP_MatrixCuda.cpp:14-53).  The host
  * arrays are read once and may be freed afterwards.  The layout is chosen
- * per operator: sliced ELL (64-row slices, column-major inside a slice)
- * when padding stays small, a diagonal vector when the operator is
- * diagonal, CSR otherwise.  `format` forces one of GIP_FMT_*.
+ * per operator: a diagonal vector when the operator is diagonal; the
+ * stencil layout when the rows of each 64-row slice share their column
+ * offsets (structured meshes: per slice and entry slot one offset col - row,
+ * values column-major, the few entries that do not fit kept in a small
+ * CSR tail summed after the slots; no column index per entry); sliced ELL
+ * (64-row slices, column-major inside a slice) when padding stays small;
+ * CSR otherwise.  `format` forces one of GIP_FMT_*; a forced stencil layout
+ * that the operator does not fit falls back to the automatic choice.
+ * Every layout sums each row in stored order (bitwise CSR results for
+ * finite inputs; the stencil layout adds +-0 for slots a row lacks).
  * `ncols` is informational (columns may exceed it, as in the reference). */
-enum { GIP_FMT_AUTO = 0, GIP_FMT_CSR = 1, GIP_FMT_ELL = 2, GIP_FMT_DIAG = 3 };
+enum { GIP_FMT_AUTO = 0, GIP_FMT_CSR = 1, GIP_FMT_ELL = 2, GIP_FMT_DIAG = 3, GIP_FMT_STENCIL = 4 };
 enum { GIP_MAT_FP32 = 1 << 8 }; /* also keep an FP32 copy of the values (mixed precision) */
 int gip_mat_create(gip_ctx *ctx, int32_t nrows, int32_t ncols, int32_t nnz, const int32_t *rowptr, const int32_t *col,
                    const double *val, int32_t flags, gip_mat **A);
@@ -93,14 +100,15 @@ int gip_mat_cols_needed(const gip_mat *A, int32_t *ncols);
  * stored order. */
 int gip_mat_set_lanes(gip_mat *A, int32_t lanes);
 /* In-project SpMV autotuner: builds the operator in each candidate layout
- * (sliced ELL, 64 rows per wavefront; CSR with 1, 2, 4 or 8 lanes per row),
+ * (sliced ELL, 64 rows per wavefront; CSR with 1, 2, 4 or 8 lanes per row;
+ * stencil),
  * times `reps` products y = A x on the device (x, y: device vectors covering
  * the operator) and returns the fastest; the rest are freed.  `report`
  * (optional) gets each candidate's mean time in microseconds (-1: not a
  * candidate) in that order. */
 typedef struct gip_tune_report {
     int32_t chosen_format, chosen_lanes;
-    float us[5];
+    float us[6];
 } gip_tune_report;
 int gip_mat_autotune(gip_ctx *ctx, int32_t nrows, int32_t ncols, int32_t nnz, const int32_t *rowptr, const int32_t *col,
                      const double *val, int32_t flags, const double *x, double *y, int32_t reps, gip_mat **A,
@@ -134,6 +142,7 @@ int gip_scsrmv(gip_ctx *ctx, const gip_mat *A, const float *x, float *y);
 int gip_dfill(gip_ctx *ctx, int32_t n, double *x, double value);                      /* x = value     */
 int gip_daxpy(gip_ctx *ctx, int32_t n, double alpha, const double *x, double *y);      /* y += alpha x  */
 int gip_daxpby(gip_ctx *ctx, int32_t n, double alpha, const double *x, double beta, double *y); /* y = ax+by */
+int gip_dcopy(gip_ctx *ctx, int32_t n, const double *x, double *y);                  /* y = x, copyTo (GIP_VectorMulticore.cpp:167-181) */
 int gip_ddot(gip_ctx *ctx, int32_t n, const double *x, const double *y, double *result);         /* host result */
 int gip_dnrm_inf(gip_ctx *ctx, int32_t n, const double *x, double *result);                    /* max |x_i|  */
 int gip_cfl(gip_ctx *ctx, int32_t n, const double *u, const double *v, const double *w, const double *dxs,

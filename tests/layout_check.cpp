@@ -1,0 +1,91 @@
+// CPU check of the stencil layout planner (gip_amd/csrc/gip_layout.h):
+// reads a CSR operator and an input vector from raw files, plans the
+// layout, forms y = A x from the layout exactly as the device kernels do
+// (slots in order with clamped columns, then the tail) and from the CSR
+// arrays in stored order, and prints one JSON line with the number of rows
+// whose results differ bitwise and the layout statistics.
+//   layout_check <nrows> <nnz> <rowptr.i32> <col.i32> <val.f64> <x.f64> <ncols_x> [max_tail_frac]
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../gip_amd/csrc/gip_layout.h"
+
+template <class T>
+static std::vector<T> load(const char *path, size_t n) {
+    std::vector<T> v(n);
+    FILE *f = std::fopen(path, "rb");
+    if (!f || std::fread(v.data(), sizeof(T), n, f) != n) {
+        std::fprintf(stderr, "cannot read %zu values from %s\n", n, path);
+        std::exit(2);
+    }
+    std::fclose(f);
+    return v;
+}
+
+int main(int argc, char **argv) {
+    if (argc < 8) {
+        std::fprintf(stderr, "usage: layout_check nrows nnz rowptr col val x ncols_x [max_tail_frac]\n");
+        return 2;
+    }
+    const int32_t nrows = std::atoi(argv[1]), nnz = std::atoi(argv[2]), ncx = std::atoi(argv[7]);
+    const double frac = argc > 8 ? std::atof(argv[8]) : 0.05;
+    auto rp = load<int32_t>(argv[3], (size_t)nrows + 1);
+    auto col = load<int32_t>(argv[4], (size_t)nnz);
+    auto val = load<double>(argv[5], (size_t)nnz);
+    auto x = load<double>(argv[6], (size_t)ncx);
+    gip::StencilPlan P;
+    const bool ok = gip::plan_stencil(nrows, nnz, rp.data(), col.data(), P, frac);
+    if (!ok) {
+        std::printf("{\"planned\": false}\n");
+        return 0;
+    }
+    // every position in range and used once
+    std::vector<double> st((size_t)P.stored, 0.0);
+    std::vector<char> used((size_t)P.stored, 0);
+    long bad_pos = 0;
+    for (int32_t j = 0; j < nnz; ++j) {
+        const int32_t p = P.pos[(size_t)j];
+        if (p < 0 || p >= P.stored || used[(size_t)p]) {
+            ++bad_pos;
+            continue;
+        }
+        used[(size_t)p] = 1;
+        st[(size_t)p] = val[(size_t)j];
+    }
+    int32_t cmax = 0;
+    for (int32_t c : col) cmax = std::max(cmax, c);
+    long bad_rows = 0, bad_cols = 0;
+    for (int32_t r = 0; r < nrows; ++r) {
+        double want = 0.0;
+        for (int32_t j = rp[(size_t)r]; j < rp[(size_t)r + 1]; ++j) want = want + x[(size_t)col[(size_t)j]] * val[(size_t)j];
+        const bool empty_csr = rp[(size_t)r] == rp[(size_t)r + 1];
+        const int32_t s = r / 64, lane = r % 64;
+        const uint16_t info = P.tinfo[(size_t)r];
+        double got = 0.0;
+        bool empty = false;
+        for (int32_t k = 0; k < P.w; ++k) {
+            int32_t c = r + P.off[(size_t)s * P.w + k];
+            c = c < 0 ? 0 : (c > cmax ? cmax : c);
+            got = got + x[(size_t)c] * st[(size_t)(((int64_t)s * P.w + k) * 64 + lane)];
+        }
+        if (P.flag[(size_t)s]) {
+            empty = (info & 0x8000) != 0;
+            const int32_t t = P.tsoff[(size_t)s] + (info & 255), nt = (info >> 8) & 127;
+            for (int32_t e = 0; e < nt; ++e) {
+                if (t + e >= (int32_t)P.tcol.size() || P.tcol[(size_t)(t + e)] > cmax) {
+                    ++bad_cols;
+                    continue;
+                }
+                got = got + x[(size_t)P.tcol[(size_t)(t + e)]] * st[(size_t)(P.tail_base + t + e)];
+            }
+        }
+        if (empty != empty_csr || (!empty && std::memcmp(&got, &want, sizeof(double)) != 0)) ++bad_rows;
+    }
+    std::printf("{\"planned\": true, \"w\": %d, \"slices\": %d, \"tail\": %zu, \"stored\": %lld, \"bytes\": %lld, "
+                "\"bad_pos\": %ld, \"bad_cols\": %ld, \"bad_rows\": %ld}\n",
+                P.w, P.ns, P.tcol.size(), (long long)P.stored, (long long)gip::stencil_bytes(P, nrows), bad_pos,
+                bad_cols, bad_rows);
+    return 0;
+}

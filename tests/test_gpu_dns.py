@@ -102,6 +102,17 @@ def test_dns_c100k_readme_and_oracle(c100k_root):
     _compare_fields(sim, orc)
 
 
+@pytest.mark.parametrize("mode", ["reference", "fused"])
+def test_dns_clustered_mesh_re1000_vs_oracle(tmp_path, mode):
+    """The tanh-clustered mesh (beta 1.5) of the 128^3 configuration, at 24^3
+    and Re=1000: non-uniform cells give every operator distinct values."""
+    cavity.generate(24, beta=1.5, gamma=1e-3).write(tmp_path, "clu24")
+    sim = dns.Simulation(root=tmp_path, name="clu24", gamma=1e-3, pcg_mode=mode)
+    orc = pyoracle.Dns(tmp_path, "clu24", gamma=1e-3)
+    _compare_run(sim, orc, 8)
+    _compare_fields(sim, orc)
+
+
 def test_dns_fused_pcg_matches_reference_sequence(cav46_dir):
     sim = dns.Simulation(root=cav46_dir, name="cav46", gamma=1e-2, pcg_mode="fused")
     orc = pyoracle.Dns(cav46_dir, "cav46", gamma=1e-2)

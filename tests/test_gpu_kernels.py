@@ -71,11 +71,58 @@ def test_spmv_bitwise_c100k(dev, c100k, name):
     _check_op(dev, c100k, name)
 
 
-@pytest.mark.parametrize("fmt", [hip.FMT_CSR, hip.FMT_ELL])
+@pytest.mark.parametrize("fmt", [hip.FMT_CSR, hip.FMT_ELL, hip.FMT_STENCIL])
 def test_spmv_forced_layouts(dev, cav46, fmt):
-    for name in ("L", "Ec", "SO", "Gx", "Lb"):
+    # Ec, Gx and Lb have empty or irregular rows: a forced stencil layout that
+    # does not fit falls back to the automatic choice
+    fits = {"L": True, "SO": True, "Mres": True, "Ec": False, "Gx": True, "Lb": False}
+    for name in fits:
         info = _check_op(dev, cav46, name, fmt=fmt)
-        assert info["format"] == fmt
+        if fmt != hip.FMT_STENCIL or fits[name]:
+            assert info["format"] == fmt, (name, info)
+        else:
+            assert info["format"] != hip.FMT_STENCIL, (name, info)
+
+
+def _banded(n, offsets, seed, extra=0.02, empty=(), shuffle_rows=0):
+    """Banded operator with random values, some rows carrying one extra
+    off-band entry at a random place in their stored order, some empty rows
+    and some rows stored out of column order."""
+    rng = np.random.default_rng(seed)
+    rowptr, cols, vals = [0], [], []
+    shuffled = set(rng.choice(n, size=shuffle_rows, replace=False).tolist()) if shuffle_rows else set()
+    for r in range(n):
+        c = [r + d for d in offsets if 0 <= r + d < n]
+        if r in empty:
+            c = []
+        elif rng.random() < extra:
+            c.insert(int(rng.integers(0, len(c) + 1)), int(rng.integers(0, n + 100)))
+        if r in shuffled:
+            rng.shuffle(c)
+        cols += c
+        vals += list(rng.standard_normal(len(c)))
+        rowptr.append(len(cols))
+    return np.array(rowptr, np.int32), np.array(cols, np.int32), np.array(vals)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_stencil_layout_bitwise_with_tails_and_empty_rows(dev, seed):
+    """Off-band entries, empty rows and rows stored out of column order go
+    to the tail; the product keeps every row's stored order."""
+    n = 5000
+    rp, cl, vl = _banded(n, (-70, -1, 0, 1, 70), seed, empty={0, 64, 65, 4999}, shuffle_rows=20)
+    ncx = int(cl.max()) + 1
+    rng = np.random.default_rng(seed + 10)
+    x, y0 = rng.standard_normal(ncx), rng.standard_normal(n)
+    for fp32 in (False, True):
+        M = hip.Matrix(dev, rp, cl, vl, ncols=ncx, fmt=hip.FMT_STENCIL, fp32=fp32)
+        assert M.info()["format"] == hip.FMT_STENCIL
+        dy = dev.to_device(y0)
+        M.spmv_ab(dev.to_device(x), dy, 0.5, -2.0)
+        want = y0.copy()
+        pyoracle.spmv_ab(pyoracle.Csr(rp, cl, vl), x, want, 0.5, -2.0)
+        assert np.array_equal(dy.download(), want), fp32
+        assert np.array_equal(M.values(), vl)
 
 
 @pytest.mark.parametrize("lanes", [2, 4, 8])
@@ -100,7 +147,7 @@ def test_autotuner_picks_a_timed_candidate(dev, cav46):
         A = cav46.op(name)
         M, rep = hip.Matrix.autotune(dev, A, reps=3)
         timed = [u for u in rep["us"] if u > 0]
-        idx = 0 if rep["format"] == hip.FMT_ELL else {1: 1, 2: 2, 4: 3, 8: 4}[rep["lanes"]]
+        idx = {hip.FMT_ELL: 0, hip.FMT_STENCIL: 5}.get(rep["format"]) or {1: 1, 2: 2, 4: 3, 8: 4}[rep["lanes"]]
         assert timed and min(timed) == pytest.approx(rep["us"][idx], rel=1e-6)
         x = np.random.default_rng(0).standard_normal(_ncols(cav46, A))
         dy = dev.to_device(np.zeros(A.nrows))
@@ -112,7 +159,10 @@ def test_autotuner_picks_a_timed_candidate(dev, cav46):
 
 
 def test_layout_choice(dev, cav46):
-    assert hip.Matrix.from_csr(dev, cav46.op("L")).info()["format"] == hip.FMT_ELL
+    # the 7-point Laplacian: per-slice offsets shared by the rows (stencil)
+    assert hip.Matrix.from_csr(dev, cav46.op("L")).info()["format"] == hip.FMT_STENCIL
+    # convection from faces (empty rows, irregular): ELL or CSR
+    assert hip.Matrix.from_csr(dev, cav46.op("Ec")).info()["format"] in (hip.FMT_ELL, hip.FMT_CSR)
     A = cav46.op("L")
     n = A.nrows
     D = hip.Matrix(dev, np.arange(n + 1), np.arange(n), np.ones(n))
@@ -178,6 +228,10 @@ def test_vector_ops(dev, n):
     assert hip.nrm_inf(dx) == (np.abs(x).max() if n else 0.0)
     hip.fill(dy, 3.5)
     assert np.all(dy.download() == 3.5)
+    if n:
+        hip.copy(dx, dy, n - 1)  # copyTo over a prefix leaves the tail alone
+        got = dy.download()
+        assert np.array_equal(got[:n - 1], x[:n - 1]) and got[n - 1] == 3.5
 
 
 def test_norm_ignores_nan_like_the_cpu(dev):
@@ -252,6 +306,29 @@ def test_pcg_mixed_precision_reaches_fp64_residual(dev, cav46):
     want, st, it, rel = pyoracle.pcg(pyoracle.Csr.of(L), b, x0, n, 5000, tol=1e-12)
     assert np.linalg.norm(x - want) <= 1e-7 * np.linalg.norm(want)
     print(f"[mixed] iters {r.iters} corrections {r.corrections} true {true_rel:.3e} fp64 iters {it}")
+
+
+@pytest.mark.parametrize("precision", [hip.PCG_FP64, hip.PCG_MIXED])
+def test_pcg_layouts_give_identical_iterates(dev, cav46, precision):
+    """Stencil, ELL and CSR operators all sum each row in stored order, so
+    the PCG iterates agree bitwise whichever layout holds L."""
+    L = cav46.op("L")
+    n, nall = cav46.n_inner, cav46.n_all
+    b = np.zeros(nall)
+    b[:n] = cavity.rhs_normal(n, 5)
+    z = dev.to_device(np.zeros(nall))
+    outs = []
+    for fmt in (hip.FMT_STENCIL, hip.FMT_ELL, hip.FMT_CSR):
+        A = hip.Matrix.from_csr(dev, L, fmt=fmt, fp32=True)
+        assert A.info()["format"] == fmt
+        solver = hip.PCG(A, nall)
+        dx = dev.array(nall)
+        r = solver.solve(dev.to_device(b), dx, z, maxit=60, precision=precision)
+        outs.append((dx.download(), r.iters, r.corrections))
+        solver.destroy()
+    for x, it, corr in outs[1:]:
+        assert it == outs[0][1] and corr == outs[0][2]
+        assert np.array_equal(x, outs[0][0])
 
 
 def test_pcg_breakdown_is_reported(dev, cav46):
