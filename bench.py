@@ -263,7 +263,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n),
                      "algorithmic_bytes": alg,
-                     "kernel": "k_ell<0,7> (FP64 sliced-ELL SpMV, 256^3 Laplacian)"},
+                     "kernel": {4: f"k_stencil<0,-{info.get('width', 7)}> (FP64 stencil-layout SpMV)",
+                                2: "k_ell<0,7> (FP64 sliced-ELL SpMV)"}.get(info["format"], "FP64 SpMV")
+                               + f", {n}^3 Laplacian"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -180,7 +180,7 @@ int fwd_step(gip_adjoint *a, double dt) {
     TRY(gip_dcsrmv(c, a->op[A_MX], a->u, a->rhs));
     TRY(gip_dcsrmv_ab(c, a->op[A_MY], a->v, a->rhs, 1.0, 1.0));
     TRY(gip_dcsrmv_ab(c, a->op[A_MZ], a->w, a->rhs, sigma, sigma));
-    gip_pcg_opts po{a->o.pcg_maxit, a->o.pcg_tol, 1, GIP_PCG_FP64, 0.0, 32};
+    gip_pcg_opts po{a->o.pcg_maxit, a->o.pcg_tol, 1, GIP_PCG_FP64, 0.0, 32, 0};
     gip_pcg_stats ps{};
     TRY(gip_pcg_solve(a->pcg, a->rhs, a->p, a->p0, &po, &ps));
     TRY(copy(a, a->p0, a->p, N));
@@ -237,7 +237,7 @@ int adj_step(gip_adjoint *a, double dt) {
     TRY(gip_dfill(c, NA, a->bp, 0.0));
     TRY(copy(a, a->bp, a->bpe, N));
     TRY(tmul(a, a->op[A_LB], a->bpe, N, 1.0, a->bp));
-    gip_pcg_opts po{a->o.pcg_maxit, a->o.pcg_tol, 1, GIP_PCG_FP64, 0.0, 32};
+    gip_pcg_opts po{a->o.pcg_maxit, a->o.pcg_tol, 1, GIP_PCG_FP64, 0.0, 32, 0};
     gip_pcg_stats ps{};
     TRY(gip_pcg_solve(a->pcg, a->bp, a->brhs, a->zero, &po, &ps));
     TRY(tmul(a, a->op[A_MX], a->brhs, NA, sigma, a->bu));

@@ -22,7 +22,7 @@ constexpr int kChunk = kBlock;  // rows per chunk = threads per workgroup
 enum { F_A1 = 0, F_A2, F_SIGMA, F_CORR, F_F1, F_F2, F_RHO, F_COUNT = 8 };
 enum { FAC_ONE = -1, FAC_ZERO = -2 };
 // per-instance PCG scalars
-enum { B_RHO = 0, B_ALPHA, B_BETA, B_RR, B_BB, B_TOL2, B_RHO_PREV, B_LAST_RR, B_NEED, B_COUNT = 16 };
+enum { B_RHO = 0, B_ALPHA, B_BETA, B_RR, B_BB, B_TOL2, B_RHO_PREV, B_LAST_RR, B_NEED, B_ZAZ, B_C, B_COUNT = 16 };
 enum { BS_RUN = 0, BS_CONV = 1, BS_BREAK = 2, BS_MAXED = 3 };
 
 struct Seg {
@@ -186,7 +186,8 @@ struct BPcg {
     double *x, *r, *s, *Ap;
     float *r32, *s32, *Ap32, *xc32;
     const float *dinv32;
-    double *sc;  // B * B_COUNT
+    const double *one, *az;  // MIXED coarse step: 1 on real inner rows (0 on padding), A 1
+    double *sc;              // B * B_COUNT
     int *state, *iters, *active;
     double *part;  // chunk totals
     int maxit;
@@ -344,11 +345,44 @@ __global__ __launch_bounds__(kBlock) void k_bp_update_fin(BPcg d, Seg S) {
     }
 }
 
-// mixed: FP32 copies of the FP64 residual of x0; rho, b.b, true r.r
+// MIXED coarse step of each instance (the single-cavity solver's, per
+// instance): chunk partials of 1.r over the running instances (only those
+// flagged for a correction when need_only) ...
+__global__ __launch_bounds__(kBlock) void k_bpm_sumr(BPcg d, Seg S, int need_only) {
+    FOR_CHUNKS(S, c) {
+        const int inst = S.chunk_inst[c];
+        if (d.state[inst] != BS_RUN || (need_only && bsc(d, inst)[B_NEED] == 0.0)) continue;
+        const int row = c * kChunk + threadIdx.x;
+        double v[1] = {d.r[row] * d.one[row]};
+        seg_store<1, RED_SUM>(v, c, d.part);
+    }
+}
+
+// ... c = (1.r)/(1.A1) per instance (0 when 1.A1 is not positive and finite)
+__global__ __launch_bounds__(kBlock) void k_bpm_c_fin(BPcg d, Seg S, int need_only) {
+    const int inst = blockIdx.x;
+    if (d.state[inst] != BS_RUN || (need_only && bsc(d, inst)[B_NEED] == 0.0)) return;
+    double out[1];
+    seg_total<1, RED_SUM>(d.part, inst, S, out);
+    if (threadIdx.x == 0) {
+        double *q = bsc(d, inst);
+        const double zaz = q[B_ZAZ];
+        double cc = (zaz > 0.0 && isfinite(zaz)) ? out[0] / zaz : 0.0;
+        q[B_C] = isfinite(cc) ? cc : 0.0;
+    }
+}
+
+// mixed: the coarse step, then FP32 copies of the FP64 residual of x0; rho, b.b, true r.r
 __global__ __launch_bounds__(kBlock) void k_bpm_setup(BPcg d, Seg S, int precond) {
     FOR_CHUNKS(S, c) {
         const int row = c * kChunk + threadIdx.x;
-        const double ri = d.r[row];
+        const double cc = bsc(d, S.chunk_inst[c])[B_C];
+        double ri = d.r[row];
+        if (cc != 0.0) {
+            d.x[row] = d.x[row] + cc * d.one[row];
+            ri = ri - cc * d.az[row];
+            d.r[row] = ri;
+        }
         const float r32 = (float)ri;
         const float s32 = precond ? r32 * d.dinv32[row] : r32;
         d.r32[row] = r32;
@@ -439,10 +473,11 @@ __global__ __launch_bounds__(kBlock) void k_bpm_fold(BPcg d, Seg S) {
     }
 }
 
-// ... then r = b - A x (FP64) replaces the FP32 residual; the direction is
-// kept and beta uses the replaced residual; convergence on the true ||r||.
+// ... then r = b - A x (FP64) replaces the FP32 residual (after the coarse
+// step); the direction is kept and beta uses the replaced residual;
+// convergence on the true ||r||.
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_bpm_resid(EllView A, BPcg d, Seg S, int precond) {
+__global__ __launch_bounds__(kBlock) void k_bpm_resid(EllView A, BPcg d, Seg S) {
     Operand<double> xo{d.x, nullptr, 0.0, d.np, false};
     FOR_CHUNKS(S, c) {
         const int inst = S.chunk_inst[c];
@@ -453,6 +488,23 @@ __global__ __launch_bounds__(kBlock) void k_bpm_resid(EllView A, BPcg d, Seg S, 
         const double t = any ? sum : d.x[row];
         const double ri = d.b[row] + -1.0 * t;
         d.r[row] = ri;
+        double v[1] = {ri * d.one[row]};
+        seg_store<1, RED_SUM>(v, c, d.part);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bpm_coarse(BPcg d, Seg S, int precond) {
+    FOR_CHUNKS(S, c) {
+        const int inst = S.chunk_inst[c];
+        if (d.state[inst] != BS_RUN || bsc(d, inst)[B_NEED] == 0.0) continue;
+        const int row = c * kChunk + threadIdx.x;
+        const double cc = bsc(d, inst)[B_C];
+        double ri = d.r[row];
+        if (cc != 0.0) {
+            d.x[row] = d.x[row] + cc * d.one[row];
+            ri = ri - cc * d.az[row];
+            d.r[row] = ri;
+        }
         const float r32 = (float)ri;
         const float s32 = precond ? r32 * d.dinv32[row] : r32;
         d.r32[row] = r32;
@@ -528,6 +580,7 @@ struct gip_batch {
     // pcg
     double *r = nullptr, *s = nullptr, *Ap = nullptr, *dinv = nullptr, *pp[2] = {};
     float *r32 = nullptr, *s32 = nullptr, *Ap32 = nullptr, *xc32 = nullptr, *dinv32 = nullptr, *pq[2] = {};
+    double *cone = nullptr, *caz = nullptr;  // MIXED coarse step: 1 on inner rows, A 1
     double *sc = nullptr, *part = nullptr;
     int *state = nullptr, *iters = nullptr, *active = nullptr, *h_active = nullptr;
     bool started = false;
@@ -650,6 +703,8 @@ BPcg pcg_view(gip_batch *b) {
     d.Ap32 = b->Ap32;
     d.xc32 = b->xc32;
     d.dinv32 = b->dinv32;
+    d.one = b->cone;
+    d.az = b->caz;
     d.sc = b->sc;
     d.state = b->state;
     d.iters = b->iters;
@@ -694,6 +749,8 @@ int enqueue_pre(gip_batch *b) {
     BDISPATCH(L, b->grid_in, k_bp_setup_resid, view_of(L), d, b->seg_in, b->p0, b->pp[0], b->pp[1],
               mixed ? b->pq[0] : nullptr, mixed ? b->pq[1] : nullptr);
     if (mixed) {
+        k_bpm_sumr<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, 0);
+        k_bpm_c_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in, 0);
         k_bpm_setup<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, 1);
         k_bpm_setup_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in, b->o.pcg_tol);
     } else {
@@ -728,7 +785,9 @@ int enqueue_iter(gip_batch *b) {
     }
     if (mixed) {
         k_bpm_fold<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in);
-        BDISPATCH(L, b->grid_in, k_bpm_resid, view_of(L), d, b->seg_in, 1);
+        BDISPATCH(L, b->grid_in, k_bpm_resid, view_of(L), d, b->seg_in);
+        k_bpm_c_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in, 1);
+        k_bpm_coarse<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, 1);
         k_bpm_resid_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in);
     }
     GIP_LAUNCH_CHECK("batch pcg iterations");
@@ -874,6 +933,8 @@ int gip_batch_create(gip_ctx *c, int32_t ninst, const gip_cavity *const *cavs, c
             if (!rc) rc = dalloc_t(b, f, NPs);
         if (!rc) rc = dalloc_t(b, &b->pq[0], NA);
         if (!rc) rc = dalloc_t(b, &b->pq[1], NA);
+        if (!rc) rc = dalloc_t(b, &b->cone, NPs);
+        if (!rc) rc = dalloc_t(b, &b->caz, NPs);
     }
     if (!rc) rc = dalloc_t(b, &b->sc, (size_t)B * B_COUNT);
     if (!rc) rc = dalloc_t(b, &b->part, (size_t)std::max(b->NP, b->FP) / kChunk * 3 + 8);
@@ -943,6 +1004,27 @@ int gip_batch_create(gip_ctx *c, int32_t ninst, const gip_cavity *const *cavs, c
             hipError_t e = hipMemcpyAsync(b->dinv, di.data(), di.size() * 8, hipMemcpyHostToDevice, c->stream);
             if (e == hipSuccess && b->dinv32)
                 e = hipMemcpyAsync(b->dinv32, di32.data(), di32.size() * 4, hipMemcpyHostToDevice, c->stream);
+            // MIXED coarse step: 1 on each instance's inner rows, A 1 in stored
+            // order (columns outside the inner rows count 0), 1.A1 per instance
+            std::vector<double> one((size_t)b->NP, 0.0), az((size_t)b->NP, 0.0), hsc((size_t)B * B_COUNT, 0.0);
+            if (b->cone) {
+                for (int i = 0; i < B; ++i)
+                    for (int64_t r = 0; r < b->N[i]; ++r) one[(size_t)(b->off_in[i] + r)] = 1.0;
+                for (int64_t r = 0; r < b->NP; ++r) {
+                    double t = 0.0;
+                    for (int32_t j = rp[(size_t)r]; j < rp[(size_t)r + 1]; ++j)
+                        t = t + vl[(size_t)j] * (cl[(size_t)j] < b->NP ? one[(size_t)cl[(size_t)j]] : 0.0);
+                    az[(size_t)r] = t;
+                }
+                for (int i = 0; i < B; ++i) {
+                    double z = 0.0;
+                    for (int64_t r = 0; r < b->N[i]; ++r) z = z + az[(size_t)(b->off_in[i] + r)];
+                    hsc[(size_t)i * B_COUNT + B_ZAZ] = z;
+                }
+                if (e == hipSuccess) e = hipMemcpyAsync(b->cone, one.data(), one.size() * 8, hipMemcpyHostToDevice, c->stream);
+                if (e == hipSuccess) e = hipMemcpyAsync(b->caz, az.data(), az.size() * 8, hipMemcpyHostToDevice, c->stream);
+                if (e == hipSuccess) e = hipMemcpyAsync(b->sc, hsc.data(), hsc.size() * 8, hipMemcpyHostToDevice, c->stream);
+            }
             if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
             if (e != hipSuccess) rc = hip_fail(e, "batch dinv upload");
         }

@@ -33,6 +33,12 @@
 // search direction is kept: reliable-update residual replacement, restarted
 // with beta = 0 when the recursive residual had drifted more than 10x from
 // the true one), and convergence is decided on that FP64 residual only.
+// Each correction also takes the FP64 Galerkin step along the constant
+// vector 1 of the solved rows, x += c 1 with c = (1.r)/(1.A1): for the
+// pinned Neumann pressure Laplacian that vector is the near-null mode
+// (eigenvalue ~ pin/N), which FP32 arithmetic cannot resolve; left to the
+// FP32 iteration it re-enters through round-off and the corrections
+// diverge.  For an SPD operator the step never increases the A-norm error.
 #include <cmath>
 #include <new>
 
@@ -51,12 +57,13 @@ enum {
     S_BB = 4,       // b.b
     S_TOL2 = 6,     // (tol ||b||)^2, negative when no tolerance
     S_TRUE_RR = 7,  // FP64 true residual of the last correction (MIXED)
+    S_ZAZ = 8,      // 1.A1 over the solved rows (MIXED coarse step)
     S_COUNT = 16
 };
 // State word: 0 running, 1 converged, 2 breakdown.
 enum { ST_RUN = 0, ST_CONV = 1, ST_BREAK = 2 };
 // Partial-sum arrays, kMaxRedBlocks doubles each.
-enum { P_DN = 0, P_RS = 1, P_RR = 2, P_BB = 3, P_RS2 = 4, P_RR2 = 5, P_COUNT = 6 };
+enum { P_DN = 0, P_RS = 1, P_RR = 2, P_BB = 3, P_RS2 = 4, P_RR2 = 5, P_SUMR = 6, P_COUNT = 7 };
 
 struct PcgDev {
     int n, G;  // solved rows, workgroups of every iteration kernel
@@ -64,7 +71,8 @@ struct PcgDev {
     const double *b, *dinv;
     float *r32, *s32, *Ap32, *xc32;
     const float *dinv32;
-    double *sc;  // S_COUNT scalars
+    const double *az;  // A 1 on the solved rows (MIXED)
+    double *sc;        // S_COUNT scalars
     int *state, *iters;
     double *part;  // P_COUNT x kMaxRedBlocks partials
     int precond;
@@ -275,11 +283,11 @@ __global__ __launch_bounds__(kBlock) void k_m_fold(PcgDev d) {
     }
 }
 
-// Defect correction: r = b - A x in FP64 (empty rows: x), r32 = r,
-// s32 = Dinv r32; partials of r32.s32 and of the true r.r.
+// Defect correction, part 1: r = b - A x in FP64 (empty rows: x) and the
+// partials of 1.r.
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_m_resid(EllView A, PcgDev d) {
-    double rs = 0.0, rr = 0.0;
+    double sr = 0.0;
     Operand<double> xo{d.x, nullptr, 0.0, d.n, false};
     FOR_ROWS(d.n, row) {
         double sum;
@@ -287,6 +295,28 @@ __global__ __launch_bounds__(kBlock) void k_m_resid(EllView A, PcgDev d) {
         double t = any ? sum : d.x[row];
         double ri = d.b[row] + -1.0 * t;
         d.r[row] = ri;
+        sr = sr + ri;
+    }
+    sr = block_reduce<RED_SUM>(sr);
+    if (threadIdx.x == 0) d.P(P_SUMR)[blockIdx.x] = sr;
+}
+
+// Part 2: the coarse step c = (1.r)/(1.A1) (skipped when off, or when 1.A1
+// is not positive and finite): x += c, r -= c A1 on the solved rows; then
+// r32 = r, s32 = Dinv r32, partials of r32.s32 and of the true r.r.
+__global__ __launch_bounds__(kBlock) void k_m_coarse(PcgDev d, int use) {
+    const double sr = sum_partials<RED_SUM>(d.P(P_SUMR), d.G);
+    const double zaz = d.sc[S_ZAZ];
+    double c = (use && zaz > 0.0 && isfinite(zaz)) ? sr / zaz : 0.0;
+    if (!isfinite(c)) c = 0.0;
+    double rs = 0.0, rr = 0.0;
+    FOR_ROWS(d.n, row) {
+        double ri = d.r[row];
+        if (c != 0.0) {
+            d.x[row] = d.x[row] + c;
+            ri = ri - c * d.az[row];
+            d.r[row] = ri;
+        }
         float r32 = (float)ri;
         d.r32[row] = r32;
         float s32 = d.precond ? r32 * d.dinv32[row] : r32;
@@ -334,6 +364,13 @@ __global__ __launch_bounds__(kBlock) void k_to32(int n, const double *__restrict
     if (i < n) b[i] = (float)a[i];
 }
 
+// z = 1 on rows [0, n), 0 on [n, n_all); az = 0 on [0, n)
+__global__ __launch_bounds__(kBlock) void k_ones(int n, int n_all, double *__restrict__ z, double *__restrict__ az) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n_all) z[i] = i < n ? 1.0 : 0.0;
+    if (i < n) az[i] = 0.0;
+}
+
 }  // namespace gip
 
 using namespace gip;
@@ -344,6 +381,7 @@ struct gip_pcg {
     int n, n_all, grid;
     double *r, *p[2], *s, *Ap, *dinv;
     float *r32, *p32[2], *s32, *Ap32, *xc32, *dinv32;
+    double *az;  // A 1 (MIXED coarse step)
     double *sc, *part;
     int *state;    // state, iters
     int *h_state;  // pinned copy
@@ -391,6 +429,7 @@ static PcgDev make_dev(gip_pcg *S, const double *b, double *x, int precond) {
     d.Ap32 = S->Ap32;
     d.xc32 = S->xc32;
     d.dinv32 = S->dinv32;
+    d.az = S->az;
     d.sc = S->sc;
     d.state = S->state;
     d.iters = S->state + 1;
@@ -454,8 +493,8 @@ int gip_pcg_destroy(gip_pcg *S) {
     if (!S) return GIP_OK;
     hipSetDevice(S->ctx->device);
     hipStreamSynchronize(S->ctx->stream);
-    void *ptrs[] = {S->r,   S->p[0], S->p[1], S->s,      S->Ap, S->dinv, S->r32,  S->p32[0],
-                    S->p32[1], S->s32, S->Ap32, S->xc32, S->dinv32, S->sc, S->part, S->state};
+    void *ptrs[] = {S->r,      S->p[0], S->p[1], S->s,      S->Ap, S->dinv, S->r32,  S->p32[0],
+                    S->p32[1], S->s32,  S->Ap32, S->xc32, S->dinv32, S->az, S->sc, S->part, S->state};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (S->h_state) hipHostFree(S->h_state);
@@ -473,6 +512,7 @@ static int ensure_mixed(gip_pcg *S) {
     size_t n = (size_t)std::max(1, S->n), na = (size_t)std::max(1, S->n_all);
     float **bufs[] = {&S->r32, &S->s32, &S->Ap32, &S->xc32, &S->dinv32};
     for (float **b : bufs) GIP_HIP_TRY(hipMalloc(b, n * 4));
+    GIP_HIP_TRY(hipMalloc(&S->az, n * 8));
     // the FP32 directions are SpMV inputs: ghost columns included, and zero
     // there (the correction leaves the ghost values of x unchanged)
     for (int k = 0; k < 2; ++k) {
@@ -484,6 +524,20 @@ static int ensure_mixed(gip_pcg *S) {
         GIP_LAUNCH_CHECK("dinv32");
     }
     return GIP_OK;
+}
+
+// az = A 1 over the solved rows and 1.A1 into S_ZAZ, from A's current
+// values (one product and one reduction per mixed solve); the first
+// direction buffer serves as the vector 1 before the setup overwrites it.
+static int coarse_setup(gip_pcg *S) {
+    gip_ctx *c = S->ctx;
+    if (S->n_all == 0) return GIP_OK;
+    double *z = S->p[1];
+    k_ones<<<(S->n_all + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(S->n, S->n_all, z, S->az);
+    GIP_LAUNCH_CHECK("coarse vector");
+    int rc = spmv_launch(c, S->A, z, S->az, 1.0, 0.0, 0, nullptr);
+    if (!rc) rc = launch_reduce(c, RED_SUM, S->n, S->az, z, S->sc + S_ZAZ);
+    return rc;
 }
 
 static int poll(gip_pcg *S, bool scalars) {
@@ -505,9 +559,14 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
         int rc = ensure_mixed(S);
         if (rc) return rc;
     }
+    if (mixed) {
+        int rc = coarse_setup(S);
+        if (rc) return rc;
+    }
     PcgDev d = make_dev(S, b, x, o->precond ? 1 : 0);
     EllView v = view(A);
     const int pollk = o->poll > 0 ? o->poll : 16;
+    const int use_coarse = o->coarse >= 0 ? 1 : 0;
     // setup, GIP_LinearSolver.h:55-80
     DISPATCH_W(A, k_setup_resid, NOTAIL, v, d, x0, S->n_all, S->p[0], S->p[1]);
     k_setup_dir<<<S->grid, kBlock, 0, st>>>(d, S->p[0]);
@@ -533,8 +592,9 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
         k_fin<<<1, kBlock, 0, st>>>(d);
         GIP_LAUNCH_CHECK("pcg final test");
     } else {
-        // FP32 residual of x0 and the first direction
+        // FP64 residual of x0 (with the coarse step), FP32 copies, first direction
         DISPATCH_W(A, k_m_resid, NOTAIL, v, d);
+        k_m_coarse<<<S->grid, kBlock, 0, st>>>(d, use_coarse);
         k_m_corr_fin<<<1, kBlock, 0, st>>>(d, 0, 1);
         k_m_start<<<S->grid, kBlock, 0, st>>>(d, S->p32[0]);
         GIP_LAUNCH_CHECK("mixed setup");
@@ -562,6 +622,7 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
             if (rr32 <= inner2 * last_rr || near_target || k >= o->maxit) {
                 k_m_fold<<<S->grid, kBlock, 0, st>>>(d);
                 DISPATCH_W(A, k_m_resid, NOTAIL, v, d);
+                k_m_coarse<<<S->grid, kBlock, 0, st>>>(d, use_coarse);
                 k_m_corr_fin<<<1, kBlock, 0, st>>>(d, k, 0);
                 GIP_LAUNCH_CHECK("mixed correction");
                 ++corrections;

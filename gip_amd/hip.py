@@ -23,7 +23,7 @@ OK, EINVAL, EHIP, ENOMEM, EBREAKDOWN, ENOCONV = 0, -1, -2, -3, -4, -5
 
 class PcgOpts(ctypes.Structure):
     _fields_ = [("maxit", c_int32), ("tol", c_double), ("precond", c_int32), ("precision", c_int32),
-                ("inner_tol", c_double), ("poll", c_int32)]
+                ("inner_tol", c_double), ("poll", c_int32), ("coarse", c_int32)]
 
 
 class PcgStats(ctypes.Structure):
@@ -359,10 +359,10 @@ class PCG:
         self.dev._adopt(self)
 
     def solve(self, b, x, x0, *, maxit=100, tol=0.0, precond=True, precision=PCG_FP64, inner_tol=0.1,
-              poll=16, raise_on_breakdown=True):
+              poll=16, coarse=True, raise_on_breakdown=True):
         if b.n < self.A.nrows or x.n < self.n_all or x0.n < self.n_all:
             raise ValueError("PCG vectors shorter than the solver's layout")
-        o = PcgOpts(maxit, tol, 1 if precond else 0, precision, inner_tol, poll)
+        o = PcgOpts(maxit, tol, 1 if precond else 0, precision, inner_tol, poll, 0 if coarse else -1)
         st = PcgStats()
         rc = lib().gip_pcg_solve(self.ptr, b.ptr, x.ptr, x0.ptr, byref(o), byref(st))
         if rc != 0 and (raise_on_breakdown or rc != EBREAKDOWN):

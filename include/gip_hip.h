@@ -171,6 +171,11 @@ typedef struct gip_pcg_opts {
     int32_t precision; /* GIP_PCG_FP64 or GIP_PCG_MIXED                         */
     double inner_tol;  /* MIXED: residual drop that triggers an FP64 correction */
     int32_t poll;      /* iterations queued between host polls (0: 16)          */
+    int32_t coarse;    /* MIXED: at every FP64 correction, the Galerkin step
+                          x += c 1, c = (1.r)/(1.A1), along the constant vector
+                          of the solved rows (the near-null mode of a pinned
+                          Neumann Laplacian, which FP32 cannot resolve):
+                          0 on (default), -1 off                                  */
 } gip_pcg_opts;
 typedef struct gip_pcg_stats {
     int32_t iters;       /* iterations done                                     */

@@ -42,6 +42,16 @@ def _rel(a, b):
     return abs(a - b) / abs(b) if b else abs(a)
 
 
+# massr = ||Mres mf||_inf = dt ||L0 p - rhs||_inf (L0: L without the pin) is
+# dt |pin p_0| plus the solver residual; on these small cases it is often at
+# round-off level (1e-17), where only an absolute comparison is meaningful.
+MASSR_FLOOR = 1e-14
+
+
+def _massr_close(g, w, rel):
+    return abs(g - w) <= rel * abs(w) + MASSR_FLOOR
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_batch_fp64_matches_oracle_per_instance(dev, tmp_path_factory, graph):
     cavs, dirs = _cases(tmp_path_factory, SIZES, GAMMAS)
@@ -62,7 +72,7 @@ def test_batch_fp64_matches_oracle_per_instance(dev, tmp_path_factory, graph):
             if same[k]:  # time and dt to 1e-10 while the solves stop at the same iteration
                 for key in ("time", "dt"):
                     assert _rel(g[key], w[key]) <= 1e-10, (step, k, key, g[key], w[key])
-                assert _rel(g["massr"], w["massr"]) <= 1e-4, (step, k, g["massr"], w["massr"])
+                assert _massr_close(g["massr"], w["massr"], 1e-4), (step, k, g["massr"], w["massr"])
             same[k] = same[k] and g["pcg_iters"] == w["pcg_iters"]
     for k, o in enumerate(orcs):
         for f in ("u", "v", "w", "p"):
@@ -83,8 +93,9 @@ def test_batch_fixed_iterations_match_oracle_to_1e10(dev, tmp_path_factory):
         for k, o in enumerate(orcs):
             w, g = o.step(), b.info(k)
             assert g["pcg_iters"] == w["pcg_iters"] == 100
-            for key in ("time", "dt", "massr"):
+            for key in ("time", "dt"):
                 assert _rel(g[key], w[key]) <= 1e-10, (step, k, key, g[key], w[key])
+            assert _massr_close(g["massr"], w["massr"], 1e-10), (step, k, g["massr"], w["massr"])
     for k, o in enumerate(orcs):
         for f in ("u", "v", "w", "p", "mf"):
             a, r = b.field(k, f), o.field(f)
@@ -120,7 +131,7 @@ def test_batch_mixed_precision_close_to_fp64_oracle(dev, tmp_path_factory):
             w = o.step()
             g = b.info(k)
             assert g["pcg_status"] == 0
-            assert _rel(g["massr"], w["massr"]) <= 1e-6
+            assert _massr_close(g["massr"], w["massr"], 1e-6), (k, g["massr"], w["massr"])
     for k, o in enumerate(orcs):
         for f in ("u", "v", "w", "p"):
             a, r = b.field(k, f), o.field(f)

@@ -147,7 +147,9 @@ def test_autotuner_picks_a_timed_candidate(dev, cav46):
         A = cav46.op(name)
         M, rep = hip.Matrix.autotune(dev, A, reps=3)
         timed = [u for u in rep["us"] if u > 0]
-        idx = {hip.FMT_ELL: 0, hip.FMT_STENCIL: 5}.get(rep["format"]) or {1: 1, 2: 2, 4: 3, 8: 4}[rep["lanes"]]
+        idx = {hip.FMT_ELL: 0, hip.FMT_STENCIL: 5}.get(rep["format"])
+        if idx is None:
+            idx = {1: 1, 2: 2, 4: 3, 8: 4}[rep["lanes"]]
         assert timed and min(timed) == pytest.approx(rep["us"][idx], rel=1e-6)
         x = np.random.default_rng(0).standard_normal(_ncols(cav46, A))
         dy = dev.to_device(np.zeros(A.nrows))
