@@ -1,12 +1,12 @@
 // Fused, device-resident Jacobi-PCG of libgip_hip (GIP_LinearSolver.h:45-117).
 //
-// FP64 iteration k, two kernels, the reference recurrences and roundings:
-//   K1  rho_k = sum of the r.s partials, beta = rho_k / rho_{k-1};
-//       p_k = 1.0 s + beta p_{k-1}   (k >= 1; formed on the fly for every
-//       gathered column, stored once for the own rows; the first iteration
-//       uses p_0 from the setup), Ap = A p_k, partials of p_k.Ap
+// FP64 iteration k, three kernels, the reference recurrences and roundings:
+//   K0  rho_k = sum of the r.s partials; convergence test ||r|| <= tol ||b||;
+//       beta = rho_k / rho_{k-1}; p = 1.0 s + beta p on the solved rows
+//       (GIP_LinearSolver.h:107-113; iteration 0 uses p from the setup)
+//   K1  Ap = A p, partials of p.Ap                             (:87-91)
 //   K2  dn = sum of the p.Ap partials, alpha = rho_k / dn;
-//       x += alpha p_k ; r += (-alpha) Ap ; s = Dinv r ; partials of r.s, r.r
+//       x += alpha p ; r += (-alpha) Ap ; s = Dinv r ; partials of r.s, r.r
 // Cross-workgroup communication is fence-free: every workgroup stores its
 // partial sums with plain stores, and every workgroup of the NEXT kernel
 // re-reduces all partials in its prologue in the same fixed order, so all
@@ -16,16 +16,13 @@
 // a kernel (rho of the previous iteration, the state word) are written by
 // workgroup 0 only, into slots nobody reads in the same kernel (rho uses two
 // slots by iteration parity).
-// The direction lives in two buffers used alternately (K1 of iteration k
-// reads p_{k-1} from one and writes p_k to the other), so no workgroup can
-// see a half-updated direction.  Forming p on the fly in K1 computes
-// exactly the expression the reference stores (GIP_LinearSolver.h:113).
-// Every kernel first reads the solver's state word and returns when the
-// solve has finished, so the host queues many iterations and polls the
-// state only now and then.  Convergence ||r|| <= tol ||b|| is tested in the
-// prologue of K1 (and by a one-workgroup kernel after the last iteration).
+// The direction update has its own streaming kernel so that the product
+// gathers one vector (p) per neighbour; forming p inside the product would
+// gather s and the old p for every neighbour.  Every kernel first reads the
+// solver's state word and returns when the solve has finished, so the host
+// queues many iterations and polls the state only now and then.
 //
-// MIXED: the same two kernels in FP32 (operator values, vectors and SpMV;
+// MIXED: the same three kernels in FP32 (operator values, vectors and SpMV;
 // dot products accumulate in FP64) on the correction equation, inside an
 // FP64 defect-correction loop: when the FP32 residual has dropped by
 // inner_tol (or reaches the target) the FP32 correction is added to the
@@ -89,14 +86,11 @@ __device__ inline double rho_slot(const PcgDev &d, int k) { return d.sc[S_RHO0 +
 // Ap = (A x0)_i or x0_i on empty rows, r = b + (-1.0) Ap   (:55-70)
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_setup_resid(EllView A, PcgDev d, const double *__restrict__ x0, int n_all,
-                                                        double *__restrict__ p0, double *__restrict__ p1) {
+                                                        double *__restrict__ p) {
     const int stride = gridDim.x * kBlock;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n_all; i += stride) {
         d.x[i] = x0[i];
-        if (i >= d.n) {
-            p0[i] = x0[i];
-            p1[i] = x0[i];
-        }
+        if (i >= d.n) p[i] = x0[i];
     }
     Operand<double> xo{x0, nullptr, 0.0, d.n, false};
     FOR_ROWS(d.n, row) {
@@ -133,12 +127,14 @@ __global__ __launch_bounds__(kBlock) void k_setup_dir(PcgDev d, double *__restri
     if (threadIdx.x == 0) d.P(P_RR)[blockIdx.x] = rr;
 }
 
-// one workgroup: b.b, r.r, the tolerance, the state word and the iteration count
+// one workgroup: rho_0, b.b, r.r, the tolerance, the state word and the iteration count
 __global__ __launch_bounds__(kBlock) void k_setup_fin(PcgDev d, double tol) {
+    const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);
     const double bb = sum_partials<RED_SUM>(d.P(P_BB), d.G);
     const double rr = sum_partials<RED_SUM>(d.P(P_RR), d.G);
     if (threadIdx.x == 0) {
         const double tol2 = tol > 0.0 ? tol * tol * bb : -1.0;
+        d.sc[S_RHO0] = rho;
         d.sc[S_BB] = bb;
         d.sc[S_RR] = rr;
         d.sc[S_TOL2] = tol2;
@@ -149,9 +145,8 @@ __global__ __launch_bounds__(kBlock) void k_setup_fin(PcgDev d, double tol) {
 
 // ---- FP64 iteration --------------------------------------------------------
 
-template <int W, bool FUSE>
-__global__ __launch_bounds__(kBlock) void k_iter_spmv(EllView A, PcgDev d, const double *__restrict__ p_old,
-                                                      double *__restrict__ p_new, int k) {
+// K0 (k >= 1): rho_k, the convergence test, beta, p = 1.0 s + beta p
+__global__ __launch_bounds__(kBlock) void k_iter_dir(PcgDev d, double *__restrict__ p, int k) {
     if (!running(d)) return;
     const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);  // :107
     const double tol2 = d.sc[S_TOL2];
@@ -162,16 +157,22 @@ __global__ __launch_bounds__(kBlock) void k_iter_spmv(EllView A, PcgDev d, const
             return;
         }
     }
-    const double beta = FUSE ? rho / rho_slot(d, k - 1) : 0.0;  // :109
+    const double beta = rho / rho_slot(d, k - 1);  // :109
     if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
-    Operand<double> po{FUSE ? p_old : p_new, d.s, beta, d.n, FUSE};
+    FOR_ROWS(d.n, row) p[row] = 1.0 * d.s[row] + beta * p[row];  // :113
+}
+
+// K1: Ap = A p (empty rows keep Ap), partials of p.Ap
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_iter_spmv(EllView A, PcgDev d, const double *__restrict__ p) {
+    if (!running(d)) return;
+    Operand<double> po{p, nullptr, 0.0, d.n, false};
     double acc = 0.0;
     FOR_ROWS(d.n, row) {
-        double pi = po.at(row);
-        if (FUSE) p_new[row] = pi;
         double sum;
-        if (row_dot<W, double, double>(A, A.val, po, row, sum)) d.Ap[row] = sum;
-        acc = acc + pi * d.Ap[row];
+        double t = row_dot<W, double, double>(A, A.val, po, row, sum) ? sum : d.Ap[row];
+        d.Ap[row] = t;
+        acc = acc + p[row] * t;
     }
     acc = block_reduce<RED_SUM>(acc);
     if (threadIdx.x == 0) d.P(P_DN)[blockIdx.x] = acc;
@@ -221,23 +222,27 @@ __global__ __launch_bounds__(kBlock) void k_fin(PcgDev d) {
 
 // ---- MIXED iteration (FP32 vectors and operator, FP64 dot accumulation) --------
 
-template <int W, bool FUSE>
-__global__ __launch_bounds__(kBlock) void k_m_spmv(EllView A, PcgDev d, const float *__restrict__ p_old,
-                                                   float *__restrict__ p_new, int k) {
+// K0 (k >= 1): rho_k, beta (0 after a restart: the previous rho slot is
+// +inf), p32 = s32 + beta p32
+__global__ __launch_bounds__(kBlock) void k_m_dir(PcgDev d, float *__restrict__ p, int k) {
     if (!running(d)) return;
     const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);
-    // after a restart the previous rho slot holds +inf: beta = 0
-    const float beta = FUSE ? (float)(rho / rho_slot(d, k - 1)) : 0.0f;
+    const float beta = (float)(rho / rho_slot(d, k - 1));
     if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
-    Operand<float> po{FUSE ? p_old : p_new, d.s32, beta, d.n, FUSE};
+    FOR_ROWS(d.n, row) p[row] = d.s32[row] + beta * p[row];
+}
+
+// K1: Ap32 = A32 p32, partials of p.Ap (FP64 accumulation)
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_m_spmv(EllView A, PcgDev d, const float *__restrict__ p) {
+    if (!running(d)) return;
+    Operand<float> po{p, nullptr, 0.0f, d.n, false};
     double acc = 0.0;
     FOR_ROWS(d.n, row) {
-        float pi = po.at(row);
-        if (FUSE) p_new[row] = pi;
         float sum;
         row_dot<W, float, float>(A, A.val32, po, row, sum);
         d.Ap32[row] = sum;
-        acc = acc + (double)pi * (double)sum;
+        acc = acc + (double)p[row] * (double)sum;
     }
     acc = block_reduce<RED_SUM>(acc);
     if (threadIdx.x == 0) d.P(P_DN)[blockIdx.x] = acc;
@@ -342,7 +347,9 @@ __global__ __launch_bounds__(kBlock) void k_m_corr_fin(PcgDev d, int k, int firs
         d.P(P_RS)[i] = d.P(P_RS2)[i];
         d.P(P_RR)[i] = d.P(P_RR2)[i];
     }
+    const double rs = first ? sum_partials<RED_SUM>(d.P(P_RS2), d.G) : 0.0;
     if (threadIdx.x == 0) {
+        if (first) d.sc[S_RHO0] = rs;  // rho of iteration 0
         if (!first && rrt > 100.0 * rrec) d.sc[S_RHO0 + ((k - 1) & 1)] = INFINITY;
         d.sc[S_TRUE_RR] = rrt;
         d.sc[S_RR] = rrt;
@@ -379,8 +386,8 @@ struct gip_pcg {
     gip_ctx *ctx;
     const gip_mat *A;
     int n, n_all, grid;
-    double *r, *p[2], *s, *Ap, *dinv;
-    float *r32, *p32[2], *s32, *Ap32, *xc32, *dinv32;
+    double *r, *p, *s, *Ap, *dinv;
+    float *r32, *p32, *s32, *Ap32, *xc32, *dinv32;
     double *az;  // A 1 (MIXED coarse step)
     double *sc, *part;
     int *state;    // state, iters
@@ -397,22 +404,19 @@ static int pcg_grid(int n) {
     return g;
 }
 
-// Launch KERNEL<W, ...> with W the operator's kernel width: a uniform ELL
-// width (5..7), a stencil width (-7, -8), or 0 for the generic loops.
-#define DISPATCH_W(A, KERNEL, TAIL, ...)                                              \
-    do {                                                                              \
-        switch (kernel_width(A)) {                                                    \
-            case 5: KERNEL<5 TAIL><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;   \
-            case 6: KERNEL<6 TAIL><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;   \
-            case 7: KERNEL<7 TAIL><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;   \
-            case -7: KERNEL<-7 TAIL><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break; \
-            case -8: KERNEL<-8 TAIL><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break; \
-            default: KERNEL<0 TAIL><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;  \
-        }                                                                             \
+// Launch KERNEL<W> with W the operator's kernel width: a uniform ELL width
+// (5..7), a stencil width (-7, -8), or 0 for the generic loops.
+#define DISPATCH_W(A, KERNEL, ...)                                              \
+    do {                                                                        \
+        switch (kernel_width(A)) {                                              \
+            case 5: KERNEL<5><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;   \
+            case 6: KERNEL<6><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;   \
+            case 7: KERNEL<7><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;   \
+            case -7: KERNEL<-7><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break; \
+            case -8: KERNEL<-8><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break; \
+            default: KERNEL<0><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;  \
+        }                                                                       \
     } while (0)
-#define NOTAIL
-#define FUSED_TAIL , true
-#define FIRST_TAIL , false
 
 static PcgDev make_dev(gip_pcg *S, const double *b, double *x, int precond) {
     PcgDev d;
@@ -459,8 +463,7 @@ int gip_pcg_create(gip_ctx *c, const gip_mat *A, int32_t n_all, gip_pcg **out) {
         if (e == hipSuccess) e = hipMalloc(p, bytes);
     };
     dalloc((void **)&S->r, n * 8);
-    dalloc((void **)&S->p[0], na * 8);
-    dalloc((void **)&S->p[1], na * 8);
+    dalloc((void **)&S->p, na * 8);
     dalloc((void **)&S->s, n * 8);
     dalloc((void **)&S->Ap, n * 8);
     dalloc((void **)&S->dinv, n * 8);
@@ -493,8 +496,8 @@ int gip_pcg_destroy(gip_pcg *S) {
     if (!S) return GIP_OK;
     hipSetDevice(S->ctx->device);
     hipStreamSynchronize(S->ctx->stream);
-    void *ptrs[] = {S->r,      S->p[0], S->p[1], S->s,      S->Ap, S->dinv, S->r32,  S->p32[0],
-                    S->p32[1], S->s32,  S->Ap32, S->xc32, S->dinv32, S->az, S->sc, S->part, S->state};
+    void *ptrs[] = {S->r,   S->p,    S->s,    S->Ap,     S->dinv, S->r32, S->p32,  S->s32,
+                    S->Ap32, S->xc32, S->dinv32, S->az, S->sc,   S->part, S->state};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (S->h_state) hipHostFree(S->h_state);
@@ -513,12 +516,10 @@ static int ensure_mixed(gip_pcg *S) {
     float **bufs[] = {&S->r32, &S->s32, &S->Ap32, &S->xc32, &S->dinv32};
     for (float **b : bufs) GIP_HIP_TRY(hipMalloc(b, n * 4));
     GIP_HIP_TRY(hipMalloc(&S->az, n * 8));
-    // the FP32 directions are SpMV inputs: ghost columns included, and zero
+    // the FP32 direction is an SpMV input: ghost columns included, and zero
     // there (the correction leaves the ghost values of x unchanged)
-    for (int k = 0; k < 2; ++k) {
-        GIP_HIP_TRY(hipMalloc(&S->p32[k], na * 4));
-        GIP_HIP_TRY(hipMemsetAsync(S->p32[k], 0, na * 4, S->ctx->stream));
-    }
+    GIP_HIP_TRY(hipMalloc(&S->p32, na * 4));
+    GIP_HIP_TRY(hipMemsetAsync(S->p32, 0, na * 4, S->ctx->stream));
     if (S->n) {
         k_to32<<<(S->n + kBlock - 1) / kBlock, kBlock, 0, S->ctx->stream>>>(S->n, S->dinv, S->dinv32);
         GIP_LAUNCH_CHECK("dinv32");
@@ -527,12 +528,12 @@ static int ensure_mixed(gip_pcg *S) {
 }
 
 // az = A 1 over the solved rows and 1.A1 into S_ZAZ, from A's current
-// values (one product and one reduction per mixed solve); the first
+// values (one product and one reduction per mixed solve); the FP64
 // direction buffer serves as the vector 1 before the setup overwrites it.
 static int coarse_setup(gip_pcg *S) {
     gip_ctx *c = S->ctx;
     if (S->n_all == 0) return GIP_OK;
-    double *z = S->p[1];
+    double *z = S->p;
     k_ones<<<(S->n_all + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(S->n, S->n_all, z, S->az);
     GIP_LAUNCH_CHECK("coarse vector");
     int rc = spmv_launch(c, S->A, z, S->az, 1.0, 0.0, 0, nullptr);
@@ -568,8 +569,8 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
     const int pollk = o->poll > 0 ? o->poll : 16;
     const int use_coarse = o->coarse >= 0 ? 1 : 0;
     // setup, GIP_LinearSolver.h:55-80
-    DISPATCH_W(A, k_setup_resid, NOTAIL, v, d, x0, S->n_all, S->p[0], S->p[1]);
-    k_setup_dir<<<S->grid, kBlock, 0, st>>>(d, S->p[0]);
+    DISPATCH_W(A, k_setup_resid, v, d, x0, S->n_all, S->p);
+    k_setup_dir<<<S->grid, kBlock, 0, st>>>(d, S->p);
     k_setup_fin<<<1, kBlock, 0, st>>>(d, o->tol);
     GIP_LAUNCH_CHECK("pcg setup");
     int corrections = 0, rc;
@@ -579,12 +580,9 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
         while (k < o->maxit && S->h_state[0] == ST_RUN) {
             int chunk = std::min(pollk, o->maxit - k);
             for (int j = 0; j < chunk; ++j, ++k) {
-                double *pn = S->p[k & 1];
-                if (k == 0)
-                    DISPATCH_W(A, k_iter_spmv, FIRST_TAIL, v, d, pn, pn, k);
-                else
-                    DISPATCH_W(A, k_iter_spmv, FUSED_TAIL, v, d, S->p[(k - 1) & 1], pn, k);
-                k_iter_update<<<S->grid, kBlock, 0, st>>>(d, pn, k);
+                if (k > 0) k_iter_dir<<<S->grid, kBlock, 0, st>>>(d, S->p, k);
+                DISPATCH_W(A, k_iter_spmv, v, d, S->p);
+                k_iter_update<<<S->grid, kBlock, 0, st>>>(d, S->p, k);
             }
             GIP_LAUNCH_CHECK("pcg iteration");
             if ((rc = poll(S, false))) return rc;
@@ -593,10 +591,10 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
         GIP_LAUNCH_CHECK("pcg final test");
     } else {
         // FP64 residual of x0 (with the coarse step), FP32 copies, first direction
-        DISPATCH_W(A, k_m_resid, NOTAIL, v, d);
+        DISPATCH_W(A, k_m_resid, v, d);
         k_m_coarse<<<S->grid, kBlock, 0, st>>>(d, use_coarse);
         k_m_corr_fin<<<1, kBlock, 0, st>>>(d, 0, 1);
-        k_m_start<<<S->grid, kBlock, 0, st>>>(d, S->p32[0]);
+        k_m_start<<<S->grid, kBlock, 0, st>>>(d, S->p32);
         GIP_LAUNCH_CHECK("mixed setup");
         if ((rc = poll(S, true))) return rc;
         double last_rr = S->h_sc[S_TRUE_RR];
@@ -606,12 +604,9 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
         while (k < o->maxit && S->h_state[0] == ST_RUN) {
             int chunk = std::min(pollk, o->maxit - k);
             for (int j = 0; j < chunk; ++j, ++k) {
-                float *pn = S->p32[k & 1];
-                if (k == 0)
-                    DISPATCH_W(A, k_m_spmv, FIRST_TAIL, v, d, pn, pn, k);
-                else
-                    DISPATCH_W(A, k_m_spmv, FUSED_TAIL, v, d, S->p32[(k - 1) & 1], pn, k);
-                k_m_update<<<S->grid, kBlock, 0, st>>>(d, pn, k);
+                if (k > 0) k_m_dir<<<S->grid, kBlock, 0, st>>>(d, S->p32, k);
+                DISPATCH_W(A, k_m_spmv, v, d, S->p32);
+                k_m_update<<<S->grid, kBlock, 0, st>>>(d, S->p32, k);
             }
             k_m_status<<<1, kBlock, 0, st>>>(d);
             GIP_LAUNCH_CHECK("mixed iteration");
@@ -621,7 +616,7 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
             const bool near_target = tol2 >= 0.0 && rr32 <= tol2;
             if (rr32 <= inner2 * last_rr || near_target || k >= o->maxit) {
                 k_m_fold<<<S->grid, kBlock, 0, st>>>(d);
-                DISPATCH_W(A, k_m_resid, NOTAIL, v, d);
+                DISPATCH_W(A, k_m_resid, v, d);
                 k_m_coarse<<<S->grid, kBlock, 0, st>>>(d, use_coarse);
                 k_m_corr_fin<<<1, kBlock, 0, st>>>(d, k, 0);
                 GIP_LAUNCH_CHECK("mixed correction");
