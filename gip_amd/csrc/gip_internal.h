@@ -7,6 +7,7 @@
 #include <string>
 
 #include "gip_hip.h"
+#include "gip_layout.h"
 
 namespace gip {
 
@@ -64,19 +65,17 @@ struct gip_mat {
     double *d_ell_val;
     float *d_ell_val32;
     int32_t *d_csr2ell;   // CSR nnz index -> position in d_ell_val (ELL and stencil)
-    // stencil layout: slice s, slot k holds the entry of column row + off[s W + k]
-    // of each of its 64 rows, value at d_ell_val[(s W + k) 64 + lane] (0 where
-    // a row lacks that column); a row's remaining entries, in stored order,
-    // form its tail: tail-local index t = tsoff[s] + (tinfo[row] & 255), count
-    // (tinfo[row] >> 8) & 127, columns tcol[t..], values d_ell_val[tail_base + t..];
-    // bit 15 of tinfo marks an empty row.  tinfo is read only in flagged slices.
+    // stencil layout (gip_layout.h): slice s, slot k holds the entry of
+    // column row + off[s W + k] of each of its 64 rows, value at
+    // d_ell_val[(s W + k) 64 + lane] (0 where a row lacks that column); a
+    // row's remaining entries, in stored order, form its tail, located
+    // through the slice header (lane masks per tail rank); tail values at
+    // d_ell_val[tail_base + t], columns d_st_tcol[t].
     int32_t st_w;
     int64_t st_tail_base;
     int32_t st_tail_nnz;
-    int32_t *d_st_off;     // nslices * st_w
-    int32_t *d_st_tsoff;   // nslices + 1
-    int32_t *d_st_flag;    // nslices: 1 when rows of the slice have a tail or are empty
-    uint16_t *d_st_tinfo;  // nrows
+    int32_t *d_st_off;        // nslices * st_w
+    gip::StencilHdr *d_st_hdr;  // nslices
     int32_t *d_st_tcol;    // st_tail_nnz
     // a_ii per row (the stored values of a DIAG operator), for Jacobi
     double *d_diag;

@@ -12,6 +12,14 @@
 // that adds the slots in order and then the tail therefore adds the row's
 // entries in stored order, with +-0 terms for the slots the row lacks: the
 // CSR result, bitwise, whenever the input vector is finite.
+//
+// Tails are located without per-row words: each slice has a header with the
+// start of its tail and one 64-bit lane mask per tail rank j (rows with more
+// than j tail entries) plus a mask of its empty rows.  The slice's tail lists
+// every row's first tail entry in lane order, then every row's second, and so
+// on; lane l's entry j sits at tsoff + (entries of ranks < j) +
+// popcount(m[j] & lanes below l).  A wavefront reads the header with scalar
+// loads, so a tail entry costs one column load and one gather, as in ELL.
 #pragma once
 
 #include <algorithm>
@@ -23,22 +31,30 @@ namespace gip {
 
 constexpr int kStencilMaxW = 8;
 constexpr int kStencilSlice = 64;
+constexpr int kStencilMaxTail = 8;  // tail entries per row
+
+// Per-slice header (80 bytes, read with scalar loads).
+struct StencilHdr {
+    int32_t tsoff;   // first tail entry of the slice (tail-local index)
+    int32_t any;     // 1 when the slice has tail entries or empty rows
+    uint64_t m[kStencilMaxTail];  // m[j]: lanes whose row has more than j tail entries
+    uint64_t empty;  // lanes whose row has no entries at all
+};
+static_assert(sizeof(StencilHdr) == 80, "slice header layout");
 
 struct StencilPlan {
     int32_t ns = 0, w = 0;
     std::vector<int32_t> off;     // ns * w offsets
-    std::vector<int32_t> tsoff;   // ns + 1: tail start of each slice
-    std::vector<int32_t> flag;    // ns: 1 when rows of the slice have a tail or are empty
-    std::vector<uint16_t> tinfo;  // nrows: bits 0-7 start in the slice's tail, 8-14 count, 15 empty row
+    std::vector<StencilHdr> hdr;  // ns slice headers
     std::vector<int32_t> tcol;    // tail columns
     std::vector<int32_t> pos;     // per stored entry: position in the value array
     int64_t tail_base = 0;        // ns * w * 64: first tail value
     int64_t stored = 0;           // tail_base + tail entries
 };
 
-// Returns false when the operator does not fit the layout: a slice or row
-// tail beyond the 8/7-bit fields, more than max_tail_frac of the entries in
-// tails, or positions beyond int32.
+// Returns false when the operator does not fit the layout: a row with more
+// than kStencilMaxTail tail entries, more than max_tail_frac of the entries
+// in tails, or positions beyond int32.
 inline bool plan_stencil(int32_t nrows, int32_t nnz, const int32_t *rowptr, const int32_t *col, StencilPlan &P,
                          double max_tail_frac = 0.05) {
     if (nrows <= 0 || nnz <= 0) return false;
@@ -81,21 +97,21 @@ inline bool plan_stencil(int32_t nrows, int32_t nnz, const int32_t *rowptr, cons
         for (int32_t k = 0; k < nsel[(size_t)s]; ++k)
             P.off[(size_t)s * w + k] = sel[(size_t)s * kStencilMaxW + k];  // unused slots: offset 0, value 0
     P.tail_base = (int64_t)ns * w * kStencilSlice;
-    P.tsoff.assign((size_t)ns + 1, 0);
-    P.flag.assign((size_t)ns, 0);
-    P.tinfo.assign((size_t)nrows, 0);
+    P.hdr.assign((size_t)ns, StencilHdr{});
     P.tcol.clear();
     P.pos.assign((size_t)nnz, -1);
     const int64_t max_tail = (int64_t)(max_tail_frac * nnz) + kStencilSlice;
     std::vector<int64_t> tail_entry;  // stored-entry index of each tail entry
+    std::vector<int32_t> rt[kStencilSlice];  // this slice's rows: stored-entry indices of their tails
     for (int32_t s = 0; s < ns; ++s) {
         const int32_t r0 = s * kStencilSlice, r1 = std::min(nrows, r0 + kStencilSlice);
-        const int64_t t_s = (int64_t)P.tcol.size();
         const int32_t *off = &P.off[(size_t)s * w];
+        StencilHdr &h = P.hdr[(size_t)s];
+        h.tsoff = (int32_t)P.tcol.size();
         for (int32_t r = r0; r < r1; ++r) {
             const int32_t lane = r - r0;
-            const int64_t pre = (int64_t)P.tcol.size() - t_s;
-            int32_t k = 0, cnt = 0;
+            rt[lane].clear();
+            int32_t k = 0;
             bool tail = false;
             for (int32_t j = rowptr[r]; j < rowptr[r + 1]; ++j) {
                 const int32_t d = col[j] - r;
@@ -109,17 +125,22 @@ inline bool plan_stencil(int32_t nrows, int32_t nnz, const int32_t *rowptr, cons
                     }
                     tail = true;
                 }
-                P.tcol.push_back(col[j]);
-                tail_entry.push_back(j);
-                ++cnt;
+                rt[lane].push_back(j);
             }
-            const bool empty = rowptr[r] == rowptr[r + 1];
-            if (cnt || empty) P.flag[(size_t)s] = 1;
-            if ((cnt && pre > 255) || cnt > 127) return false;
-            P.tinfo[(size_t)r] = (uint16_t)((cnt ? pre : 0) | (cnt << 8) | (empty ? 0x8000 : 0));
-            if ((int64_t)P.tcol.size() > max_tail) return false;
+            if ((int)rt[lane].size() > kStencilMaxTail) return false;
+            for (size_t t = 0; t < rt[lane].size(); ++t) h.m[t] |= 1ull << lane;
+            if (rowptr[r] == rowptr[r + 1]) h.empty |= 1ull << lane;
         }
-        P.tsoff[(size_t)s + 1] = (int32_t)P.tcol.size();
+        // the slice's tail: rank 0 entries in lane order, then rank 1, ...
+        for (int t = 0; t < kStencilMaxTail; ++t)
+            for (int32_t r = r0; r < r1; ++r)
+                if ((int)rt[r - r0].size() > t) {
+                    const int32_t j = rt[r - r0][(size_t)t];
+                    P.tcol.push_back(col[j]);
+                    tail_entry.push_back(j);
+                }
+        h.any = (h.m[0] | h.empty) ? 1 : 0;
+        if ((int64_t)P.tcol.size() > max_tail) return false;
     }
     P.stored = P.tail_base + (int64_t)P.tcol.size();
     if (P.stored > INT32_MAX) return false;
@@ -127,13 +148,19 @@ inline bool plan_stencil(int32_t nrows, int32_t nnz, const int32_t *rowptr, cons
     return true;
 }
 
+// Position of tail entry `rank` of lane `lane` in the slice's tail
+// (tail-local index), from the slice header.
+inline int32_t stencil_tail_index(const StencilHdr &h, int lane, int rank) {
+    int32_t t = h.tsoff;
+    for (int j = 0; j < rank; ++j) t += __builtin_popcountll(h.m[j]);
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    return t + __builtin_popcountll(h.m[rank] & below);
+}
+
 // Bytes one FP64 product streams for the operator in this layout (values,
-// tail columns, the per-row tail words of flagged slices, slice headers).
-inline int64_t stencil_bytes(const StencilPlan &P, int32_t nrows) {
-    int64_t flagged_rows = 0;
-    for (int32_t s = 0; s < P.ns; ++s)
-        if (P.flag[(size_t)s]) flagged_rows += std::min<int64_t>(kStencilSlice, nrows - (int64_t)s * kStencilSlice);
-    return 8 * P.stored + 4 * (int64_t)P.tcol.size() + 2 * flagged_rows + (int64_t)P.ns * (4 * P.w + 12);
+// tail columns, slice offsets and headers).
+inline int64_t stencil_bytes(const StencilPlan &P, int32_t) {
+    return 8 * P.stored + 4 * (int64_t)P.tcol.size() + (int64_t)P.ns * (4 * P.w + (int64_t)sizeof(StencilHdr));
 }
 
 }  // namespace gip

@@ -15,8 +15,8 @@ struct EllView {
     const double *val;
     const float *val32;
     // stencil layout
-    const int32_t *st_off, *st_tsoff, *st_flag, *st_tcol;
-    const uint16_t *st_tinfo;
+    const int32_t *st_off, *st_tcol;
+    const StencilHdr *st_hdr;
     int st_w, tail_base, cmax;
 };
 
@@ -30,10 +30,8 @@ static EllView view(const gip_mat *A) {
     v.val = packed ? A->d_ell_val : A->d_val;
     v.val32 = packed ? A->d_ell_val32 : A->d_val32;
     v.st_off = A->d_st_off;
-    v.st_tsoff = A->d_st_tsoff;
-    v.st_flag = A->d_st_flag;
     v.st_tcol = A->d_st_tcol;
-    v.st_tinfo = A->d_st_tinfo;
+    v.st_hdr = A->d_st_hdr;
     v.st_w = A->st_w;
     v.tail_base = (int)A->st_tail_base;
     v.cmax = A->cols_needed > 0 ? A->cols_needed - 1 : 0;
@@ -66,7 +64,8 @@ struct Operand {
 // Stencil-layout row sum: the slots in order (column row + offset, clamped
 // into the vector for slots the row lacks, whose value is 0), then the
 // row's tail in stored order.  The wavefront's 64 rows are one slice, so
-// the slice's offsets and header are wavefront-uniform (scalar loads).
+// the slice's offsets and header are wavefront-uniform (scalar loads); a
+// tail entry's position follows from the header's lane masks.
 template <int SW, class T, class V>
 __device__ inline bool stencil_row(const EllView &A, const V *__restrict__ val, const Operand<T> &x, int row, T &sum) {
     const int w = SW > 0 ? SW : A.st_w;
@@ -90,11 +89,18 @@ __device__ inline bool stencil_row(const EllView &A, const V *__restrict__ val, 
             sum = sum + x.at(cc < 0 ? 0 : (cc > A.cmax ? A.cmax : cc)) * vs[64 * k];
         }
     }
-    if (A.st_flag[s]) {
-        const unsigned info = A.st_tinfo[row];
-        if (info & 0x8000u) return false;  // empty row: left untouched
-        const int t = A.st_tsoff[s] + (int)(info & 255u), nt = (int)((info >> 8) & 127u);
-        for (int e = 0; e < nt; ++e) sum = sum + x.at(A.st_tcol[t + e]) * val[A.tail_base + t + e];
+    const StencilHdr *h = A.st_hdr + s;
+    if (h->any) {
+        const uint64_t bit = 1ull << (row & 63), below = bit - 1ull;
+        if (h->empty & bit) return false;  // empty row: left untouched
+        int t = h->tsoff;
+        for (int j = 0; j < kStencilMaxTail; ++j) {
+            const uint64_t mj = h->m[j];
+            if (!(mj & bit)) break;
+            const int e = t + __popcll(mj & below);
+            sum = sum + x.at(A.st_tcol[e]) * val[A.tail_base + e];
+            t += __popcll(mj);
+        }
     }
     return true;
 }

@@ -231,7 +231,7 @@ static int upload(gip_ctx *c, T **dptr, const T *host, size_t count) {
 static void free_mat_arrays(gip_mat *A) {
     void *ptrs[] = {A->d_rowptr,   A->d_col,     A->d_val,      A->d_val32,    A->d_slice_off, A->d_ell_col,
                     A->d_ell_val,  A->d_ell_val32, A->d_csr2ell, A->d_diag,     A->d_tsrc,      A->d_tdst,
-                    A->d_st_off,   A->d_st_tsoff, A->d_st_flag,  A->d_st_tinfo, A->d_st_tcol};
+                    A->d_st_off,   A->d_st_hdr,   A->d_st_tcol};
     for (void *p : ptrs)
         if (p) hipFree(p);
 }
@@ -479,9 +479,7 @@ int gip_mat_create(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, const 
         rc = upload(c, &A->d_ell_val, sval.data(), sval.size());
         if (!rc) rc = upload(c, &A->d_csr2ell, plan.pos.data(), plan.pos.size());
         if (!rc) rc = upload(c, &A->d_st_off, plan.off.data(), plan.off.size());
-        if (!rc) rc = upload(c, &A->d_st_tsoff, plan.tsoff.data(), plan.tsoff.size());
-        if (!rc) rc = upload(c, &A->d_st_flag, plan.flag.data(), plan.flag.size());
-        if (!rc) rc = upload(c, &A->d_st_tinfo, plan.tinfo.data(), plan.tinfo.size());
+        if (!rc) rc = upload(c, &A->d_st_hdr, plan.hdr.data(), plan.hdr.size());
         if (!rc) rc = upload(c, &A->d_st_tcol, plan.tcol.data(), plan.tcol.size());
         if (!rc) rc = upload(c, &A->d_col, col, (size_t)nnz);
         if (!rc) {

@@ -62,7 +62,7 @@ int main(int argc, char **argv) {
         for (int32_t j = rp[(size_t)r]; j < rp[(size_t)r + 1]; ++j) want = want + x[(size_t)col[(size_t)j]] * val[(size_t)j];
         const bool empty_csr = rp[(size_t)r] == rp[(size_t)r + 1];
         const int32_t s = r / 64, lane = r % 64;
-        const uint16_t info = P.tinfo[(size_t)r];
+        const gip::StencilHdr &h = P.hdr[(size_t)s];
         double got = 0.0;
         bool empty = false;
         for (int32_t k = 0; k < P.w; ++k) {
@@ -70,15 +70,16 @@ int main(int argc, char **argv) {
             c = c < 0 ? 0 : (c > cmax ? cmax : c);
             got = got + x[(size_t)c] * st[(size_t)(((int64_t)s * P.w + k) * 64 + lane)];
         }
-        if (P.flag[(size_t)s]) {
-            empty = (info & 0x8000) != 0;
-            const int32_t t = P.tsoff[(size_t)s] + (info & 255), nt = (info >> 8) & 127;
-            for (int32_t e = 0; e < nt; ++e) {
-                if (t + e >= (int32_t)P.tcol.size() || P.tcol[(size_t)(t + e)] > cmax) {
+        if (h.any) {
+            const uint64_t bit = 1ull << lane;
+            empty = (h.empty & bit) != 0;
+            for (int j = 0; j < gip::kStencilMaxTail && (h.m[j] & bit); ++j) {
+                const int32_t t = gip::stencil_tail_index(h, lane, j);
+                if (t < 0 || t >= (int32_t)P.tcol.size() || P.tcol[(size_t)t] > cmax) {
                     ++bad_cols;
                     continue;
                 }
-                got = got + x[(size_t)P.tcol[(size_t)(t + e)]] * st[(size_t)(P.tail_base + t + e)];
+                got = got + x[(size_t)P.tcol[(size_t)t]] * st[(size_t)(P.tail_base + t)];
             }
         }
         if (empty != empty_csr || (!empty && std::memcmp(&got, &want, sizeof(double)) != 0)) ++bad_rows;
