@@ -116,8 +116,18 @@ __device__ inline bool last_block_reduce2(double v0, double v1, double *partials
 template <int OP>
 __device__ inline double sum_partials(const double *__restrict__ part, int n) {
     __shared__ double s_all;
+    // n <= kMaxRedBlocks: all of a thread's loads are issued before the first
+    // is used (one memory round trip), then combined in index order
+    constexpr int K = kMaxRedBlocks / kBlock;
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = threadIdx.x + k * kBlock;
+        v[k] = i < n ? part[i] : RedOp<OP>::identity();
+    }
     double acc = RedOp<OP>::identity();
-    for (int i = threadIdx.x; i < n; i += kBlock) acc = RedOp<OP>::combine(acc, part[i]);
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc = RedOp<OP>::combine(acc, v[k]);
     acc = block_reduce<OP>(acc);
     if (threadIdx.x == 0) s_all = acc;
     __syncthreads();

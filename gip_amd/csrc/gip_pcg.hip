@@ -76,7 +76,10 @@ struct PcgDev {
     __device__ double *P(int k) const { return part + k * kMaxRedBlocks; }
 };
 
-__device__ inline bool running(const PcgDev &d) { return *(volatile int *)d.state == ST_RUN; }
+// The state word is written by earlier kernels only (a workgroup that sets
+// it in the current kernel returns with every other workgroup reaching the
+// same decision), so a plain load, ordered by the kernel boundary, suffices.
+__device__ inline bool running(const PcgDev &d) { return *d.state == ST_RUN; }
 __device__ inline bool leader() { return blockIdx.x == 0 && threadIdx.x == 0; }
 __device__ inline double rho_slot(const PcgDev &d, int k) { return d.sc[S_RHO0 + (k & 1)]; }
 
@@ -159,7 +162,11 @@ __global__ __launch_bounds__(kBlock) void k_iter_dir(PcgDev d, double *__restric
     }
     const double beta = rho / rho_slot(d, k - 1);  // :109
     if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
-    FOR_ROWS(d.n, row) p[row] = 1.0 * d.s[row] + beta * p[row];  // :113
+    const double *__restrict__ s = d.s;
+    FOR_ROWS(d.n, row) {
+        const double si = s[row], po = p[row];
+        p[row] = 1.0 * si + beta * po;  // :113
+    }
 }
 
 // K1: Ap = A p (empty rows keep Ap), partials of p.Ap
@@ -167,12 +174,17 @@ template <int W>
 __global__ __launch_bounds__(kBlock) void k_iter_spmv(EllView A, PcgDev d, const double *__restrict__ p) {
     if (!running(d)) return;
     Operand<double> po{p, nullptr, 0.0, d.n, false};
+    double *__restrict__ Ap = d.Ap;
     double acc = 0.0;
     FOR_ROWS(d.n, row) {
+        const double pr = p[row];  // issued with the row's gathers
         double sum;
-        double t = row_dot<W, double, double>(A, A.val, po, row, sum) ? sum : d.Ap[row];
-        d.Ap[row] = t;
-        acc = acc + p[row] * t;
+        if (row_dot<W, double, double>(A, A.val, po, row, sum)) {
+            Ap[row] = sum;
+        } else {
+            sum = Ap[row];  // empty row: Ap keeps its value
+        }
+        acc = acc + pr * sum;
     }
     acc = block_reduce<RED_SUM>(acc);
     if (threadIdx.x == 0) d.P(P_DN)[blockIdx.x] = acc;
@@ -187,19 +199,25 @@ __global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, const double *
         return;
     }
     const double malpha = -1.0 * alpha;
+    double *__restrict__ x = d.x, *__restrict__ r = d.r, *__restrict__ s = d.s;
+    const double *__restrict__ Ap = d.Ap, *__restrict__ dinv = d.dinv;
+    const bool precond = d.precond != 0;
     double rs = 0.0, rr = 0.0;
     FOR_ROWS(d.n, row) {
-        d.x[row] = d.x[row] + alpha * p[row];       // :95
-        double ri = d.r[row] + malpha * d.Ap[row];  // :97
-        d.r[row] = ri;
+        // every operand first, so the row costs one memory round trip
+        const double xo = x[row], pr = p[row], ro = r[row], apr = Ap[row];
+        const double di = precond ? dinv[row] : 0.0;
+        x[row] = xo + alpha * pr;        // :95
+        const double ri = ro + malpha * apr;  // :97
+        r[row] = ri;
         double si;
-        if (d.precond) {  // :99-105
+        if (precond) {  // :99-105
             si = 0.0;
-            si = si + ri * d.dinv[row];
+            si = si + ri * di;
         } else {
             si = ri;
         }
-        d.s[row] = si;
+        s[row] = si;
         rs = rs + ri * si;
         rr = rr + ri * ri;
     }
@@ -229,7 +247,11 @@ __global__ __launch_bounds__(kBlock) void k_m_dir(PcgDev d, float *__restrict__ 
     const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);
     const float beta = (float)(rho / rho_slot(d, k - 1));
     if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
-    FOR_ROWS(d.n, row) p[row] = d.s32[row] + beta * p[row];
+    const float *__restrict__ s = d.s32;
+    FOR_ROWS(d.n, row) {
+        const float si = s[row], po = p[row];
+        p[row] = si + beta * po;
+    }
 }
 
 // K1: Ap32 = A32 p32, partials of p.Ap (FP64 accumulation)
@@ -237,12 +259,14 @@ template <int W>
 __global__ __launch_bounds__(kBlock) void k_m_spmv(EllView A, PcgDev d, const float *__restrict__ p) {
     if (!running(d)) return;
     Operand<float> po{p, nullptr, 0.0f, d.n, false};
+    float *__restrict__ Ap = d.Ap32;
     double acc = 0.0;
     FOR_ROWS(d.n, row) {
+        const float pr = p[row];  // issued with the row's gathers
         float sum;
         row_dot<W, float, float>(A, A.val32, po, row, sum);
-        d.Ap32[row] = sum;
-        acc = acc + (double)p[row] * (double)sum;
+        Ap[row] = sum;
+        acc = acc + (double)pr * (double)sum;
     }
     acc = block_reduce<RED_SUM>(acc);
     if (threadIdx.x == 0) d.P(P_DN)[blockIdx.x] = acc;
@@ -257,13 +281,19 @@ __global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, const float *__re
         return;
     }
     const float a32 = (float)alpha;
+    float *__restrict__ xc = d.xc32, *__restrict__ r = d.r32, *__restrict__ s = d.s32;
+    const float *__restrict__ Ap = d.Ap32, *__restrict__ dinv = d.dinv32;
+    const bool precond = d.precond != 0;
     double rs = 0.0, rr = 0.0;
     FOR_ROWS(d.n, row) {
-        d.xc32[row] = d.xc32[row] + a32 * p[row];
-        float ri = d.r32[row] - a32 * d.Ap32[row];
-        d.r32[row] = ri;
-        float si = d.precond ? ri * d.dinv32[row] : ri;
-        d.s32[row] = si;
+        // every operand first, so the row costs one memory round trip
+        const float xo = xc[row], pr = p[row], ro = r[row], apr = Ap[row];
+        const float di = precond ? dinv[row] : 0.0f;
+        xc[row] = xo + a32 * pr;
+        const float ri = ro - a32 * apr;
+        r[row] = ri;
+        const float si = precond ? ri * di : ri;
+        s[row] = si;
         rs = rs + (double)ri * (double)si;
         rr = rr + (double)ri * (double)ri;
     }
