@@ -20,6 +20,9 @@ Other suites (one JSON line each, not the default):
                       [100, 3200]) advanced 50 steps as one batch vs one by one
     --suite adjoint   64^3, Re=400, 200 steps, checkpoints every 20: forward +
                       backward time and 16 central finite-difference checks
+    --suite clustered 128^3 tanh-clustered mesh (beta 1.5), Re=1000, 100 steps,
+                      PCG to 1e-10: steps/s FP64 vs mixed, the L SpMV rate and
+                      the FP64/mixed PCG on a zero-mean N(0,1) right-hand side
     --probe spmv      a few launches of the FP64 SpMV and of a plain copy of
                       known size, for rocprofv3 --pmc traffic runs
 """
@@ -161,7 +164,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=60)
     ap.add_argument("--skip-ns", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--suite", choices=("pcg", "batch", "adjoint"), default="pcg")
+    ap.add_argument("--suite", choices=("pcg", "batch", "adjoint", "clustered"), default="pcg")
     ap.add_argument("--probe", choices=("spmv",), default=None)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -174,6 +177,8 @@ def main():
         return batch_suite(args)
     if args.suite == "adjoint":
         return adjoint_suite(args)
+    if args.suite == "clustered":
+        return clustered_suite(args)
 
     from gip_amd import cavity, dns, hip
 
@@ -350,6 +355,51 @@ def batch_suite(args):
         "cells_total": cells, "sizes": sizes, "re": [round(float(v), 1) for v in re],
         "pcg_iters_last_step": [i["pcg_iters"] for i in bt_info],
         "data": "synthetic: generated uniform cavities, lid u=1, from rest"}), flush=True)
+
+
+def clustered_suite(args):
+    """128^3 cavity on the tanh-clustered mesh x_i = (1 + tanh(beta (2i/n - 1))
+    / tanh(beta)) / 2, beta = 1.5, Re = 1000: 100 Navier-Stokes steps from
+    rest with the pressure solved to 1e-10 (FP64 and mixed precision), plus
+    the L SpMV rate and both PCG solvers on a seeded zero-mean N(0,1)
+    right-hand side (the 128^3 microbenchmarks)."""
+    from gip_amd import cavity, dns, hip
+    n, steps = 128, 100
+    t0 = time.perf_counter()
+    cav = cavity.generate(n, beta=1.5, gamma=1e-3)
+    log(f"generated clustered {n}^3 cavity in {time.perf_counter() - t0:.1f} s")
+    L = cav.op("L")
+    N, NA = cav.n_inner, cav.n_all
+    dev = hip.Device(0)
+    A = hip.Matrix.from_csr(dev, L, fp32=True)
+    info = A.info()
+    x = dev.to_device(np.random.default_rng(1).standard_normal(NA))
+    y = dev.array(NA)
+    spmv_ms = time_spmv(hip, dev, A, x, y, 50)
+    alg = spmv_bytes(info["nnz"], N, NA)
+    b = np.zeros(NA)
+    b[:N] = cavity.rhs_normal(N, args.seed)
+    bnorm = float(np.linalg.norm(b[:N]))
+    b_dev = dev.to_device(b)
+    fp64 = pcg_leg(hip, dev, A, b_dev, N, NA, hip.PCG_FP64, 2, 1, 20000, bnorm)
+    mixed = pcg_leg(hip, dev, A, b_dev, N, NA, hip.PCG_MIXED, 2, 1, 20000, bnorm)
+    A.destroy()
+    del x, y, b_dev
+    r64, f64 = ns_leg(dns, cav, "fused", steps, 1)
+    r32, f32 = ns_leg(dns, cav, "mixed", steps, 1)
+    err = {k: float(np.abs(f32[k][:N] - f64[k][:N]).max() / max(1e-300, np.abs(f64[k][:N]).max())) for k in f64}
+    log(f"clustered 128^3: spmv {alg / (spmv_ms * 1e-3) / 1e9:.0f} GB/s; pcg fp64 {fp64['ms']:.1f} ms "
+        f"({fp64['iters']} it), mixed {mixed['ms']:.1f} ms ({mixed['iters']} it); ns steps/s fp64 "
+        f"{r64['steps_per_s']:.2f}, mixed {r32['steps_per_s']:.2f}")
+    print(json.dumps({
+        "metric": "clustered 128^3 cavity (beta 1.5), Re=1000, PCG to 1e-10: Navier-Stokes steps/s, mixed precision",
+        "value": r32["steps_per_s"], "unit": "steps/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "higher_is_better": True, "ns_fp64": r64, "ns_mixed": r32, "mixed_vs_fp64_rel_err": err,
+        "spmv_fp64": {"ms": spmv_ms, "gbs": alg / (spmv_ms * 1e-3) / 1e9, "algorithmic_bytes": alg,
+                      "layout": hip.FORMAT_NAMES.get(info["format"], "?")},
+        "pcg_fp64": fp64, "pcg_mixed": mixed,
+        "data": "synthetic: generated tanh-clustered 128^3 cavity, lid u=1, from rest; seeded N(0,1) zero-mean RHS"}),
+        flush=True)
 
 
 def adjoint_suite(args):

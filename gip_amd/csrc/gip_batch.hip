@@ -214,18 +214,12 @@ __global__ void k_bp_init(BPcg d) {
 // Ap = (A x0) or x0 on empty rows; r = b + (-1.0) Ap    (GIP_LinearSolver.h:55-70)
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_bp_setup_resid(EllView A, BPcg d, Seg S, const double *__restrict__ x0,
-                                                           double *__restrict__ p0, double *__restrict__ p1,
-                                                           float *__restrict__ q0, float *__restrict__ q1) {
+                                                           double *__restrict__ p, float *__restrict__ q) {
     const int stride = gridDim.x * kBlock;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < d.nall; i += stride) {
         d.x[i] = x0[i];
-        const double pv = i >= d.np ? x0[i] : 0.0;
-        p0[i] = pv;
-        p1[i] = pv;
-        if (q0) {
-            q0[i] = 0.0f;
-            q1[i] = 0.0f;
-        }
+        p[i] = i >= d.np ? x0[i] : 0.0;
+        if (q) q[i] = 0.0f;
     }
     Operand<double> xo{x0, nullptr, 0.0, d.np, false};
     FOR_CHUNKS(S, c) {
@@ -272,19 +266,36 @@ __global__ __launch_bounds__(kBlock) void k_bp_setup_fin(BPcg d, Seg S, double t
     }
 }
 
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_bp_spmv(EllView A, BPcg d, Seg S, const double *__restrict__ p_old,
-                                                    double *__restrict__ p_new) {
+// direction of the running instances: p = 1.0 s + beta p (GIP_LinearSolver.h:113;
+// the first iteration has beta = 0 and p = 0 on the solved rows, so p = s)
+__global__ __launch_bounds__(kBlock) void k_bp_dir(BPcg d, Seg S, double *__restrict__ p) {
+    const double *__restrict__ s = d.s;
     FOR_CHUNKS(S, c) {
         const int inst = S.chunk_inst[c];
         if (d.state[inst] != BS_RUN) continue;
         const int row = c * kChunk + threadIdx.x;
-        Operand<double> po{p_old, d.s, bsc(d, inst)[B_BETA], d.np, true};
-        const double pi = po.at(row);
-        p_new[row] = pi;
+        const double beta = bsc(d, inst)[B_BETA];
+        const double si = s[row], po = p[row];
+        p[row] = 1.0 * si + beta * po;
+    }
+}
+
+// Ap = A p (one gather of p per neighbour), chunk partials of p.Ap
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_bp_spmv(EllView A, BPcg d, Seg S, const double *__restrict__ p) {
+    double *__restrict__ Ap = d.Ap;
+    Operand<double> po{p, nullptr, 0.0, d.np, false};
+    FOR_CHUNKS(S, c) {
+        const int inst = S.chunk_inst[c];
+        if (d.state[inst] != BS_RUN) continue;
+        const int row = c * kChunk + threadIdx.x;
+        const double pi = p[row];
         double sum;
-        if (row_dot<W, double, double>(A, A.val, po, row, sum)) d.Ap[row] = sum;
-        double v[1] = {pi * d.Ap[row]};
+        if (row_dot<W, double, double>(A, A.val, po, row, sum))
+            Ap[row] = sum;
+        else
+            sum = Ap[row];
+        double v[1] = {pi * sum};
         seg_store<1, RED_SUM>(v, c, d.part);
     }
 }
@@ -308,15 +319,17 @@ __global__ __launch_bounds__(kBlock) void k_bp_update(BPcg d, Seg S, const doubl
         const int inst = S.chunk_inst[c];
         if (d.state[inst] != BS_RUN) continue;
         const int row = c * kChunk + threadIdx.x;
-        double *q = bsc(d, inst);
-        const double alpha = q[B_ALPHA];
-        d.x[row] = d.x[row] + alpha * p[row];
-        const double ri = d.r[row] + (-1.0 * alpha) * d.Ap[row];
+        const double alpha = bsc(d, inst)[B_ALPHA];
+        // every operand first: one memory round trip per row
+        const double xo = d.x[row], pr = p[row], ro = d.r[row], apr = d.Ap[row];
+        const double di = precond ? d.dinv[row] : 0.0;
+        d.x[row] = xo + alpha * pr;
+        const double ri = ro + (-1.0 * alpha) * apr;
         d.r[row] = ri;
         double si;
         if (precond) {
             si = 0.0;
-            si = si + ri * d.dinv[row];
+            si = si + ri * di;
         } else {
             si = ri;
         }
@@ -410,19 +423,30 @@ __global__ __launch_bounds__(kBlock) void k_bpm_setup_fin(BPcg d, Seg S, double 
     }
 }
 
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_bpm_spmv(EllView A, BPcg d, Seg S, const float *__restrict__ p_old,
-                                                     float *__restrict__ p_new) {
+__global__ __launch_bounds__(kBlock) void k_bpm_dir(BPcg d, Seg S, float *__restrict__ p) {
+    const float *__restrict__ s = d.s32;
     FOR_CHUNKS(S, c) {
         const int inst = S.chunk_inst[c];
         if (d.state[inst] != BS_RUN) continue;
         const int row = c * kChunk + threadIdx.x;
-        Operand<float> po{p_old, d.s32, (float)bsc(d, inst)[B_BETA], d.np, true};
-        const float pi = po.at(row);
-        p_new[row] = pi;
+        const float beta = (float)bsc(d, inst)[B_BETA];
+        const float si = s[row], po = p[row];
+        p[row] = 1.0f * si + beta * po;
+    }
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_bpm_spmv(EllView A, BPcg d, Seg S, const float *__restrict__ p) {
+    float *__restrict__ Ap = d.Ap32;
+    Operand<float> po{p, nullptr, 0.0f, d.np, false};
+    FOR_CHUNKS(S, c) {
+        const int inst = S.chunk_inst[c];
+        if (d.state[inst] != BS_RUN) continue;
+        const int row = c * kChunk + threadIdx.x;
+        const float pi = p[row];
         float sum;
         row_dot<W, float, float>(A, A.val32, po, row, sum);
-        d.Ap32[row] = sum;
+        Ap[row] = sum;
         double v[1] = {(double)pi * (double)sum};
         seg_store<1, RED_SUM>(v, c, d.part);
     }
@@ -433,12 +457,13 @@ __global__ __launch_bounds__(kBlock) void k_bpm_update(BPcg d, Seg S, const floa
         const int inst = S.chunk_inst[c];
         if (d.state[inst] != BS_RUN) continue;
         const int row = c * kChunk + threadIdx.x;
-        double *q = bsc(d, inst);
-        const float a32 = (float)q[B_ALPHA];
-        d.xc32[row] = d.xc32[row] + a32 * p[row];
-        const float ri = d.r32[row] - a32 * d.Ap32[row];
+        const float a32 = (float)bsc(d, inst)[B_ALPHA];
+        const float xo = d.xc32[row], pr = p[row], ro = d.r32[row], apr = d.Ap32[row];
+        const float di = precond ? d.dinv32[row] : 0.0f;
+        d.xc32[row] = xo + a32 * pr;
+        const float ri = ro - a32 * apr;
         d.r32[row] = ri;
-        const float si = precond ? ri * d.dinv32[row] : ri;
+        const float si = precond ? ri * di : ri;
         d.s32[row] = si;
         double v[2] = {(double)ri * (double)si, (double)ri * (double)ri};
         seg_store<2, RED_SUM>(v, c, d.part);
@@ -578,8 +603,8 @@ struct gip_batch {
     double *dt = nullptr, *sigma = nullptr, *time = nullptr, *gamma = nullptr, *massr = nullptr, *dt_used = nullptr,
            *fac = nullptr;
     // pcg
-    double *r = nullptr, *s = nullptr, *Ap = nullptr, *dinv = nullptr, *pp[2] = {};
-    float *r32 = nullptr, *s32 = nullptr, *Ap32 = nullptr, *xc32 = nullptr, *dinv32 = nullptr, *pq[2] = {};
+    double *r = nullptr, *s = nullptr, *Ap = nullptr, *dinv = nullptr, *pp = nullptr;  // pp: the direction
+    float *r32 = nullptr, *s32 = nullptr, *Ap32 = nullptr, *xc32 = nullptr, *dinv32 = nullptr, *pq = nullptr;
     double *cone = nullptr, *caz = nullptr;  // MIXED coarse step: 1 on inner rows, A 1
     double *sc = nullptr, *part = nullptr;
     int *state = nullptr, *iters = nullptr, *active = nullptr, *h_active = nullptr;
@@ -746,8 +771,7 @@ int enqueue_pre(gip_batch *b) {
     const gip_mat *L = b->op[OP_L];
     const bool mixed = b->o.precision == GIP_PCG_MIXED;
     k_bp_init<<<(b->B + 63) / 64, 64, 0, st>>>(d);
-    BDISPATCH(L, b->grid_in, k_bp_setup_resid, view_of(L), d, b->seg_in, b->p0, b->pp[0], b->pp[1],
-              mixed ? b->pq[0] : nullptr, mixed ? b->pq[1] : nullptr);
+    BDISPATCH(L, b->grid_in, k_bp_setup_resid, view_of(L), d, b->seg_in, b->p0, b->pp, mixed ? b->pq : nullptr);
     if (mixed) {
         k_bpm_sumr<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, 0);
         k_bpm_c_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in, 0);
@@ -762,24 +786,24 @@ int enqueue_pre(gip_batch *b) {
     return GIP_OK;
 }
 
-// `poll` fused iterations (poll is even, so the direction buffers alternate
-// back to the same parity at the end of the group)
+// `poll` iterations: direction, product, alpha, update, beta and convergence
 int enqueue_iter(gip_batch *b) {
     hipStream_t st = b->c->stream;
     BPcg d = pcg_view(b);
     const gip_mat *L = b->op[OP_L];
     const bool mixed = b->o.precision == GIP_PCG_MIXED;
     for (int k = 0; k < b->o.poll; ++k) {
-        const int cur = k & 1, old = cur ^ 1;
         if (mixed) {
-            BDISPATCH(L, b->grid_in, k_bpm_spmv, view_of(L), d, b->seg_in, b->pq[old], b->pq[cur]);
+            k_bpm_dir<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, b->pq);
+            BDISPATCH(L, b->grid_in, k_bpm_spmv, view_of(L), d, b->seg_in, b->pq);
             k_bp_spmv_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in);
-            k_bpm_update<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, b->pq[cur], 1);
+            k_bpm_update<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, b->pq, 1);
             k_bpm_update_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in);
         } else {
-            BDISPATCH(L, b->grid_in, k_bp_spmv, view_of(L), d, b->seg_in, b->pp[old], b->pp[cur]);
+            k_bp_dir<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, b->pp);
+            BDISPATCH(L, b->grid_in, k_bp_spmv, view_of(L), d, b->seg_in, b->pp);
             k_bp_spmv_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in);
-            k_bp_update<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, b->pp[cur], 1);
+            k_bp_update<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, b->pp, 1);
             k_bp_update_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in);
         }
     }
@@ -875,7 +899,6 @@ int gip_batch_create(gip_ctx *c, int32_t ninst, const gip_cavity *const *cavs, c
     b->rho = rho;
     b->o = *opts;
     if (b->o.poll <= 0) b->o.poll = 16;
-    if (b->o.poll & 1) b->o.poll += 1;
     // layout
     const int B = ninst;
     b->N.resize(B);
@@ -925,14 +948,12 @@ int gip_batch_create(gip_ctx *c, int32_t ninst, const gip_cavity *const *cavs, c
     double **pw[] = {&b->r, &b->s, &b->Ap, &b->dinv};
     for (double **f : pw)
         if (!rc) rc = dalloc_t(b, f, NPs);
-    if (!rc) rc = dalloc_t(b, &b->pp[0], NA);
-    if (!rc) rc = dalloc_t(b, &b->pp[1], NA);
+    if (!rc) rc = dalloc_t(b, &b->pp, NA);
     if (!rc && b->o.precision == GIP_PCG_MIXED) {
         float **fw[] = {&b->r32, &b->s32, &b->Ap32, &b->xc32, &b->dinv32};
         for (float **f : fw)
             if (!rc) rc = dalloc_t(b, f, NPs);
-        if (!rc) rc = dalloc_t(b, &b->pq[0], NA);
-        if (!rc) rc = dalloc_t(b, &b->pq[1], NA);
+        if (!rc) rc = dalloc_t(b, &b->pq, NA);
         if (!rc) rc = dalloc_t(b, &b->cone, NPs);
         if (!rc) rc = dalloc_t(b, &b->caz, NPs);
     }

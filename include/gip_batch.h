@@ -41,7 +41,7 @@ typedef struct gip_batch_opts {
     double pcg_tol;      /* relative residual target of each instance's solve (> 0)     */
     int32_t precision;   /* GIP_PCG_FP64 or GIP_PCG_MIXED                                */
     double inner_tol;    /* MIXED: FP32 residual drop that triggers an FP64 correction    */
-    int32_t poll;        /* iterations per launch group (even; 0: 16)                     */
+    int32_t poll;        /* iterations per launch group (0: 16)                           */
     int32_t use_graph;   /* capture the step phases as hipGraphs and replay them          */
     int32_t autotune;    /* choose each operator's SpMV layout with gip_mat_autotune      */
 } gip_batch_opts;
