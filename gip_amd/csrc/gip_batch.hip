@@ -84,7 +84,7 @@ template <int W>
 __global__ __launch_bounds__(kBlock) void k_b_spmv_seg(EllView A, int nrows, const double *__restrict__ x,
                                                        double *__restrict__ y, Seg S, const double *__restrict__ fac,
                                                        int ia, int ib) {
-    Operand<double> xo{x, nullptr, 0.0, 0, false};
+    Operand<double> xo{x};
     FOR_CHUNKS(S, c) {
         const int row = c * kChunk + threadIdx.x;
         if (row >= nrows) continue;
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_bp_setup_resid(EllView A, BPcg d, Se
         p[i] = i >= d.np ? x0[i] : 0.0;
         if (q) q[i] = 0.0f;
     }
-    Operand<double> xo{x0, nullptr, 0.0, d.np, false};
+    Operand<double> xo{x0};
     FOR_CHUNKS(S, c) {
         const int row = c * kChunk + threadIdx.x;
         double sum;
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_bp_dir(BPcg d, Seg S, double *__rest
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_bp_spmv(EllView A, BPcg d, Seg S, const double *__restrict__ p) {
     double *__restrict__ Ap = d.Ap;
-    Operand<double> po{p, nullptr, 0.0, d.np, false};
+    Operand<double> po{p};
     FOR_CHUNKS(S, c) {
         const int inst = S.chunk_inst[c];
         if (d.state[inst] != BS_RUN) continue;
@@ -438,7 +438,7 @@ __global__ __launch_bounds__(kBlock) void k_bpm_dir(BPcg d, Seg S, float *__rest
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_bpm_spmv(EllView A, BPcg d, Seg S, const float *__restrict__ p) {
     float *__restrict__ Ap = d.Ap32;
-    Operand<float> po{p, nullptr, 0.0f, d.np, false};
+    Operand<float> po{p};
     FOR_CHUNKS(S, c) {
         const int inst = S.chunk_inst[c];
         if (d.state[inst] != BS_RUN) continue;
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(kBlock) void k_bpm_fold(BPcg d, Seg S) {
 // convergence on the true ||r||.
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_bpm_resid(EllView A, BPcg d, Seg S) {
-    Operand<double> xo{d.x, nullptr, 0.0, d.np, false};
+    Operand<double> xo{d.x};
     FOR_CHUNKS(S, c) {
         const int inst = S.chunk_inst[c];
         if (d.state[inst] != BS_RUN || bsc(d, inst)[B_NEED] == 0.0) continue;

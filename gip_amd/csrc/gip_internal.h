@@ -75,7 +75,8 @@ struct gip_mat {
     int64_t st_tail_base;
     int32_t st_tail_nnz;
     int32_t *d_st_off;        // nslices * st_w
-    gip::StencilHdr *d_st_hdr;  // nslices
+    gip::StencilHdr *d_st_hdr;    // nslices (read for kind-2 slices only)
+    gip::StencilTail *d_st_tail;  // nslices
     int32_t *d_st_tcol;    // st_tail_nnz
     // a_ii per row (the stored values of a DIAG operator), for Jacobi
     double *d_diag;

@@ -42,10 +42,21 @@ struct StencilHdr {
 };
 static_assert(sizeof(StencilHdr) == 80, "slice header layout");
 
+// Compact per-slice tail record (16 bytes, read for every slice): kind 0 no
+// tail and no empty row; kind 1 only first tail entries (lane mask m0, no
+// empty row); kind 2 anything else (the full header decides).
+struct StencilTail {
+    int32_t tsoff;
+    int32_t kind;
+    uint64_t m0;
+};
+static_assert(sizeof(StencilTail) == 16, "slice tail record layout");
+
 struct StencilPlan {
     int32_t ns = 0, w = 0;
     std::vector<int32_t> off;     // ns * w offsets
     std::vector<StencilHdr> hdr;  // ns slice headers
+    std::vector<StencilTail> tail;  // ns compact tail records
     std::vector<int32_t> tcol;    // tail columns
     std::vector<int32_t> pos;     // per stored entry: position in the value array
     int64_t tail_base = 0;        // ns * w * 64: first tail value
@@ -142,6 +153,14 @@ inline bool plan_stencil(int32_t nrows, int32_t nnz, const int32_t *rowptr, cons
         h.any = (h.m[0] | h.empty) ? 1 : 0;
         if ((int64_t)P.tcol.size() > max_tail) return false;
     }
+    P.tail.assign((size_t)ns, StencilTail{});
+    for (int32_t s = 0; s < ns; ++s) {
+        const StencilHdr &h = P.hdr[(size_t)s];
+        StencilTail &t = P.tail[(size_t)s];
+        t.tsoff = h.tsoff;
+        t.m0 = h.m[0];
+        t.kind = !h.any ? 0 : (h.empty == 0 && h.m[1] == 0) ? 1 : 2;
+    }
     P.stored = P.tail_base + (int64_t)P.tcol.size();
     if (P.stored > INT32_MAX) return false;
     for (size_t t = 0; t < tail_entry.size(); ++t) P.pos[(size_t)tail_entry[t]] = (int32_t)(P.tail_base + (int64_t)t);
@@ -158,9 +177,13 @@ inline int32_t stencil_tail_index(const StencilHdr &h, int lane, int rank) {
 }
 
 // Bytes one FP64 product streams for the operator in this layout (values,
-// tail columns, slice offsets and headers).
+// tail columns, slice offsets, tail records, the full headers of kind-2
+// slices).
 inline int64_t stencil_bytes(const StencilPlan &P, int32_t) {
-    return 8 * P.stored + 4 * (int64_t)P.tcol.size() + (int64_t)P.ns * (4 * P.w + (int64_t)sizeof(StencilHdr));
+    int64_t general = 0;
+    for (const StencilTail &t : P.tail) general += t.kind == 2;
+    return 8 * P.stored + 4 * (int64_t)P.tcol.size() + (int64_t)P.ns * (4 * P.w + (int64_t)sizeof(StencilTail)) +
+           general * (int64_t)sizeof(StencilHdr);
 }
 
 }  // namespace gip

@@ -1,4 +1,4 @@
-// Fused, device-resident Jacobi-PCG of libgip_hip (GIP_LinearSolver.h:45-117).
+// Device-resident Jacobi-PCG of libgip_hip (GIP_LinearSolver.h:45-117).
 //
 // FP64 iteration k, three kernels, the reference recurrences and roundings:
 //   K0  rho_k = sum of the r.s partials; convergence test ||r|| <= tol ||b||;
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_resid(EllView A, PcgDev d, con
         d.x[i] = x0[i];
         if (i >= d.n) p[i] = x0[i];
     }
-    Operand<double> xo{x0, nullptr, 0.0, d.n, false};
+    Operand<double> xo{x0};
     FOR_ROWS(d.n, row) {
         double sum;
         bool any = row_dot<W, double, double>(A, A.val, xo, row, sum);
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_iter_dir(PcgDev d, double *__restric
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_iter_spmv(EllView A, PcgDev d, const double *__restrict__ p) {
     if (!running(d)) return;
-    Operand<double> po{p, nullptr, 0.0, d.n, false};
+    Operand<double> po{p};
     double *__restrict__ Ap = d.Ap;
     double acc = 0.0;
     FOR_ROWS(d.n, row) {
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_m_dir(PcgDev d, float *__restrict__ 
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_m_spmv(EllView A, PcgDev d, const float *__restrict__ p) {
     if (!running(d)) return;
-    Operand<float> po{p, nullptr, 0.0f, d.n, false};
+    Operand<float> po{p};
     float *__restrict__ Ap = d.Ap32;
     double acc = 0.0;
     FOR_ROWS(d.n, row) {
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(kBlock) void k_m_fold(PcgDev d) {
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_m_resid(EllView A, PcgDev d) {
     double sr = 0.0;
-    Operand<double> xo{d.x, nullptr, 0.0, d.n, false};
+    Operand<double> xo{d.x};
     FOR_ROWS(d.n, row) {
         double sum;
         bool any = row_dot<W, double, double>(A, A.val, xo, row, sum);

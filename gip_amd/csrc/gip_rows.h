@@ -1,6 +1,6 @@
-// Device helpers shared by the PCG and batch kernels: a read-only view of an
-// operator, the product operand (plain, or the PCG direction formed on the
-// fly), the stored-order row sum, and the XCD-contiguous row sweep.
+// Device helpers shared by the SpMV, PCG and batch kernels: a read-only
+// view of an operator, the product operand, the stored-order row sum, and
+// the XCD-contiguous row sweep.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,6 +17,7 @@ struct EllView {
     // stencil layout
     const int32_t *st_off, *st_tcol;
     const StencilHdr *st_hdr;
+    const StencilTail *st_tail;
     int st_w, tail_base, cmax;
 };
 
@@ -32,6 +33,7 @@ static EllView view(const gip_mat *A) {
     v.st_off = A->d_st_off;
     v.st_tcol = A->d_st_tcol;
     v.st_hdr = A->d_st_hdr;
+    v.st_tail = A->d_st_tail;
     v.st_w = A->st_w;
     v.tail_base = (int)A->st_tail_base;
     v.cmax = A->cols_needed > 0 ? A->cols_needed - 1 : 0;
@@ -46,19 +48,11 @@ static inline int kernel_width(const gip_mat *A) {
     return 0;
 }
 
-// The operand of a product: either a plain vector, or the direction formed
-// on the fly as 1.0 s_j + beta p_j on the solved rows (p_j beyond them).
+// The input vector of a product.
 template <class T>
 struct Operand {
-    const T *p;
-    const T *s;
-    T beta;
-    int n;
-    bool fused;
-    __device__ inline T at(int j) const {
-        if (!fused || j >= n) return p[j];
-        return T(1) * s[j] + beta * p[j];
-    }
+    const T *__restrict__ p;
+    __device__ inline T at(int j) const { return p[j]; }
 };
 
 // Stencil-layout row sum: the slots in order (column row + offset, clamped
@@ -89,8 +83,19 @@ __device__ inline bool stencil_row(const EllView &A, const V *__restrict__ val, 
             sum = sum + x.at(cc < 0 ? 0 : (cc > A.cmax ? A.cmax : cc)) * vs[64 * k];
         }
     }
-    const StencilHdr *h = A.st_hdr + s;
-    if (h->any) {
+    // compact 16-byte record per slice; the 80-byte header only for the
+    // rare slices with empty rows or rows of several tail entries
+    const StencilTail *tl = A.st_tail + s;
+    const int kind = tl->kind;
+    if (kind == 1) {
+        const uint64_t bit = 1ull << (row & 63);
+        const uint64_t m0 = tl->m0;
+        if (m0 & bit) {
+            const int e = tl->tsoff + __popcll(m0 & (bit - 1ull));
+            sum = sum + x.at(A.st_tcol[e]) * val[A.tail_base + e];
+        }
+    } else if (kind == 2) {
+        const StencilHdr *h = A.st_hdr + s;
         const uint64_t bit = 1ull << (row & 63), below = bit - 1ull;
         if (h->empty & bit) return false;  // empty row: left untouched
         int t = h->tsoff;

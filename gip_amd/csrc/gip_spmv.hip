@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil(EllView A, int nrows, const 
                                                    const int32_t *__restrict__ scatter) {
     const int row = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (row >= nrows) return;
-    Operand<double> xo{x, nullptr, 0.0, 0, false};
+    Operand<double> xo{x};
     double sum;
     if (!row_dot<W, double, double>(A, A.val, xo, row, sum)) return;
     const int out = scatter ? scatter[row] : row;
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil32(EllView A, int nrows, cons
                                                      float *__restrict__ y) {
     const int row = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (row >= nrows) return;
-    Operand<float> xo{x, nullptr, 0.0f, 0, false};
+    Operand<float> xo{x};
     float sum;
     row_dot<0, float, float>(A, A.val32, xo, row, sum);
     y[row] = sum;
@@ -231,7 +231,7 @@ static int upload(gip_ctx *c, T **dptr, const T *host, size_t count) {
 static void free_mat_arrays(gip_mat *A) {
     void *ptrs[] = {A->d_rowptr,   A->d_col,     A->d_val,      A->d_val32,    A->d_slice_off, A->d_ell_col,
                     A->d_ell_val,  A->d_ell_val32, A->d_csr2ell, A->d_diag,     A->d_tsrc,      A->d_tdst,
-                    A->d_st_off,   A->d_st_hdr,   A->d_st_tcol};
+                    A->d_st_off,   A->d_st_hdr,   A->d_st_tail,  A->d_st_tcol};
     for (void *p : ptrs)
         if (p) hipFree(p);
 }
@@ -480,6 +480,7 @@ int gip_mat_create(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, const 
         if (!rc) rc = upload(c, &A->d_csr2ell, plan.pos.data(), plan.pos.size());
         if (!rc) rc = upload(c, &A->d_st_off, plan.off.data(), plan.off.size());
         if (!rc) rc = upload(c, &A->d_st_hdr, plan.hdr.data(), plan.hdr.size());
+        if (!rc) rc = upload(c, &A->d_st_tail, plan.tail.data(), plan.tail.size());
         if (!rc) rc = upload(c, &A->d_st_tcol, plan.tcol.data(), plan.tcol.size());
         if (!rc) rc = upload(c, &A->d_col, col, (size_t)nnz);
         if (!rc) {

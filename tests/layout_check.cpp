@@ -63,6 +63,7 @@ int main(int argc, char **argv) {
         const bool empty_csr = rp[(size_t)r] == rp[(size_t)r + 1];
         const int32_t s = r / 64, lane = r % 64;
         const gip::StencilHdr &h = P.hdr[(size_t)s];
+        const gip::StencilTail &tl = P.tail[(size_t)s];
         double got = 0.0;
         bool empty = false;
         for (int32_t k = 0; k < P.w; ++k) {
@@ -70,7 +71,17 @@ int main(int argc, char **argv) {
             c = c < 0 ? 0 : (c > cmax ? cmax : c);
             got = got + x[(size_t)c] * st[(size_t)(((int64_t)s * P.w + k) * 64 + lane)];
         }
-        if (h.any) {
+        if (tl.kind == 1) {  // first tail entries only: the compact record decides
+            const uint64_t bit = 1ull << lane;
+            if (tl.m0 & bit) {
+                const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+                const int32_t t = tl.tsoff + __builtin_popcountll(tl.m0 & below);
+                if (t < 0 || t >= (int32_t)P.tcol.size() || P.tcol[(size_t)t] > cmax)
+                    ++bad_cols;
+                else
+                    got = got + x[(size_t)P.tcol[(size_t)t]] * st[(size_t)(P.tail_base + t)];
+            }
+        } else if (tl.kind == 2) {
             const uint64_t bit = 1ull << lane;
             empty = (h.empty & bit) != 0;
             for (int j = 0; j < gip::kStencilMaxTail && (h.m[j] & bit); ++j) {
@@ -82,6 +93,7 @@ int main(int argc, char **argv) {
                 got = got + x[(size_t)P.tcol[(size_t)t]] * st[(size_t)(P.tail_base + t)];
             }
         }
+        if ((tl.kind == 0) != (h.any == 0) || tl.tsoff != h.tsoff || tl.m0 != h.m[0]) ++bad_pos;
         if (empty != empty_csr || (!empty && std::memcmp(&got, &want, sizeof(double)) != 0)) ++bad_rows;
     }
     std::printf("{\"planned\": true, \"w\": %d, \"slices\": %d, \"tail\": %zu, \"stored\": %lld, \"bytes\": %lld, "
