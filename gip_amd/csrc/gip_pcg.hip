@@ -5,6 +5,8 @@
 //       beta = rho_k / rho_{k-1}; p = 1.0 s + beta p on the solved rows
 //       (GIP_LinearSolver.h:107-113; iteration 0 uses p from the setup)
 //   K1  Ap = A p, partials of p.Ap                             (:87-91)
+//       (one row per thread over the whole matrix, one partial per 256-row
+//       chunk, folded to one partial per 256 chunks by a small kernel)
 //   K2  dn = sum of the p.Ap partials, alpha = rho_k / dn;
 //       x += alpha p ; r += (-alpha) Ap ; s = Dinv r ; partials of r.s, r.r
 // Cross-workgroup communication is fence-free: every workgroup stores its
@@ -63,7 +65,10 @@ enum { ST_RUN = 0, ST_CONV = 1, ST_BREAK = 2 };
 enum { P_DN = 0, P_RS = 1, P_RR = 2, P_BB = 3, P_RS2 = 4, P_RR2 = 5, P_SUMR = 6, P_COUNT = 7 };
 
 struct PcgDev {
-    int n, G;  // solved rows, workgroups of every iteration kernel
+    int n, G;      // solved rows, workgroups of the sweeping kernels (<= kMaxRedBlocks)
+    int nchunk;    // 256-row chunks: workgroups of the product kernels
+    int Gdn;       // p.Ap partials after the fold (<= kMaxRedBlocks)
+    double *chunk; // nchunk per-chunk partials of the product
     double *x, *r, *s, *Ap;
     const double *b, *dinv;
     float *r32, *s32, *Ap32, *xc32;
@@ -169,30 +174,41 @@ __global__ __launch_bounds__(kBlock) void k_iter_dir(PcgDev d, double *__restric
     }
 }
 
-// K1: Ap = A p (empty rows keep Ap), partials of p.Ap
+// K1: Ap = A p (empty rows keep Ap), one partial of p.Ap per 256-row chunk;
+// one row per thread, workgroups in XCD-contiguous order
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_iter_spmv(EllView A, PcgDev d, const double *__restrict__ p) {
     if (!running(d)) return;
-    Operand<double> po{p};
-    double *__restrict__ Ap = d.Ap;
+    const int c = xcd_block(blockIdx.x, gridDim.x);
+    const int row = c * kBlock + threadIdx.x;
     double acc = 0.0;
-    FOR_ROWS(d.n, row) {
-        const double pr = p[row];  // issued with the row's gathers
+    if (row < d.n) {
+        Operand<double> po{p};
+        double *__restrict__ Ap = d.Ap;
+        const double pr = p[row];
         double sum;
-        if (row_dot<W, double, double>(A, A.val, po, row, sum)) {
+        if (row_dot<W, double, double>(A, A.val, po, row, sum))
             Ap[row] = sum;
-        } else {
+        else
             sum = Ap[row];  // empty row: Ap keeps its value
-        }
-        acc = acc + pr * sum;
+        acc = pr * sum;
     }
     acc = block_reduce<RED_SUM>(acc);
-    if (threadIdx.x == 0) d.P(P_DN)[blockIdx.x] = acc;
+    if (threadIdx.x == 0) d.chunk[c] = acc;
+}
+
+// the per-chunk partials, folded in index order into Gdn partials of P_DN
+__global__ __launch_bounds__(kBlock) void k_fold_dn(PcgDev d) {
+    if (!running(d)) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double v = i < d.nchunk ? d.chunk[i] : 0.0;
+    v = block_reduce<RED_SUM>(v);
+    if (threadIdx.x == 0) d.P(P_DN)[blockIdx.x] = v;
 }
 
 __global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, const double *__restrict__ p, int k) {
     if (!running(d)) return;
-    const double dn = sum_partials<RED_SUM>(d.P(P_DN), d.G);
+    const double dn = sum_partials<RED_SUM>(d.P(P_DN), d.Gdn);
     const double alpha = rho_slot(d, k) / dn;  // :93
     if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) {
         if (leader()) *d.state = ST_BREAK;
@@ -254,27 +270,28 @@ __global__ __launch_bounds__(kBlock) void k_m_dir(PcgDev d, float *__restrict__ 
     }
 }
 
-// K1: Ap32 = A32 p32, partials of p.Ap (FP64 accumulation)
+// K1: Ap32 = A32 p32, one partial of p.Ap (FP64 accumulation) per chunk
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_m_spmv(EllView A, PcgDev d, const float *__restrict__ p) {
     if (!running(d)) return;
-    Operand<float> po{p};
-    float *__restrict__ Ap = d.Ap32;
+    const int c = xcd_block(blockIdx.x, gridDim.x);
+    const int row = c * kBlock + threadIdx.x;
     double acc = 0.0;
-    FOR_ROWS(d.n, row) {
-        const float pr = p[row];  // issued with the row's gathers
+    if (row < d.n) {
+        Operand<float> po{p};
+        const float pr = p[row];
         float sum;
         row_dot<W, float, float>(A, A.val32, po, row, sum);
-        Ap[row] = sum;
-        acc = acc + (double)pr * (double)sum;
+        d.Ap32[row] = sum;
+        acc = (double)pr * (double)sum;
     }
     acc = block_reduce<RED_SUM>(acc);
-    if (threadIdx.x == 0) d.P(P_DN)[blockIdx.x] = acc;
+    if (threadIdx.x == 0) d.chunk[c] = acc;
 }
 
 __global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, const float *__restrict__ p, int k) {
     if (!running(d)) return;
-    const double dn = sum_partials<RED_SUM>(d.P(P_DN), d.G);
+    const double dn = sum_partials<RED_SUM>(d.P(P_DN), d.Gdn);
     const double alpha = rho_slot(d, k) / dn;
     if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) {
         if (leader()) *d.state = ST_BREAK;
@@ -419,13 +436,15 @@ struct gip_pcg {
     double *r, *p, *s, *Ap, *dinv;
     float *r32, *p32, *s32, *Ap32, *xc32, *dinv32;
     double *az;  // A 1 (MIXED coarse step)
-    double *sc, *part;
+    double *sc, *part, *chunk;
+    int nchunk, gdn;
     int *state;    // state, iters
     int *h_state;  // pinned copy
     double *h_sc;  // pinned scalars
 };
 
 // Multiple of 8 (one share per XCD), at most kMaxRedBlocks (one partial each).
+// The product kernels run one workgroup per 256-row chunk instead.
 static int pcg_grid(int n) {
     int nchunk = (n + kBlock - 1) / kBlock;
     int g = ((nchunk + 7) / 8) * 8;
@@ -436,22 +455,26 @@ static int pcg_grid(int n) {
 
 // Launch KERNEL<W> with W the operator's kernel width: a uniform ELL width
 // (5..7), a stencil width (-7, -8), or 0 for the generic loops.
-#define DISPATCH_W(A, KERNEL, ...)                                              \
+#define DISPATCH_WG(A, GRID, KERNEL, ...)                                     \
     do {                                                                        \
         switch (kernel_width(A)) {                                              \
-            case 5: KERNEL<5><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;   \
-            case 6: KERNEL<6><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;   \
-            case 7: KERNEL<7><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;   \
-            case -7: KERNEL<-7><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break; \
-            case -8: KERNEL<-8><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break; \
-            default: KERNEL<0><<<S->grid, kBlock, 0, st>>>(__VA_ARGS__); break;  \
+            case 5: KERNEL<5><<<GRID, kBlock, 0, st>>>(__VA_ARGS__); break;      \
+            case 6: KERNEL<6><<<GRID, kBlock, 0, st>>>(__VA_ARGS__); break;      \
+            case 7: KERNEL<7><<<GRID, kBlock, 0, st>>>(__VA_ARGS__); break;      \
+            case -7: KERNEL<-7><<<GRID, kBlock, 0, st>>>(__VA_ARGS__); break;    \
+            case -8: KERNEL<-8><<<GRID, kBlock, 0, st>>>(__VA_ARGS__); break;    \
+            default: KERNEL<0><<<GRID, kBlock, 0, st>>>(__VA_ARGS__); break;     \
         }                                                                       \
     } while (0)
+#define DISPATCH_W(A, KERNEL, ...) DISPATCH_WG(A, S->grid, KERNEL, __VA_ARGS__)
 
 static PcgDev make_dev(gip_pcg *S, const double *b, double *x, int precond) {
     PcgDev d;
     d.n = S->n;
     d.G = S->grid;
+    d.nchunk = S->nchunk;
+    d.Gdn = S->gdn;
+    d.chunk = S->chunk;
     d.x = x;
     d.r = S->r;
     d.s = S->s;
@@ -486,6 +509,12 @@ int gip_pcg_create(gip_ctx *c, const gip_mat *A, int32_t n_all, gip_pcg **out) {
     S->n = A->nrows;
     S->n_all = n_all;
     S->grid = pcg_grid(A->nrows);
+    S->nchunk = std::max(1, (A->nrows + kBlock - 1) / kBlock);
+    S->gdn = (S->nchunk + kBlock - 1) / kBlock;
+    if (S->gdn > kMaxRedBlocks) {
+        delete S;
+        return set_error(GIP_EINVAL, "gip_pcg_create: more rows than the product partials cover (2^27)");
+    }
     size_t n = (size_t)std::max(1, A->nrows);
     size_t na = (size_t)std::max(1, n_all);  // directions are SpMV inputs: ghost columns included
     hipError_t e = hipSetDevice(c->device);
@@ -499,6 +528,7 @@ int gip_pcg_create(gip_ctx *c, const gip_mat *A, int32_t n_all, gip_pcg **out) {
     dalloc((void **)&S->dinv, n * 8);
     dalloc((void **)&S->sc, S_COUNT * 8);
     dalloc((void **)&S->part, (size_t)P_COUNT * kMaxRedBlocks * 8);
+    dalloc((void **)&S->chunk, (size_t)S->nchunk * 8);
     dalloc((void **)&S->state, 2 * sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_state, 2 * sizeof(int), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_sc, S_COUNT * 8, hipHostMallocDefault);
@@ -527,7 +557,7 @@ int gip_pcg_destroy(gip_pcg *S) {
     hipSetDevice(S->ctx->device);
     hipStreamSynchronize(S->ctx->stream);
     void *ptrs[] = {S->r,   S->p,    S->s,    S->Ap,     S->dinv, S->r32, S->p32,  S->s32,
-                    S->Ap32, S->xc32, S->dinv32, S->az, S->sc,   S->part, S->state};
+                    S->Ap32, S->xc32, S->dinv32, S->az, S->sc,   S->part, S->chunk, S->state};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (S->h_state) hipHostFree(S->h_state);
@@ -611,7 +641,8 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
             int chunk = std::min(pollk, o->maxit - k);
             for (int j = 0; j < chunk; ++j, ++k) {
                 if (k > 0) k_iter_dir<<<S->grid, kBlock, 0, st>>>(d, S->p, k);
-                DISPATCH_W(A, k_iter_spmv, v, d, S->p);
+                DISPATCH_WG(A, S->nchunk, k_iter_spmv, v, d, S->p);
+                k_fold_dn<<<S->gdn, kBlock, 0, st>>>(d);
                 k_iter_update<<<S->grid, kBlock, 0, st>>>(d, S->p, k);
             }
             GIP_LAUNCH_CHECK("pcg iteration");
@@ -635,7 +666,8 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
             int chunk = std::min(pollk, o->maxit - k);
             for (int j = 0; j < chunk; ++j, ++k) {
                 if (k > 0) k_m_dir<<<S->grid, kBlock, 0, st>>>(d, S->p32, k);
-                DISPATCH_W(A, k_m_spmv, v, d, S->p32);
+                DISPATCH_WG(A, S->nchunk, k_m_spmv, v, d, S->p32);
+                k_fold_dn<<<S->gdn, kBlock, 0, st>>>(d);
                 k_m_update<<<S->grid, kBlock, 0, st>>>(d, S->p32, k);
             }
             k_m_status<<<1, kBlock, 0, st>>>(d);

@@ -48,6 +48,14 @@ static inline int kernel_width(const gip_mat *A) {
     return 0;
 }
 
+// Workgroup b of G is placed on XCD b % 8 by the dispatcher; renumber the
+// workgroups so that each XCD sweeps one contiguous range of rows and the
+// neighbour gathers of a row range stay in that XCD's L2.
+__device__ inline int xcd_block(int b, int G) {
+    const int x = b & 7, idx = b >> 3, q = G >> 3, r = G & 7;
+    return x * q + (x < r ? x : r) + idx;
+}
+
 // The input vector of a product.
 template <class T>
 struct Operand {

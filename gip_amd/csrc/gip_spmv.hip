@@ -26,14 +26,6 @@
 
 namespace gip {
 
-// Workgroup b of G is placed on XCD b % 8 by the dispatcher; renumber the
-// workgroups so that each XCD sweeps one contiguous range of rows and the
-// neighbour gathers of a row range stay in that XCD's L2.
-__device__ inline int xcd_block(int b, int G) {
-    const int x = b & 7, idx = b >> 3, q = G >> 3, r = G & 7;
-    return x * q + (x < r ? x : r) + idx;
-}
-
 template <int MODE, int W>
 __global__ __launch_bounds__(kBlock) void k_ell(int nrows, const int32_t *__restrict__ soff,
                                                const int32_t *__restrict__ col, const double *__restrict__ val,
