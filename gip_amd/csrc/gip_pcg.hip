@@ -2,13 +2,16 @@
 //
 // FP64 iteration k, three kernels, the reference recurrences and roundings:
 //   K0  rho_k = sum of the r.s partials; convergence test ||r|| <= tol ||b||;
-//       beta = rho_k / rho_{k-1}; p = 1.0 s + beta p on the solved rows
-//       (GIP_LinearSolver.h:107-113; iteration 0 uses p from the setup)
+//       x += alpha_{k-1} p (the previous iteration's update, :95, moved here
+//       so that p is read once per iteration); beta = rho_k / rho_{k-1};
+//       p = 1.0 s + beta p on the solved rows (GIP_LinearSolver.h:107-113;
+//       iteration 0 uses p from the setup; the last iteration's x update is
+//       applied by a flush kernel)
 //   K1  Ap = A p, partials of p.Ap                             (:87-91)
 //       (one row per thread over the whole matrix, one partial per 256-row
 //       chunk, folded to one partial per 256 chunks by a small kernel)
 //   K2  dn = sum of the p.Ap partials, alpha = rho_k / dn;
-//       x += alpha p ; r += (-alpha) Ap ; s = Dinv r ; partials of r.s, r.r
+//       r += (-alpha) Ap ; s = Dinv r ; partials of r.s, r.r
 // Cross-workgroup communication is fence-free: every workgroup stores its
 // partial sums with plain stores, and every workgroup of the NEXT kernel
 // re-reduces all partials in its prologue in the same fixed order, so all
@@ -153,25 +156,51 @@ __global__ __launch_bounds__(kBlock) void k_setup_fin(PcgDev d, double tol) {
 
 // ---- FP64 iteration --------------------------------------------------------
 
-// K0 (k >= 1): rho_k, the convergence test, beta, p = 1.0 s + beta p
+// alpha of iteration k from its p.Ap partials (the value K2 of k used)
+__device__ inline double alpha_of(const PcgDev &d, int k) {
+    return rho_slot(d, k) / sum_partials<RED_SUM>(d.P(P_DN), d.Gdn);
+}
+
+// x += alpha_k p over the solved rows
+__device__ inline void x_update(const PcgDev &d, const double *__restrict__ p, double alpha) {
+    double *__restrict__ x = d.x;
+    FOR_ROWS(d.n, row) {
+        const double xo = x[row], po = p[row];
+        x[row] = xo + alpha * po;  // :95
+    }
+}
+
+// K0 (k >= 1): rho_k, the convergence test, x += alpha_{k-1} p,
+// beta, p = 1.0 s + beta p
 __global__ __launch_bounds__(kBlock) void k_iter_dir(PcgDev d, double *__restrict__ p, int k) {
     if (!running(d)) return;
     const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);  // :107
+    const double alpha = alpha_of(d, k - 1);
     const double tol2 = d.sc[S_TOL2];
     if (tol2 >= 0.0) {
         const double rr = sum_partials<RED_SUM>(d.P(P_RR), d.G);
         if (rr <= tol2) {  // every workgroup reaches the same decision
+            x_update(d, p, alpha);
             if (leader()) *d.state = ST_CONV;
             return;
         }
     }
     const double beta = rho / rho_slot(d, k - 1);  // :109
     if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
+    double *__restrict__ x = d.x;
     const double *__restrict__ s = d.s;
     FOR_ROWS(d.n, row) {
-        const double si = s[row], po = p[row];
+        const double xo = x[row], si = s[row], po = p[row];
+        x[row] = xo + alpha * po;       // :95 of iteration k-1
         p[row] = 1.0 * si + beta * po;  // :113
     }
+}
+
+// after the loop: the last iteration's x update, unless the solve stopped in
+// K0 (converged: already applied) or on a breakdown (no valid alpha)
+__global__ __launch_bounds__(kBlock) void k_x_flush(PcgDev d, const double *__restrict__ p, int k) {
+    if (!running(d)) return;
+    x_update(d, p, alpha_of(d, k));
 }
 
 // K1: Ap = A p (empty rows keep Ap), one partial of p.Ap per 256-row chunk;
@@ -206,7 +235,7 @@ __global__ __launch_bounds__(kBlock) void k_fold_dn(PcgDev d) {
     if (threadIdx.x == 0) d.P(P_DN)[blockIdx.x] = v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, const double *__restrict__ p, int k) {
+__global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, int k) {
     if (!running(d)) return;
     const double dn = sum_partials<RED_SUM>(d.P(P_DN), d.Gdn);
     const double alpha = rho_slot(d, k) / dn;  // :93
@@ -215,15 +244,14 @@ __global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, const double *
         return;
     }
     const double malpha = -1.0 * alpha;
-    double *__restrict__ x = d.x, *__restrict__ r = d.r, *__restrict__ s = d.s;
+    double *__restrict__ r = d.r, *__restrict__ s = d.s;
     const double *__restrict__ Ap = d.Ap, *__restrict__ dinv = d.dinv;
     const bool precond = d.precond != 0;
     double rs = 0.0, rr = 0.0;
     FOR_ROWS(d.n, row) {
         // every operand first, so the row costs one memory round trip
-        const double xo = x[row], pr = p[row], ro = r[row], apr = Ap[row];
+        const double ro = r[row], apr = Ap[row];
         const double di = precond ? dinv[row] : 0.0;
-        x[row] = xo + alpha * pr;        // :95
         const double ri = ro + malpha * apr;  // :97
         r[row] = ri;
         double si;
@@ -256,16 +284,20 @@ __global__ __launch_bounds__(kBlock) void k_fin(PcgDev d) {
 
 // ---- MIXED iteration (FP32 vectors and operator, FP64 dot accumulation) --------
 
-// K0 (k >= 1): rho_k, beta (0 after a restart: the previous rho slot is
-// +inf), p32 = s32 + beta p32
-__global__ __launch_bounds__(kBlock) void k_m_dir(PcgDev d, float *__restrict__ p, int k) {
+// K0 (k >= 1): rho_k, the correction update xc += alpha_{k-1} p of the
+// previous iteration (`pending`: not yet folded by a defect correction),
+// beta (0 after a restart: the previous rho slot is +inf), p32 = s32 + beta p32
+__global__ __launch_bounds__(kBlock) void k_m_dir(PcgDev d, float *__restrict__ p, int k, int pending) {
     if (!running(d)) return;
     const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);
+    const float a32 = pending ? (float)alpha_of(d, k - 1) : 0.0f;
     const float beta = (float)(rho / rho_slot(d, k - 1));
     if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
+    float *__restrict__ xc = d.xc32;
     const float *__restrict__ s = d.s32;
     FOR_ROWS(d.n, row) {
-        const float si = s[row], po = p[row];
+        const float xo = xc[row], si = s[row], po = p[row];
+        if (pending) xc[row] = xo + a32 * po;
         p[row] = si + beta * po;
     }
 }
@@ -289,7 +321,7 @@ __global__ __launch_bounds__(kBlock) void k_m_spmv(EllView A, PcgDev d, const fl
     if (threadIdx.x == 0) d.chunk[c] = acc;
 }
 
-__global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, const float *__restrict__ p, int k) {
+__global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, int k) {
     if (!running(d)) return;
     const double dn = sum_partials<RED_SUM>(d.P(P_DN), d.Gdn);
     const double alpha = rho_slot(d, k) / dn;
@@ -298,15 +330,14 @@ __global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, const float *__re
         return;
     }
     const float a32 = (float)alpha;
-    float *__restrict__ xc = d.xc32, *__restrict__ r = d.r32, *__restrict__ s = d.s32;
+    float *__restrict__ r = d.r32, *__restrict__ s = d.s32;
     const float *__restrict__ Ap = d.Ap32, *__restrict__ dinv = d.dinv32;
     const bool precond = d.precond != 0;
     double rs = 0.0, rr = 0.0;
     FOR_ROWS(d.n, row) {
         // every operand first, so the row costs one memory round trip
-        const float xo = xc[row], pr = p[row], ro = r[row], apr = Ap[row];
+        const float ro = r[row], apr = Ap[row];
         const float di = precond ? dinv[row] : 0.0f;
-        xc[row] = xo + a32 * pr;
         const float ri = ro - a32 * apr;
         r[row] = ri;
         const float si = precond ? ri * di : ri;
@@ -327,10 +358,12 @@ __global__ __launch_bounds__(kBlock) void k_m_status(PcgDev d) {
     if (threadIdx.x == 0) d.sc[S_RR] = rr;
 }
 
-// x += xc, xc = 0
-__global__ __launch_bounds__(kBlock) void k_m_fold(PcgDev d) {
+// x += (xc + alpha_k p) (the last iteration's pending update first), xc = 0
+__global__ __launch_bounds__(kBlock) void k_m_fold(PcgDev d, const float *__restrict__ p, int k) {
+    const float a32 = (float)alpha_of(d, k);
     FOR_ROWS(d.n, row) {
-        d.x[row] = d.x[row] + (double)d.xc32[row];
+        const float xcv = d.xc32[row] + a32 * p[row];
+        d.x[row] = d.x[row] + (double)xcv;
         d.xc32[row] = 0.0f;
     }
 }
@@ -643,11 +676,12 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
                 if (k > 0) k_iter_dir<<<S->grid, kBlock, 0, st>>>(d, S->p, k);
                 DISPATCH_WG(A, S->nchunk, k_iter_spmv, v, d, S->p);
                 k_fold_dn<<<S->gdn, kBlock, 0, st>>>(d);
-                k_iter_update<<<S->grid, kBlock, 0, st>>>(d, S->p, k);
+                k_iter_update<<<S->grid, kBlock, 0, st>>>(d, k);
             }
             GIP_LAUNCH_CHECK("pcg iteration");
             if ((rc = poll(S, false))) return rc;
         }
+        if (k > 0) k_x_flush<<<S->grid, kBlock, 0, st>>>(d, S->p, k - 1);
         k_fin<<<1, kBlock, 0, st>>>(d);
         GIP_LAUNCH_CHECK("pcg final test");
     } else {
@@ -661,14 +695,15 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
         double last_rr = S->h_sc[S_TRUE_RR];
         const double it = o->inner_tol > 0.0 ? o->inner_tol : 0.1;
         const double inner2 = it * it;
-        int k = 0;
+        int k = 0, pending = 0;  // pending: the last iteration's correction update is not yet applied
         while (k < o->maxit && S->h_state[0] == ST_RUN) {
             int chunk = std::min(pollk, o->maxit - k);
             for (int j = 0; j < chunk; ++j, ++k) {
-                if (k > 0) k_m_dir<<<S->grid, kBlock, 0, st>>>(d, S->p32, k);
+                if (k > 0) k_m_dir<<<S->grid, kBlock, 0, st>>>(d, S->p32, k, pending);
+                pending = 1;
                 DISPATCH_WG(A, S->nchunk, k_m_spmv, v, d, S->p32);
                 k_fold_dn<<<S->gdn, kBlock, 0, st>>>(d);
-                k_m_update<<<S->grid, kBlock, 0, st>>>(d, S->p32, k);
+                k_m_update<<<S->grid, kBlock, 0, st>>>(d, k);
             }
             k_m_status<<<1, kBlock, 0, st>>>(d);
             GIP_LAUNCH_CHECK("mixed iteration");
@@ -677,7 +712,8 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
             const double rr32 = S->h_sc[S_RR], tol2 = S->h_sc[S_TOL2];
             const bool near_target = tol2 >= 0.0 && rr32 <= tol2;
             if (rr32 <= inner2 * last_rr || near_target || k >= o->maxit) {
-                k_m_fold<<<S->grid, kBlock, 0, st>>>(d);
+                k_m_fold<<<S->grid, kBlock, 0, st>>>(d, S->p32, k - 1);
+                pending = 0;
                 DISPATCH_W(A, k_m_resid, v, d);
                 k_m_coarse<<<S->grid, kBlock, 0, st>>>(d, use_coarse);
                 k_m_corr_fin<<<1, kBlock, 0, st>>>(d, k, 0);
