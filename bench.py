@@ -205,7 +205,20 @@ def main():
     alg = spmv_bytes(info["nnz"], N, NA)
     spmv_gbs = alg / (spmv_ms * 1e-3) / 1e9
     log(f"spmv fp64: {spmv_ms:.4f} ms, {spmv_gbs:.0f} GB/s algorithmic ({alg / 1e9:.3f} GB)")
-    del x, y
+    # the FP32 product of the mixed solver's inner iteration
+    x32 = dev.to_device(np.random.default_rng(2).standard_normal(NA).astype(np.float32))
+    y32 = dev.array(NA, np.float32)
+    e0, e1 = hip.Event(dev), hip.Event(dev)
+    for _ in range(3):
+        A.spmv32(x32, y32)
+    e0.record()
+    for _ in range(50):
+        A.spmv32(x32, y32)
+    e1.record()
+    spmv32_ms = e0.elapsed_ms(e1) / 50
+    alg32 = spmv32_bytes(info["nnz"], N, NA)
+    log(f"spmv fp32: {spmv32_ms:.4f} ms, {alg32 / (spmv32_ms * 1e-3) / 1e9:.0f} GB/s algorithmic ({alg32 / 1e9:.3f} GB)")
+    del x, y, x32, y32
 
     fp64 = pcg_leg(hip, dev, A, b_dev, N, NA, hip.PCG_FP64, args.steps, args.warmup, 20000, bnorm)
     mixed = pcg_leg(hip, dev, A, b_dev, N, NA, hip.PCG_MIXED, args.steps, args.warmup, 20000, bnorm)
@@ -265,6 +278,7 @@ def main():
         "pcg_fp64": fp64,
         "pcg_mixed": mixed,
         "spmv_fp64": {"ms": spmv_ms, "gbs": spmv_gbs, "algorithmic_bytes": alg},
+        "spmv_fp32": {"ms": spmv32_ms, "gbs": alg32 / (spmv32_ms * 1e-3) / 1e9, "algorithmic_bytes": alg32},
         "ns_steps": ns,
         "roofline": {"bound": "hbm", "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n, hip.FORMAT_NAMES.get(info["format"], "?")),

@@ -83,13 +83,14 @@ __global__ __launch_bounds__(kBlock) void k_stencil(EllView A, int nrows, const 
         y[out] = a * sum + b * y[out];
 }
 
+template <int W>
 __global__ __launch_bounds__(kBlock) void k_stencil32(EllView A, int nrows, const float *__restrict__ x,
                                                      float *__restrict__ y) {
     const int row = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (row >= nrows) return;
     Operand<float> xo{x};
     float sum;
-    row_dot<0, float, float>(A, A.val32, xo, row, sum);
+    row_dot<W, float, float>(A, A.val32, xo, row, sum);
     y[row] = sum;
 }
 
@@ -324,7 +325,13 @@ int spmv32_launch(gip_ctx *c, const gip_mat *A, const float *x, float *y) {
         k_ell32<<<nblocks(A->nrows), kBlock, 0, c->stream>>>(A->nrows, A->d_slice_off, A->d_ell_col, A->d_ell_val32, x, y);
     } else if (A->format == GIP_FMT_STENCIL) {
         if (!A->d_ell_val32) return set_error(GIP_EINVAL, "operator has no FP32 values");
-        k_stencil32<<<nblocks(A->nrows), kBlock, 0, c->stream>>>(view(A), A->nrows, x, y);
+        const EllView v = view(A);
+        const int g = nblocks(A->nrows);
+        switch (-A->st_w) {
+            case -7: k_stencil32<-7><<<g, kBlock, 0, c->stream>>>(v, A->nrows, x, y); break;
+            case -8: k_stencil32<-8><<<g, kBlock, 0, c->stream>>>(v, A->nrows, x, y); break;
+            default: k_stencil32<0><<<g, kBlock, 0, c->stream>>>(v, A->nrows, x, y); break;
+        }
     } else if (A->format == GIP_FMT_CSR) {
         if (!A->d_val32) return set_error(GIP_EINVAL, "operator has no FP32 values");
         k_csr32<<<nblocks(A->nrows), kBlock, 0, c->stream>>>(A->nrows, A->d_rowptr, A->d_col, A->d_val32, x, y);

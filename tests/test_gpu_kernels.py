@@ -124,6 +124,17 @@ def test_stencil_layout_bitwise_with_tails_and_empty_rows(dev, seed):
         pyoracle.spmv_ab(pyoracle.Csr(rp, cl, vl), x, want, 0.5, -2.0)
         assert np.array_equal(dy.download(), want), fp32
         assert np.array_equal(M.values(), vl)
+        if fp32:  # the FP32 product, stored order in FP32 arithmetic
+            x32, v32 = x.astype(np.float32), vl.astype(np.float32)
+            y32 = dev.array(n, np.float32)
+            M.spmv32(dev.to_device(x32), y32)
+            want32 = np.zeros(n, np.float32)
+            for r in range(n):
+                acc = np.float32(0.0)
+                for j in range(rp[r], rp[r + 1]):
+                    acc = np.float32(acc + np.float32(x32[cl[j]] * v32[j]))
+                want32[r] = acc
+            assert np.array_equal(y32.download(), want32)
 
 
 @pytest.mark.parametrize("lanes", [2, 4, 8])
