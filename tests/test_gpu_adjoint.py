@@ -1,7 +1,7 @@
 """Hand-written adjoint of the time integration (include/gip_adjoint.h)
 against central finite differences of its own forward map (dt frozen).
 
-Bar (north star): adjoint directional derivatives match central differences
+Bar: adjoint directional derivatives match central differences
 (eps = 1e-6) to 1e-6 relative on random directions in the space of the lid
 profile and the initial velocity field.  The forward map itself is checked
 against the oracle's integrator (same operators, same dt sequence)."""

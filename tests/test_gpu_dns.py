@@ -2,7 +2,7 @@
 GIP_MatrixHIP/GIP_VectorHIP/GIP_ArchHIP/GIP_NonLinearHIP, through
 include/gip_dns.h) against the oracle on identical operators.
 
-Bar (north star): per-step massr, dt, time and the u, v, w, p fields within
+Bar: per-step massr, dt, time and the u, v, w, p fields within
 1e-10 relative of the CPU path in FP64; tolerance-mode PCG iteration
 counts within +-1; the README golden lines for C100K.
 """
