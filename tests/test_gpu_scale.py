@@ -1,0 +1,85 @@
+"""The device path at the launch shapes of the large configurations, where
+the grid and partial-sum sizes differ from the small cases (the sweeping
+kernels at their 2048-workgroup cap, products of 8192+ workgroups whose
+partials are folded), against the oracle on the same inputs.
+
+Bar: FP64 PCG iterates within 1e-10 of the oracle's (only dot-product
+summation orders differ); the mixed-precision solve reaches a 1e-10 FP64
+true residual (checked with scipy's FP64 product); the integrator's
+per-step time, dt, massr and fields within 1e-10 of the oracle's.
+"""
+import numpy as np
+import pytest
+
+from gip_amd import cavity, dns, hip
+from oracle import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = hip.Device(0)
+    yield d
+    d.close()
+
+
+@pytest.fixture(scope="module")
+def lap128():
+    c = cavity.generate(128, gamma=1e-3, ops=cavity.OP_BITS["L"])
+    L = c.op("L")
+    b = np.zeros(c.n_all)
+    b[:c.n_inner] = cavity.rhs_normal(c.n_inner, 7)
+    return c, L, b
+
+
+def test_pcg_fp64_128_fixed_iterations_vs_oracle(dev, lap128):
+    c, L, b = lap128
+    n, nall = c.n_inner, c.n_all
+    x0 = np.zeros(nall)
+    A = hip.Matrix.from_csr(dev, L)
+    assert A.info()["format"] == hip.FMT_STENCIL
+    solver = hip.PCG(A, nall)
+    dx = dev.array(nall)
+    r = solver.solve(dev.to_device(b), dx, dev.to_device(x0), maxit=50)
+    want, st, it, rel = pyoracle.pcg(pyoracle.Csr.of(L), b, x0, n, 50)
+    got = dx.download()
+    print(f"[128^3 fp64] iters {r.iters} relres {r.relres:.3e} oracle {rel:.3e}", flush=True)
+    assert r.iters == it == 50 and r.status == 0
+    assert np.linalg.norm(got - want) <= 1e-10 * np.linalg.norm(want)
+    assert abs(r.relres - rel) <= 1e-8 * rel
+
+
+def test_pcg_mixed_128_true_residual(dev, lap128):
+    c, L, b = lap128
+    n, nall = c.n_inner, c.n_all
+    A = hip.Matrix.from_csr(dev, L, fp32=True)
+    solver = hip.PCG(A, nall)
+    dx = dev.array(nall)
+    r = solver.solve(dev.to_device(b), dx, dev.to_device(np.zeros(nall)), maxit=20000, tol=1e-10,
+                     precision=hip.PCG_MIXED, poll=32)
+    x = dx.download()
+    Ls = L.to_scipy(nall)
+    res = b[:n] - (Ls @ x)[:n]
+    true_rel = np.linalg.norm(res) / np.linalg.norm(b[:n])
+    print(f"[128^3 mixed] iters {r.iters} corrections {r.corrections} true {true_rel:.3e}", flush=True)
+    assert r.status == 0 and true_rel <= 1.01e-10
+    assert np.array_equal(x[n:], np.zeros(nall - n))  # ghost entries carried over from x0
+
+
+def test_dns_96_reference_sequence_vs_oracle(tmp_path):
+    cavity.generate(96, gamma=1e-3).write(tmp_path, "cav96")
+    sim = dns.Simulation(root=tmp_path, name="cav96", gamma=1e-3, pcg_mode="fused")
+    orc = pyoracle.Dns(tmp_path, "cav96", gamma=1e-3)
+    for k in range(3):
+        g, w = sim.step(), orc.step()
+        print(f"[96^3 step {k + 1}] gpu massr {g['massr']:.9e} dt {g['dt']:.9e} | cpu massr {w['massr']:.9e}",
+              flush=True)
+        assert g["pcg_iters"] == w["pcg_iters"] == 100
+        for key in ("time", "dt", "massr"):
+            assert abs(g[key] - w[key]) <= 1e-10 * abs(w[key]), (k, key, g[key], w[key])
+    for f in ("u", "v", "w", "p", "mf"):
+        a, r = sim.field(f), orc.field(f)
+        err = np.linalg.norm(a - r) / np.linalg.norm(r)
+        print(f"[96^3 field {f}] rel L2 {err:.3e}", flush=True)
+        assert err <= 1e-10, (f, err)
