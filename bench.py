@@ -20,6 +20,10 @@ Other suites (one JSON line each, not the default):
                       [100, 3200]) advanced 50 steps as one batch vs one by one
     --suite adjoint   64^3, Re=400, 200 steps, checkpoints every 20: forward +
                       backward time and 16 central finite-difference checks
+    --suite reference the reference's own integrator (GIP_DNS over the Multicore
+                      specification, one rank, one core, -O2) beside the GPU
+                      integrator on the same cases (C100K and generated 46^3 /
+                      96^3 cavities, fixed 100 PCG iterations): ms per step
     --suite clustered 128^3 tanh-clustered mesh (beta 1.5), Re=1000, 100 steps,
                       PCG to 1e-10: steps/s FP64 vs mixed, the L SpMV rate and
                       the FP64/mixed PCG on a zero-mean N(0,1) right-hand side
@@ -164,7 +168,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=60)
     ap.add_argument("--skip-ns", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--suite", choices=("pcg", "batch", "adjoint", "clustered"), default="pcg")
+    ap.add_argument("--suite", choices=("pcg", "batch", "adjoint", "clustered", "reference"), default="pcg")
     ap.add_argument("--probe", choices=("spmv",), default=None)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -179,6 +183,8 @@ def main():
         return adjoint_suite(args)
     if args.suite == "clustered":
         return clustered_suite(args)
+    if args.suite == "reference":
+        return reference_suite(args)
 
     from gip_amd import cavity, dns, hip
 
@@ -414,6 +420,55 @@ def clustered_suite(args):
         "pcg_fp64": fp64, "pcg_mixed": mixed,
         "data": "synthetic: generated tanh-clustered 128^3 cavity, lid u=1, from rest; seeded N(0,1) zero-mean RHS"}),
         flush=True)
+
+
+def reference_suite(args):
+    """The reference's GIP_DNS::integrate over its Multicore specification
+    (oracle/_ref/gip_ref_harness dns: the reference sources compiled -O2,
+    one MPI rank, one core, timed with MPI_Wtime around integrate()) and the
+    GPU integrator (the HIP specification through include/gip_dns.h, with
+    the reference's PCG sequence and with the device-resident PCG), on the
+    same one-partition cases with the reference's fixed 100 PCG iterations."""
+    import tempfile
+    from gip_amd import cavity, dns
+    exe = ROOT / "oracle" / "_ref" / "gip_ref_harness"
+    if not exe.exists():
+        log("reference harness absent (built where the reference sources are present)")
+        return
+    tmp = pathlib.Path(tempfile.mkdtemp(prefix="gip_ref_"))
+    cases = []
+    if (ROOT / "data" / "InputFiles" / "Topos" / "C100K" / "C100K_1p").exists():
+        cases.append(("C100K", ROOT / "data", "C100K", 1e-3, 10))
+    for n, g, steps in ((46, 1e-2, 10), (96, 1e-3, 3)):
+        cavity.generate(n, gamma=g).write(tmp, f"cav{n}")
+        cases.append((f"cavity {n}^3", tmp, f"cav{n}", g, steps))
+    rows = []
+    for label, root, name, gamma, steps in cases:
+        out = subprocess.run([str(exe), "dns", name, str(steps), str(gamma), "100"], cwd=root, capture_output=True,
+                             text=True, timeout=900, env=dict(os.environ, OMP_NUM_THREADS="1"))
+        cpu_ms = None
+        for line in out.stderr.splitlines():
+            if line.startswith("integrate_ms"):
+                cpu_ms = float(line.split()[1])
+        res = {"case": label, "steps": steps, "cpu_ms_per_step": cpu_ms / steps if cpu_ms else None}
+        for mode in ("reference", "fused"):
+            sim = dns.Simulation(root=root, name=name, gamma=gamma, pcg_mode=mode)
+            sim.step()  # first step: uploads and lazy set-up out of the timed region
+            sim.sync()
+            t0 = time.perf_counter()
+            for _ in range(steps - 1):
+                sim.step()
+            sim.sync()
+            res[f"gpu_{mode}_ms_per_step"] = 1e3 * (time.perf_counter() - t0) / max(1, steps - 1)
+            sim.close()
+        log(f"{label}: cpu {res['cpu_ms_per_step']} ms/step, gpu reference-sequence "
+            f"{res['gpu_reference_ms_per_step']:.2f}, device PCG {res['gpu_fused_ms_per_step']:.2f} ms/step")
+        rows.append(res)
+    print(json.dumps({
+        "metric": "integrator ms per step, reference CPU path (1 core) vs GPU, fixed 100 PCG iterations",
+        "value": rows[-1]["gpu_fused_ms_per_step"], "unit": "ms/step", "n_gpus": 1, "higher_is_better": False,
+        "cases": rows, "cpu": {"cores": 1, "kind": "reference", "host": os.uname().nodename},
+        "data": "C100K (the reference's shipped case, merged to one partition) and generated cavities"}), flush=True)
 
 
 def adjoint_suite(args):
