@@ -104,9 +104,19 @@ def build_ref():
 
 
 def build_data():
-    """One-partition C100K (merged from the reference's four partitions) under data/."""
+    """C100K under data/: the reference's four-partition files (the ones its
+    driver reads with four ranks, for timing the reference CPU path) and the
+    one-partition case merged from them (for the GPU)."""
     if not (REFERENCE / "InputFiles").exists():
         return False
+    src = REFERENCE / "InputFiles"
+    for d in src.rglob("C100K_4p"):
+        if d.is_dir():
+            dst = DATA / d.relative_to(REFERENCE)
+            dst.mkdir(parents=True, exist_ok=True)
+            for f in d.glob("*_4p-*"):
+                if not (dst / f.name).exists():
+                    shutil.copyfile(f, dst / f.name)
     marker = DATA / "InputFiles" / "Topos" / "C100K" / "C100K_1p" / "topoColsC100K_1p-0.topo"
     if marker.exists():
         return True
