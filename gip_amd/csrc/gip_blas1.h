@@ -109,6 +109,35 @@ __device__ inline bool last_block_reduce2(double v0, double v1, double *partials
     return true;
 }
 
+// M sums of partials at once, for partial arrays of kMaxRedBlocks doubles
+// whose entries past their producer's grid hold zeros: every slot of every
+// array is loaded unconditionally and all loads are issued before the first
+// is used, so the M sums cost one memory round trip (a load under a bounds
+// test would be turned into a branch with a wait at its join).  Adding the
+// zero slots gives bitwise the total sum_partials<RED_SUM> gives over the
+// first n entries.
+template <int M>
+__device__ inline void sum_padded(const double *const (&part)[M], double (&out)[M]) {
+    __shared__ double s_all[M];
+    constexpr int K = kMaxRedBlocks / kBlock;
+    double v[M][K];
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[m][k] = part[m][threadIdx.x + k * kBlock];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc = acc + v[m][k];
+        acc = block_reduce<RED_SUM>(acc);
+        if (threadIdx.x == 0) s_all[m] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < M; ++m) out[m] = s_all[m];
+}
+
 // Combine n partials in index order (thread t takes t, t+256, ..., then the
 // fixed tree): the same value in every workgroup that calls it.  Used in
 // kernel prologues to read the previous kernel's per-workgroup partials

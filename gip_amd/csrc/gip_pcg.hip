@@ -9,7 +9,7 @@
 //       applied by a flush kernel)
 //   K1  Ap = A p, partials of p.Ap                             (:87-91)
 //       (one row per thread over the whole matrix, one partial per 256-row
-//       chunk, folded to one partial per 256 chunks by a small kernel)
+//       chunk; beyond 2048 chunks a small kernel folds each 256 of them)
 //   K2  dn = sum of the p.Ap partials, alpha = rho_k / dn;
 //       r += (-alpha) Ap ; s = Dinv r ; partials of r.s, r.r
 // Cross-workgroup communication is fence-free: every workgroup stores its
@@ -19,13 +19,17 @@
 // stores before the loads; no atomics, no device-scope fences (a fence per
 // workgroup costs an L2 write-back on every XCD).  Scalars that must survive
 // a kernel (rho of the previous iteration, the state word) are written by
-// workgroup 0 only, into slots nobody reads in the same kernel (rho uses two
-// slots by iteration parity).
+// workgroup 0 only: rho into the slot of its iteration's parity, which no
+// workgroup of the same kernel reads; the state word as described at
+// running().
 // The direction update has its own streaming kernel so that the product
 // gathers one vector (p) per neighbour; forming p inside the product would
-// gather s and the old p for every neighbour.  Every kernel first reads the
+// gather s and the old p for every neighbour.  Every kernel reads the
 // solver's state word and returns when the solve has finished, so the host
-// queues many iterations and polls the state only now and then.
+// queues many iterations and polls the state only now and then.  The
+// streaming kernels request the state word, the partials and their first
+// row's operands together, so that a small operator's kernel costs one
+// memory round trip before it stores.
 //
 // MIXED: the same three kernels in FP32 (operator values, vectors and SpMV;
 // dot products accumulate in FP64) on the correction equation, inside an
@@ -62,16 +66,18 @@ enum {
     S_ZAZ = 8,      // 1.A1 over the solved rows (MIXED coarse step)
     S_COUNT = 16
 };
-// State word: 0 running, 1 converged, 2 breakdown.
-enum { ST_RUN = 0, ST_CONV = 1, ST_BREAK = 2 };
+// State word: ST_RUN, ST_BREAK, or ST_CONV; the direction kernel of
+// iteration k writes ST_CONV | k << 2, so readers look at the low two bits.
+enum { ST_RUN = 0, ST_CONV = 1, ST_BREAK = 2, ST_MASK = 3 };
 // Partial-sum arrays, kMaxRedBlocks doubles each.
 enum { P_DN = 0, P_RS = 1, P_RR = 2, P_BB = 3, P_RS2 = 4, P_RR2 = 5, P_SUMR = 6, P_COUNT = 7 };
 
 struct PcgDev {
     int n, G;      // solved rows, workgroups of the sweeping kernels (<= kMaxRedBlocks)
     int nchunk;    // 256-row chunks: workgroups of the product kernels
-    int Gdn;       // p.Ap partials after the fold (<= kMaxRedBlocks)
-    double *chunk; // nchunk per-chunk partials of the product
+    int Gdn;       // p.Ap partials the next kernels read (<= kMaxRedBlocks)
+    double *chunk; // nchunk per-chunk partials of the product (the P_DN
+                   // partials themselves when no fold is needed)
     double *x, *r, *s, *Ap;
     const double *b, *dinv;
     float *r32, *s32, *Ap32, *xc32;
@@ -79,15 +85,29 @@ struct PcgDev {
     const double *az;  // A 1 on the solved rows (MIXED)
     double *sc;        // S_COUNT scalars
     int *state, *iters;
-    double *part;  // P_COUNT x kMaxRedBlocks partials
+    double *part;  // P_COUNT x kMaxRedBlocks partials; each array has one
+                   // producer grid size and stays zero past it
     int precond;
     __device__ double *P(int k) const { return part + k * kMaxRedBlocks; }
 };
 
-// The state word is written by earlier kernels only (a workgroup that sets
-// it in the current kernel returns with every other workgroup reaching the
-// same decision), so a plain load, ordered by the kernel boundary, suffices.
+// A kernel reads the state word once, with a plain load: words written by
+// earlier kernels are ordered by the kernel boundary.  A workgroup may or
+// may not see the word workgroup 0 of its own kernel writes, and each
+// writer makes both outcomes give the same result: the direction kernel
+// stamps a convergence with its iteration, and a workgroup that reads its
+// own kernel's stamp takes the convergence path every workgroup computes;
+// a breakdown leaves the iterates undefined, so skipping work is harmless.
 __device__ inline bool running(const PcgDev &d) { return *d.state == ST_RUN; }
+// A word read at the top of a kernel and used only after its prologue: an
+// atomic load (workgroup scope, relaxed: a plain load instruction) is an
+// ordered memory reference, so the compiler issues it where it is written,
+// together with the other loads, instead of sinking it to its first use.
+__device__ inline int early_load(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
+__device__ inline double early_load(const double *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ inline int conv_stamp(int k) { return ST_CONV | (k << 2); }
 __device__ inline bool leader() { return blockIdx.x == 0 && threadIdx.x == 0; }
 __device__ inline double rho_slot(const PcgDev &d, int k) { return d.sc[S_RHO0 + (k & 1)]; }
 
@@ -170,29 +190,76 @@ __device__ inline void x_update(const PcgDev &d, const double *__restrict__ p, d
     }
 }
 
+// Kernel prologues.  SMALL (operators of at most 2^19 rows, where the grid
+// gives each thread at most one row and a kernel lasts a few microseconds):
+// the state word, the scalars, the partials and the first row's operands
+// are requested together, so the kernel spends one memory round trip before
+// its first store.  Otherwise (long kernels, where that latency is hidden
+// by the sweep) the state word is tested first, with a scalar load, and the
+// partial arrays are summed one at a time, which keeps the register count,
+// and with it the occupancy of the sweep, where it was.
+
 // K0 (k >= 1): rho_k, the convergence test, x += alpha_{k-1} p,
 // beta, p = 1.0 s + beta p
+template <bool SMALL>
 __global__ __launch_bounds__(kBlock) void k_iter_dir(PcgDev d, double *__restrict__ p, int k) {
-    if (!running(d)) return;
-    const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);  // :107
-    const double alpha = alpha_of(d, k - 1);
-    const double tol2 = d.sc[S_TOL2];
-    if (tol2 >= 0.0) {
-        const double rr = sum_partials<RED_SUM>(d.P(P_RR), d.G);
-        if (rr <= tol2) {  // every workgroup reaches the same decision
-            x_update(d, p, alpha);
-            if (leader()) *d.state = ST_CONV;
-            return;
-        }
-    }
-    const double beta = rho / rho_slot(d, k - 1);  // :109
-    if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
     double *__restrict__ x = d.x;
     const double *__restrict__ s = d.s;
-    FOR_ROWS(d.n, row) {
-        const double xo = x[row], si = s[row], po = p[row];
+    if constexpr (!SMALL) {
+        const int st = *d.state;
+        if (st != ST_RUN && st != conv_stamp(k)) return;
+        const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);  // :107
+        const double alpha = alpha_of(d, k - 1);
+        const double tol2 = d.sc[S_TOL2];
+        if (tol2 >= 0.0) {
+            const double rr = sum_partials<RED_SUM>(d.P(P_RR), d.G);
+            if (rr <= tol2) {  // every workgroup reaches the same decision
+                x_update(d, p, alpha);
+                if (leader()) *d.state = conv_stamp(k);
+                return;
+            }
+        }
+        const double beta = rho / rho_slot(d, k - 1);  // :109
+        if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
+        FOR_ROWS(d.n, row) {
+            const double xo = x[row], si = s[row], po = p[row];
+            x[row] = xo + alpha * po;       // :95 of iteration k-1
+            p[row] = 1.0 * si + beta * po;  // :113
+        }
+        return;
+    }
+    ThreadRows t(d.n);
+    double xo = 0.0, si = 0.0, po = 0.0;
+    const int st = early_load(d.state);
+    if (t.valid()) {
+        xo = x[t.row];
+        si = s[t.row];
+        po = p[t.row];
+    }
+    const double tol2 = early_load(d.sc + S_TOL2);
+    const double rho_prev = early_load(d.sc + S_RHO0 + ((k - 1) & 1));
+    double sums[3];
+    sum_padded<3>({d.P(P_RS), d.P(P_DN), d.P(P_RR)}, sums);
+    if (st != ST_RUN && st != conv_stamp(k)) return;
+    const double rho = sums[0];
+    const double alpha = rho_prev / sums[1];  // alpha_{k-1}, bitwise as alpha_of gives it
+    if (tol2 >= 0.0 && sums[2] <= tol2) {
+        x_update(d, p, alpha);
+        if (leader()) *d.state = conv_stamp(k);
+        return;
+    }
+    const double beta = rho / rho_prev;
+    if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
+    while (t.valid()) {
+        const int row = t.row;
         x[row] = xo + alpha * po;       // :95 of iteration k-1
         p[row] = 1.0 * si + beta * po;  // :113
+        t.next();
+        if (t.valid()) {
+            xo = x[t.row];
+            si = s[t.row];
+            po = p[t.row];
+        }
     }
 }
 
@@ -204,26 +271,58 @@ __global__ __launch_bounds__(kBlock) void k_x_flush(PcgDev d, const double *__re
 }
 
 // K1: Ap = A p (empty rows keep Ap), one partial of p.Ap per 256-row chunk;
-// one row per thread, workgroups in XCD-contiguous order
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_iter_spmv(EllView A, PcgDev d, const double *__restrict__ p) {
-    if (!running(d)) return;
+// one row per thread, workgroups in XCD-contiguous order.  SMALL: the state
+// word is tested after the row's loads instead of before them, taking its
+// latency off the kernel's critical path; a finished solve then still reads
+// the operator, but stores nothing.
+template <int W, bool SMALL>
+__device__ inline void iter_spmv(const EllView &A, const PcgDev &d, const double *__restrict__ p) {
+    if constexpr (!SMALL) {
+        if (!running(d)) return;
+        const int c = xcd_block(blockIdx.x, gridDim.x);
+        const int row = c * kBlock + threadIdx.x;
+        double acc = 0.0;
+        if (row < d.n) {
+            Operand<double> po{p};
+            double *__restrict__ Ap = d.Ap;
+            const double pr = p[row];
+            double sum;
+            if (row_dot<W, double, double>(A, A.val, po, row, sum))
+                Ap[row] = sum;
+            else
+                sum = Ap[row];  // empty row: Ap keeps its value
+            acc = pr * sum;
+        }
+        acc = block_reduce<RED_SUM>(acc);
+        if (threadIdx.x == 0) d.chunk[c] = acc;
+        return;
+    }
+    const int st = early_load(d.state);
     const int c = xcd_block(blockIdx.x, gridDim.x);
     const int row = c * kBlock + threadIdx.x;
-    double acc = 0.0;
+    double pr = 0.0, sum = 0.0;
+    bool any = false;
     if (row < d.n) {
         Operand<double> po{p};
-        double *__restrict__ Ap = d.Ap;
-        const double pr = p[row];
-        double sum;
-        if (row_dot<W, double, double>(A, A.val, po, row, sum))
-            Ap[row] = sum;
-        else
-            sum = Ap[row];  // empty row: Ap keeps its value
-        acc = pr * sum;
+        pr = p[row];
+        any = row_dot<W, double, double>(A, A.val, po, row, sum);
+        if (!any) sum = d.Ap[row];  // empty row: Ap keeps its value
     }
+    if (st != ST_RUN) return;
+    if (any) d.Ap[row] = sum;
+    double acc = row < d.n ? pr * sum : 0.0;
     acc = block_reduce<RED_SUM>(acc);
     if (threadIdx.x == 0) d.chunk[c] = acc;
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_iter_spmv(EllView A, PcgDev d, const double *__restrict__ p) {
+    iter_spmv<W, false>(A, d, p);
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_iter_spmv_small(EllView A, PcgDev d, const double *__restrict__ p) {
+    iter_spmv<W, true>(A, d, p);
 }
 
 // the per-chunk partials, folded in index order into Gdn partials of P_DN
@@ -235,35 +334,81 @@ __global__ __launch_bounds__(kBlock) void k_fold_dn(PcgDev d) {
     if (threadIdx.x == 0) d.P(P_DN)[blockIdx.x] = v;
 }
 
+// K2: alpha_k, r += (-alpha) Ap, s = Dinv r, partials of r.s and r.r; each
+// row's operands are loaded together (one round trip per row)
+template <bool SMALL>
 __global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, int k) {
-    if (!running(d)) return;
-    const double dn = sum_partials<RED_SUM>(d.P(P_DN), d.Gdn);
-    const double alpha = rho_slot(d, k) / dn;  // :93
-    if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) {
-        if (leader()) *d.state = ST_BREAK;
-        return;
-    }
-    const double malpha = -1.0 * alpha;
     double *__restrict__ r = d.r, *__restrict__ s = d.s;
     const double *__restrict__ Ap = d.Ap, *__restrict__ dinv = d.dinv;
     const bool precond = d.precond != 0;
     double rs = 0.0, rr = 0.0;
-    FOR_ROWS(d.n, row) {
-        // every operand first, so the row costs one memory round trip
-        const double ro = r[row], apr = Ap[row];
-        const double di = precond ? dinv[row] : 0.0;
-        const double ri = ro + malpha * apr;  // :97
-        r[row] = ri;
-        double si;
-        if (precond) {  // :99-105
-            si = 0.0;
-            si = si + ri * di;
-        } else {
-            si = ri;
+    if constexpr (!SMALL) {
+        if (!running(d)) return;
+        const double dn = sum_partials<RED_SUM>(d.P(P_DN), d.Gdn);
+        const double alpha = rho_slot(d, k) / dn;  // :93
+        if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) {
+            if (leader()) *d.state = ST_BREAK;
+            return;
         }
-        s[row] = si;
-        rs = rs + ri * si;
-        rr = rr + ri * ri;
+        const double malpha = -1.0 * alpha;
+        FOR_ROWS(d.n, row) {
+            // every operand first, so the row costs one memory round trip
+            const double ro = r[row], apr = Ap[row];
+            const double di = precond ? dinv[row] : 0.0;
+            const double ri = ro + malpha * apr;  // :97
+            r[row] = ri;
+            double si;
+            if (precond) {  // :99-105
+                si = 0.0;
+                si = si + ri * di;
+            } else {
+                si = ri;
+            }
+            s[row] = si;
+            rs = rs + ri * si;
+            rr = rr + ri * ri;
+        }
+    } else {
+        ThreadRows t(d.n);
+        double ro = 0.0, apr = 0.0, di = 0.0;
+        const int st = early_load(d.state);
+        if (t.valid()) {
+            ro = r[t.row];
+            apr = Ap[t.row];
+            if (precond) di = dinv[t.row];
+        }
+        const double rho = early_load(d.sc + S_RHO0 + (k & 1));
+        double sums[1];
+        sum_padded<1>({d.P(P_DN)}, sums);
+        const double dn = sums[0];
+        if (st != ST_RUN) return;
+        const double alpha = rho / dn;
+        if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) {
+            if (leader()) *d.state = ST_BREAK;
+            return;
+        }
+        const double malpha = -1.0 * alpha;
+        while (t.valid()) {
+            const int row = t.row;
+            const double ri = ro + malpha * apr;  // :97
+            r[row] = ri;
+            double si;
+            if (precond) {  // :99-105
+                si = 0.0;
+                si = si + ri * di;
+            } else {
+                si = ri;
+            }
+            s[row] = si;
+            rs = rs + ri * si;
+            rr = rr + ri * ri;
+            t.next();
+            if (t.valid()) {
+                ro = r[t.row];
+                apr = Ap[t.row];
+                if (precond) di = dinv[t.row];
+            }
+        }
     }
     rs = block_reduce<RED_SUM>(rs);
     if (threadIdx.x == 0) d.P(P_RS)[blockIdx.x] = rs;
@@ -287,33 +432,84 @@ __global__ __launch_bounds__(kBlock) void k_fin(PcgDev d) {
 // K0 (k >= 1): rho_k, the correction update xc += alpha_{k-1} p of the
 // previous iteration (`pending`: not yet folded by a defect correction),
 // beta (0 after a restart: the previous rho slot is +inf), p32 = s32 + beta p32
+template <bool SMALL>
 __global__ __launch_bounds__(kBlock) void k_m_dir(PcgDev d, float *__restrict__ p, int k, int pending) {
-    if (!running(d)) return;
-    const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);
-    const float a32 = pending ? (float)alpha_of(d, k - 1) : 0.0f;
-    const float beta = (float)(rho / rho_slot(d, k - 1));
-    if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
     float *__restrict__ xc = d.xc32;
     const float *__restrict__ s = d.s32;
-    FOR_ROWS(d.n, row) {
-        const float xo = xc[row], si = s[row], po = p[row];
+    if constexpr (!SMALL) {
+        if (!running(d)) return;
+        const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);
+        const float a32 = pending ? (float)alpha_of(d, k - 1) : 0.0f;
+        const float beta = (float)(rho / rho_slot(d, k - 1));
+        if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
+        FOR_ROWS(d.n, row) {
+            const float xo = xc[row], si = s[row], po = p[row];
+            if (pending) xc[row] = xo + a32 * po;
+            p[row] = si + beta * po;
+        }
+        return;
+    }
+    ThreadRows t(d.n);
+    float xo = 0.0f, si = 0.0f, po = 0.0f;
+    const int st = early_load(d.state);
+    if (t.valid()) {
+        xo = xc[t.row];
+        si = s[t.row];
+        po = p[t.row];
+    }
+    const double rho_prev = early_load(d.sc + S_RHO0 + ((k - 1) & 1));
+    double sums[2];
+    sum_padded<2>({d.P(P_RS), d.P(P_DN)}, sums);
+    if (st != ST_RUN) return;
+    const double rho = sums[0];
+    const float a32 = pending ? (float)(rho_prev / sums[1]) : 0.0f;  // alpha_{k-1}, as alpha_of gives it
+    const float beta = (float)(rho / rho_prev);
+    if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
+    while (t.valid()) {
+        const int row = t.row;
         if (pending) xc[row] = xo + a32 * po;
         p[row] = si + beta * po;
+        t.next();
+        if (t.valid()) {
+            xo = xc[t.row];
+            si = s[t.row];
+            po = p[t.row];
+        }
     }
 }
 
-// K1: Ap32 = A32 p32, one partial of p.Ap (FP64 accumulation) per chunk
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_m_spmv(EllView A, PcgDev d, const float *__restrict__ p) {
-    if (!running(d)) return;
+// K1 in FP32 (SMALL as in iter_spmv)
+template <int W, bool SMALL>
+__device__ inline void m_spmv(const EllView &A, const PcgDev &d, const float *__restrict__ p) {
+    if constexpr (!SMALL) {
+        if (!running(d)) return;
+        const int c = xcd_block(blockIdx.x, gridDim.x);
+        const int row = c * kBlock + threadIdx.x;
+        double acc = 0.0;
+        if (row < d.n) {
+            Operand<float> po{p};
+            const float pr = p[row];
+            float sum;
+            row_dot<W, float, float>(A, A.val32, po, row, sum);
+            d.Ap32[row] = sum;
+            acc = (double)pr * (double)sum;
+        }
+        acc = block_reduce<RED_SUM>(acc);
+        if (threadIdx.x == 0) d.chunk[c] = acc;
+        return;
+    }
+    const int st = early_load(d.state);
     const int c = xcd_block(blockIdx.x, gridDim.x);
     const int row = c * kBlock + threadIdx.x;
-    double acc = 0.0;
+    float pr = 0.0f, sum = 0.0f;
     if (row < d.n) {
         Operand<float> po{p};
-        const float pr = p[row];
-        float sum;
+        pr = p[row];
         row_dot<W, float, float>(A, A.val32, po, row, sum);
+    }
+    if (st != ST_RUN) return;
+    double acc = 0.0;
+    if (row < d.n) {
         d.Ap32[row] = sum;
         acc = (double)pr * (double)sum;
     }
@@ -321,29 +517,77 @@ __global__ __launch_bounds__(kBlock) void k_m_spmv(EllView A, PcgDev d, const fl
     if (threadIdx.x == 0) d.chunk[c] = acc;
 }
 
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_m_spmv(EllView A, PcgDev d, const float *__restrict__ p) {
+    m_spmv<W, false>(A, d, p);
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_m_spmv_small(EllView A, PcgDev d, const float *__restrict__ p) {
+    m_spmv<W, true>(A, d, p);
+}
+
+template <bool SMALL>
 __global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, int k) {
-    if (!running(d)) return;
-    const double dn = sum_partials<RED_SUM>(d.P(P_DN), d.Gdn);
-    const double alpha = rho_slot(d, k) / dn;
-    if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) {
-        if (leader()) *d.state = ST_BREAK;
-        return;
-    }
-    const float a32 = (float)alpha;
     float *__restrict__ r = d.r32, *__restrict__ s = d.s32;
     const float *__restrict__ Ap = d.Ap32, *__restrict__ dinv = d.dinv32;
     const bool precond = d.precond != 0;
     double rs = 0.0, rr = 0.0;
-    FOR_ROWS(d.n, row) {
-        // every operand first, so the row costs one memory round trip
-        const float ro = r[row], apr = Ap[row];
-        const float di = precond ? dinv[row] : 0.0f;
-        const float ri = ro - a32 * apr;
-        r[row] = ri;
-        const float si = precond ? ri * di : ri;
-        s[row] = si;
-        rs = rs + (double)ri * (double)si;
-        rr = rr + (double)ri * (double)ri;
+    if constexpr (!SMALL) {
+        if (!running(d)) return;
+        const double dn = sum_partials<RED_SUM>(d.P(P_DN), d.Gdn);
+        const double alpha = rho_slot(d, k) / dn;
+        if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) {
+            if (leader()) *d.state = ST_BREAK;
+            return;
+        }
+        const float a32 = (float)alpha;
+        FOR_ROWS(d.n, row) {
+            // every operand first, so the row costs one memory round trip
+            const float ro = r[row], apr = Ap[row];
+            const float di = precond ? dinv[row] : 0.0f;
+            const float ri = ro - a32 * apr;
+            r[row] = ri;
+            const float si = precond ? ri * di : ri;
+            s[row] = si;
+            rs = rs + (double)ri * (double)si;
+            rr = rr + (double)ri * (double)ri;
+        }
+    } else {
+        ThreadRows t(d.n);
+        float ro = 0.0f, apr = 0.0f, di = 0.0f;
+        const int st = early_load(d.state);
+        if (t.valid()) {
+            ro = r[t.row];
+            apr = Ap[t.row];
+            if (precond) di = dinv[t.row];
+        }
+        const double rho = early_load(d.sc + S_RHO0 + (k & 1));
+        double sums[1];
+        sum_padded<1>({d.P(P_DN)}, sums);
+        const double dn = sums[0];
+        if (st != ST_RUN) return;
+        const double alpha = rho / dn;
+        if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) {
+            if (leader()) *d.state = ST_BREAK;
+            return;
+        }
+        const float a32 = (float)alpha;
+        while (t.valid()) {
+            const int row = t.row;
+            const float ri = ro - a32 * apr;
+            r[row] = ri;
+            const float si = precond ? ri * di : ri;
+            s[row] = si;
+            rs = rs + (double)ri * (double)si;
+            rr = rr + (double)ri * (double)ri;
+            t.next();
+            if (t.valid()) {
+                ro = r[t.row];
+                apr = Ap[t.row];
+                if (precond) di = dinv[t.row];
+            }
+        }
     }
     rs = block_reduce<RED_SUM>(rs);
     if (threadIdx.x == 0) d.P(P_RS)[blockIdx.x] = rs;
@@ -471,6 +715,7 @@ struct gip_pcg {
     double *az;  // A 1 (MIXED coarse step)
     double *sc, *part, *chunk;
     int nchunk, gdn;
+    bool fold;  // nchunk > kMaxRedBlocks: the chunk partials need the fold kernel
     int *state;    // state, iters
     int *h_state;  // pinned copy
     double *h_sc;  // pinned scalars
@@ -507,7 +752,7 @@ static PcgDev make_dev(gip_pcg *S, const double *b, double *x, int precond) {
     d.G = S->grid;
     d.nchunk = S->nchunk;
     d.Gdn = S->gdn;
-    d.chunk = S->chunk;
+    d.chunk = S->fold ? S->chunk : S->part + P_DN * kMaxRedBlocks;
     d.x = x;
     d.r = S->r;
     d.s = S->s;
@@ -543,7 +788,11 @@ int gip_pcg_create(gip_ctx *c, const gip_mat *A, int32_t n_all, gip_pcg **out) {
     S->n_all = n_all;
     S->grid = pcg_grid(A->nrows);
     S->nchunk = std::max(1, (A->nrows + kBlock - 1) / kBlock);
-    S->gdn = (S->nchunk + kBlock - 1) / kBlock;
+    // up to kMaxRedBlocks chunks (2^19 rows) the next kernels read the
+    // per-chunk partials directly; beyond, a fold kernel sums each run of
+    // 256 chunks first
+    S->fold = S->nchunk > kMaxRedBlocks;
+    S->gdn = S->fold ? (S->nchunk + kBlock - 1) / kBlock : S->nchunk;
     if (S->gdn > kMaxRedBlocks) {
         delete S;
         return set_error(GIP_EINVAL, "gip_pcg_create: more rows than the product partials cover (2^27)");
@@ -561,11 +810,13 @@ int gip_pcg_create(gip_ctx *c, const gip_mat *A, int32_t n_all, gip_pcg **out) {
     dalloc((void **)&S->dinv, n * 8);
     dalloc((void **)&S->sc, S_COUNT * 8);
     dalloc((void **)&S->part, (size_t)P_COUNT * kMaxRedBlocks * 8);
-    dalloc((void **)&S->chunk, (size_t)S->nchunk * 8);
+    if (S->fold) dalloc((void **)&S->chunk, (size_t)S->nchunk * 8);
     dalloc((void **)&S->state, 2 * sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_state, 2 * sizeof(int), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_sc, S_COUNT * 8, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMemsetAsync(S->sc, 0, S_COUNT * 8, c->stream);
+    // partial slots past each producer's grid stay zero for sum_padded
+    if (e == hipSuccess) e = hipMemsetAsync(S->part, 0, (size_t)P_COUNT * kMaxRedBlocks * 8, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(S->state, 0, 2 * sizeof(int), c->stream);
     if (e != hipSuccess) {
         int rc = hip_fail(e, "gip_pcg_create");
@@ -673,10 +924,16 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
         while (k < o->maxit && S->h_state[0] == ST_RUN) {
             int chunk = std::min(pollk, o->maxit - k);
             for (int j = 0; j < chunk; ++j, ++k) {
-                if (k > 0) k_iter_dir<<<S->grid, kBlock, 0, st>>>(d, S->p, k);
-                DISPATCH_WG(A, S->nchunk, k_iter_spmv, v, d, S->p);
-                k_fold_dn<<<S->gdn, kBlock, 0, st>>>(d);
-                k_iter_update<<<S->grid, kBlock, 0, st>>>(d, k);
+                if (S->fold) {
+                    if (k > 0) k_iter_dir<false><<<S->grid, kBlock, 0, st>>>(d, S->p, k);
+                    DISPATCH_WG(A, S->nchunk, k_iter_spmv, v, d, S->p);
+                    k_fold_dn<<<S->gdn, kBlock, 0, st>>>(d);
+                    k_iter_update<false><<<S->grid, kBlock, 0, st>>>(d, k);
+                } else {
+                    if (k > 0) k_iter_dir<true><<<S->grid, kBlock, 0, st>>>(d, S->p, k);
+                    DISPATCH_WG(A, S->nchunk, k_iter_spmv_small, v, d, S->p);
+                    k_iter_update<true><<<S->grid, kBlock, 0, st>>>(d, k);
+                }
             }
             GIP_LAUNCH_CHECK("pcg iteration");
             if ((rc = poll(S, false))) return rc;
@@ -699,11 +956,17 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
         while (k < o->maxit && S->h_state[0] == ST_RUN) {
             int chunk = std::min(pollk, o->maxit - k);
             for (int j = 0; j < chunk; ++j, ++k) {
-                if (k > 0) k_m_dir<<<S->grid, kBlock, 0, st>>>(d, S->p32, k, pending);
+                if (S->fold) {
+                    if (k > 0) k_m_dir<false><<<S->grid, kBlock, 0, st>>>(d, S->p32, k, pending);
+                    DISPATCH_WG(A, S->nchunk, k_m_spmv, v, d, S->p32);
+                    k_fold_dn<<<S->gdn, kBlock, 0, st>>>(d);
+                    k_m_update<false><<<S->grid, kBlock, 0, st>>>(d, k);
+                } else {
+                    if (k > 0) k_m_dir<true><<<S->grid, kBlock, 0, st>>>(d, S->p32, k, pending);
+                    DISPATCH_WG(A, S->nchunk, k_m_spmv_small, v, d, S->p32);
+                    k_m_update<true><<<S->grid, kBlock, 0, st>>>(d, k);
+                }
                 pending = 1;
-                DISPATCH_WG(A, S->nchunk, k_m_spmv, v, d, S->p32);
-                k_fold_dn<<<S->gdn, kBlock, 0, st>>>(d);
-                k_m_update<<<S->grid, kBlock, 0, st>>>(d, k);
             }
             k_m_status<<<1, kBlock, 0, st>>>(d);
             GIP_LAUNCH_CHECK("mixed iteration");
@@ -726,9 +989,10 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
     }
     if ((rc = poll(S, true))) return rc;
     int status = GIP_OK;
-    if (S->h_state[0] == ST_BREAK)
+    const int fin = S->h_state[0] & ST_MASK;
+    if (fin == ST_BREAK)
         status = GIP_EBREAKDOWN;
-    else if (o->tol > 0.0 && S->h_state[0] != ST_CONV)
+    else if (o->tol > 0.0 && fin != ST_CONV)
         status = GIP_ENOCONV;
     if (stats) {
         stats->iters = S->h_state[1];

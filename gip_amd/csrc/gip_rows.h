@@ -172,4 +172,19 @@ struct RowSweep {
     for (RowSweep rs__(n); rs__.c < rs__.c1; rs__.c += rs__.step) \
         for (int row = rs__.c * kBlock + (int)threadIdx.x; row < (rs__.c + 1) * kBlock && row < (n); row = (n))
 
+// The rows FOR_ROWS gives this thread, as a cursor, so that a kernel can
+// request its first row's operands before its prologue and each next row's
+// right after storing the current one.  Only the last chunk of the matrix
+// holds rows >= n, so the first such row ends the sweep.
+struct ThreadRows {
+    RowSweep rs;
+    int row, n;
+    __device__ explicit ThreadRows(int n_) : rs(n_), n(n_) { row = rs.c * kBlock + (int)threadIdx.x; }
+    __device__ bool valid() const { return rs.c < rs.c1 && row < n; }
+    __device__ void next() {
+        rs.c += rs.step;
+        row = rs.c * kBlock + (int)threadIdx.x;
+    }
+};
+
 }  // namespace gip

@@ -302,7 +302,10 @@ def test_pcg_tolerance_iteration_count_within_one(dev, cav46, tol):
     assert st == 0 and r.status == 0
     assert abs(r.iters - it) <= 1, (r.iters, it)
     assert r.relres <= tol
-    assert np.linalg.norm(dx.download() - want) <= 1e-6 * np.linalg.norm(want)
+    # the same stopping iteration gives the same iterate (every row's last
+    # x update applied); one iteration apart, the iterates differ by a step
+    bound = 1e-10 if r.iters == it else 1e-6
+    assert np.linalg.norm(dx.download() - want) <= bound * np.linalg.norm(want)
 
 
 def test_pcg_mixed_precision_reaches_fp64_residual(dev, cav46):

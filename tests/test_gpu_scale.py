@@ -4,12 +4,14 @@ kernels at their 2048-workgroup cap, products of 8192+ workgroups whose
 partials are folded), against the oracle on the same inputs.
 
 Bar: FP64 PCG iterates within 1e-10 of the oracle's (only dot-product
-summation orders differ); the mixed-precision solve reaches a 1e-10 FP64
-true residual (checked with scipy's FP64 product); the integrator's
-per-step time, dt, massr and fields within 1e-10 of the oracle's.
+summation orders differ), tolerance stops within one iteration of the
+oracle's; the mixed-precision solve reaches a 1e-10 FP64 true residual
+(checked with scipy's FP64 product); the integrator's per-step time, dt,
+massr and fields within 1e-10 of the oracle's.
 """
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 from gip_amd import cavity, dns, hip
 from oracle import pyoracle
@@ -50,6 +52,29 @@ def test_pcg_fp64_128_fixed_iterations_vs_oracle(dev, lap128):
     assert abs(r.relres - rel) <= 1e-8 * rel
 
 
+def test_pcg_fp64_128_tolerance_stop_vs_oracle(dev, lap128):
+    """A solve stopped by its tolerance: the direction kernel that detects
+    convergence applies the last x update on every row (2048 workgroups)."""
+    c, L, b = lap128
+    n, nall = c.n_inner, c.n_all
+    x0 = np.zeros(nall)
+    A = hip.Matrix.from_csr(dev, L)
+    solver = hip.PCG(A, nall)
+    dx = dev.array(nall)
+    tol = 1e-5
+    r = solver.solve(dev.to_device(b), dx, dev.to_device(x0), maxit=5000, tol=tol)
+    want, st, it, rel = pyoracle.pcg(pyoracle.Csr.of(L), b, x0, n, 5000, tol=tol)
+    got = dx.download()
+    res = b[:n] - (L.to_scipy(nall) @ got)[:n]
+    true_rel = np.linalg.norm(res) / np.linalg.norm(b[:n])
+    print(f"[128^3 fp64 tol {tol:g}] iters {r.iters} (oracle {it}) relres {r.relres:.3e} true {true_rel:.3e}",
+          flush=True)
+    assert r.status == 0 and st == 0 and abs(r.iters - it) <= 1
+    assert r.relres <= tol and true_rel <= 1.01 * tol
+    if r.iters == it:
+        assert np.linalg.norm(got - want) <= 1e-10 * np.linalg.norm(want)
+
+
 def test_pcg_mixed_128_true_residual(dev, lap128):
     c, L, b = lap128
     n, nall = c.n_inner, c.n_all
@@ -83,3 +108,33 @@ def test_dns_96_reference_sequence_vs_oracle(tmp_path):
         err = np.linalg.norm(a - r) / np.linalg.norm(r)
         print(f"[96^3 field {f}] rel L2 {err:.3e}", flush=True)
         assert err <= 1e-10, (f, err)
+
+
+@pytest.mark.parametrize("n", [2 ** 19, 2 ** 19 + 1])
+def test_pcg_either_side_of_the_partial_fold(dev, n):
+    """2^19 rows: 2048 product chunks, read directly by the next kernels;
+    one row more: the chunk partials go through the fold kernel.  A shifted
+    1-D Laplacian: 40 fixed FP64 iterations and a tolerance stop against the
+    oracle, and a mixed-precision solve to a 1e-10 true residual."""
+    T = sp.diags([-1.0, 2.05, -1.0], [-1, 0, 1], shape=(n, n)).tocsr()
+    T.sort_indices()
+    b = np.random.default_rng(n).standard_normal(n)
+    x0 = np.zeros(n)
+    orc = pyoracle.Csr(T.indptr, T.indices, T.data, n)
+    M = hip.Matrix(dev, T.indptr, T.indices, T.data, ncols=n, fp32=True)
+    solver = hip.PCG(M, n)
+    dx = dev.array(n)
+    r = solver.solve(dev.to_device(b), dx, dev.to_device(x0), maxit=40)
+    want, st, it, rel = pyoracle.pcg(orc, b, x0, n, 40)
+    assert r.iters == it == 40 and r.status == 0
+    assert np.linalg.norm(dx.download() - want) <= 1e-10 * np.linalg.norm(want)
+    assert abs(r.relres - rel) <= 1e-8 * rel
+    r = solver.solve(dev.to_device(b), dx, dev.to_device(x0), maxit=1000, tol=1e-9)
+    want, st, it, rel = pyoracle.pcg(orc, b, x0, n, 1000, tol=1e-9)
+    print(f"[n={n}] tolerance stop: iters {r.iters} (oracle {it})", flush=True)
+    assert r.status == 0 and st == 0 and abs(r.iters - it) <= 1
+    if r.iters == it:
+        assert np.linalg.norm(dx.download() - want) <= 1e-10 * np.linalg.norm(want)
+    r = solver.solve(dev.to_device(b), dx, dev.to_device(x0), maxit=1000, tol=1e-10, precision=hip.PCG_MIXED)
+    x = dx.download()
+    assert r.status == 0 and np.linalg.norm(b - T @ x) <= 1.01e-10 * np.linalg.norm(b)
