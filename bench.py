@@ -428,10 +428,13 @@ def reference_suite(args):
     (oracle/_ref/gip_ref_harness dns: the reference sources compiled -O2,
     one MPI rank, one core, timed with MPI_Wtime around integrate()) and the
     GPU integrator (the HIP specification through include/gip_dns.h, with
-    the reference's PCG sequence and with the device-resident PCG), on the
-    same one-partition cases with the reference's fixed 100 PCG iterations."""
+    the reference's PCG sequence and with the device-resident PCG; and the
+    whole step device-resident and replayed from hipGraphs, as a
+    one-instance batch), on the same one-partition cases with the
+    reference's fixed 100 PCG iterations."""
     import tempfile
-    from gip_amd import cavity, dns
+    from gip_amd import batch as gbatch
+    from gip_amd import cavity, dns, hip
     exe = ROOT / "oracle" / "_ref" / "gip_ref_harness"
     if not exe.exists():
         log("reference harness absent (built where the reference sources are present)")
@@ -452,6 +455,7 @@ def reference_suite(args):
             if line.startswith("integrate_ms"):
                 cpu_ms = float(line.split()[1])
         res = {"case": label, "steps": steps, "cpu_ms_per_step": cpu_ms / steps if cpu_ms else None}
+        u_fused = None
         for mode in ("reference", "fused"):
             sim = dns.Simulation(root=root, name=name, gamma=gamma, pcg_mode=mode)
             sim.step()  # first step: uploads and lazy set-up out of the timed region
@@ -461,7 +465,23 @@ def reference_suite(args):
                 sim.step()
             sim.sync()
             res[f"gpu_{mode}_ms_per_step"] = 1e3 * (time.perf_counter() - t0) / max(1, steps - 1)
+            if mode == "fused":
+                u_fused = sim.field("u")
             sim.close()
+        dev = hip.Device(0)
+        bt = gbatch.Batch(dev, [cavity.read(root, name)], [gamma], pcg_tol=1e-300, pcg_maxit=100,
+                          precision=hip.PCG_FP64, poll=100, use_graph=True)
+        bt.step()  # graph capture
+        dev.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps - 1):
+            bt.step()
+        dev.sync()
+        res["gpu_graph_step_ms_per_step"] = 1e3 * (time.perf_counter() - t0) / max(1, steps - 1)
+        u_graph = bt.field(0, "u")
+        res["graph_vs_fused_max_rel_u"] = float(np.abs(u_graph - u_fused).max() / max(1e-30, np.abs(u_fused).max()))
+        bt.close()
+        dev.close()
         if name == "C100K" and (root / "InputFiles" / "Topos" / "C100K" / "C100K_4p").exists():
             # the reference's own decomposition: four ranks, four cores
             mpirun = shutil.which("mpirun") or "/opt/conda/bin/mpirun"
@@ -474,7 +494,9 @@ def reference_suite(args):
                         res["cpu_4ranks_ms_per_step"] = float(line.split()[1]) / steps
         log(f"{label}: cpu {res['cpu_ms_per_step']} ms/step (4 ranks: {res.get('cpu_4ranks_ms_per_step')}), "
             f"gpu reference-sequence "
-            f"{res['gpu_reference_ms_per_step']:.2f}, device PCG {res['gpu_fused_ms_per_step']:.2f} ms/step")
+            f"{res['gpu_reference_ms_per_step']:.2f}, device PCG {res['gpu_fused_ms_per_step']:.2f}, "
+            f"graph step {res['gpu_graph_step_ms_per_step']:.2f} ms/step "
+            f"(u vs device PCG {res['graph_vs_fused_max_rel_u']:.1e})")
         rows.append(res)
     print(json.dumps({
         "metric": "integrator ms per step, reference CPU path (1 core) vs GPU, fixed 100 PCG iterations",

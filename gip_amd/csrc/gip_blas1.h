@@ -110,32 +110,48 @@ __device__ inline bool last_block_reduce2(double v0, double v1, double *partials
 }
 
 // M sums of partials at once, for partial arrays of kMaxRedBlocks doubles
-// whose entries past their producer's grid hold zeros: every slot of every
-// array is loaded unconditionally and all loads are issued before the first
-// is used, so the M sums cost one memory round trip (a load under a bounds
-// test would be turned into a branch with a wait at its join).  Adding the
-// zero slots gives bitwise the total sum_partials<RED_SUM> gives over the
-// first n entries.
+// whose entries past their producer's grid hold zeros.  Every load is
+// issued unconditionally (a load under a bounds test would be turned into a
+// branch with a wait at its join): a slot past n[m] is read from the array's
+// last entry, a zero whenever n[m] < kMaxRedBlocks, so those reads are one
+// broadcast cache line.  All loads are issued before the first is used
+// (load(), early in a kernel) and reduce() sums them: the M sums cost one
+// memory round trip.  The zero slots make each total bitwise the one
+// sum_partials<RED_SUM> gives over the first n[m] entries.
 template <int M>
-__device__ inline void sum_padded(const double *const (&part)[M], double (&out)[M]) {
-    __shared__ double s_all[M];
-    constexpr int K = kMaxRedBlocks / kBlock;
+struct PaddedSums {
+    static constexpr int K = kMaxRedBlocks / kBlock;
     double v[M][K];
+    __device__ inline void load(const double *const (&part)[M], const int (&n)[M]) {
 #pragma unroll
-    for (int m = 0; m < M; ++m)
+        for (int m = 0; m < M; ++m)
 #pragma unroll
-        for (int k = 0; k < K; ++k) v[m][k] = part[m][threadIdx.x + k * kBlock];
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-        double acc = 0.0;
-#pragma unroll
-        for (int k = 0; k < K; ++k) acc = acc + v[m][k];
-        acc = block_reduce<RED_SUM>(acc);
-        if (threadIdx.x == 0) s_all[m] = acc;
+            for (int k = 0; k < K; ++k) {
+                const int i = threadIdx.x + k * kBlock;
+                v[m][k] = part[m][i < n[m] ? i : kMaxRedBlocks - 1];
+            }
     }
-    __syncthreads();
+    __device__ inline void reduce(double (&out)[M]) {
+        __shared__ double s_all[M];
 #pragma unroll
-    for (int m = 0; m < M; ++m) out[m] = s_all[m];
+        for (int m = 0; m < M; ++m) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) acc = acc + v[m][k];
+            acc = block_reduce<RED_SUM>(acc);
+            if (threadIdx.x == 0) s_all[m] = acc;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < M; ++m) out[m] = s_all[m];
+    }
+};
+
+template <int M>
+__device__ inline void sum_padded(const double *const (&part)[M], const int (&n)[M], double (&out)[M]) {
+    PaddedSums<M> ps;
+    ps.load(part, n);
+    ps.reduce(out);
 }
 
 // Combine n partials in index order (thread t takes t, t+256, ..., then the

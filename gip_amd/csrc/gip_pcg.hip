@@ -70,7 +70,10 @@ enum {
 // iteration k writes ST_CONV | k << 2, so readers look at the low two bits.
 enum { ST_RUN = 0, ST_CONV = 1, ST_BREAK = 2, ST_MASK = 3 };
 // Partial-sum arrays, kMaxRedBlocks doubles each.
-enum { P_DN = 0, P_RS = 1, P_RR = 2, P_BB = 3, P_RS2 = 4, P_RR2 = 5, P_SUMR = 6, P_COUNT = 7 };
+// P_DN2: p.Ap partials of odd iterations on operators small enough for the
+// fused direction-product kernel (which reads the previous iteration's
+// partials while writing its own).
+enum { P_DN = 0, P_RS = 1, P_RR = 2, P_BB = 3, P_RS2 = 4, P_RR2 = 5, P_SUMR = 6, P_DN2 = 7, P_COUNT = 8 };
 
 struct PcgDev {
     int n, G;      // solved rows, workgroups of the sweeping kernels (<= kMaxRedBlocks)
@@ -110,6 +113,8 @@ __device__ inline double early_load(const double *p) {
 __device__ inline int conv_stamp(int k) { return ST_CONV | (k << 2); }
 __device__ inline bool leader() { return blockIdx.x == 0 && threadIdx.x == 0; }
 __device__ inline double rho_slot(const PcgDev &d, int k) { return d.sc[S_RHO0 + (k & 1)]; }
+// p.Ap partials of iteration k in the fused (small-operator) iteration
+__device__ inline double *dn_part(const PcgDev &d, int k) { return d.P((k & 1) ? P_DN2 : P_DN); }
 
 // ---- setup ---------------------------------------------------------------
 
@@ -117,11 +122,14 @@ __device__ inline double rho_slot(const PcgDev &d, int k) { return d.sc[S_RHO0 +
 // Ap = (A x0)_i or x0_i on empty rows, r = b + (-1.0) Ap   (:55-70)
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_setup_resid(EllView A, PcgDev d, const double *__restrict__ x0, int n_all,
-                                                        double *__restrict__ p) {
+                                                        double *__restrict__ p, double *__restrict__ p2) {
     const int stride = gridDim.x * kBlock;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n_all; i += stride) {
         d.x[i] = x0[i];
-        if (i >= d.n) p[i] = x0[i];
+        if (i >= d.n) {
+            p[i] = x0[i];
+            if (p2) p2[i] = x0[i];
+        }
     }
     Operand<double> xo{x0};
     FOR_ROWS(d.n, row) {
@@ -190,84 +198,48 @@ __device__ inline void x_update(const PcgDev &d, const double *__restrict__ p, d
     }
 }
 
-// Kernel prologues.  SMALL (operators of at most 2^19 rows, where the grid
-// gives each thread at most one row and a kernel lasts a few microseconds):
-// the state word, the scalars, the partials and the first row's operands
-// are requested together, so the kernel spends one memory round trip before
-// its first store.  Otherwise (long kernels, where that latency is hidden
-// by the sweep) the state word is tested first, with a scalar load, and the
+// Kernel prologues.  SMALL (operators of at most 2^19 rows, where every
+// thread holds at most one row and a kernel lasts a few microseconds): the
+// state word, the scalars, the partials and the row's operands are requested
+// together, so the kernel spends one memory round trip before its first
+// store, and the direction update is fused into the product (two kernels
+// per iteration).  Otherwise (long kernels, where that latency is hidden by
+// the sweep) the state word is tested first, with a scalar load, and the
 // partial arrays are summed one at a time, which keeps the register count,
-// and with it the occupancy of the sweep, where it was.
+// and with it the occupancy of the sweep, low.
 
 // K0 (k >= 1): rho_k, the convergence test, x += alpha_{k-1} p,
 // beta, p = 1.0 s + beta p
-template <bool SMALL>
 __global__ __launch_bounds__(kBlock) void k_iter_dir(PcgDev d, double *__restrict__ p, int k) {
     double *__restrict__ x = d.x;
     const double *__restrict__ s = d.s;
-    if constexpr (!SMALL) {
-        const int st = *d.state;
-        if (st != ST_RUN && st != conv_stamp(k)) return;
-        const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);  // :107
-        const double alpha = alpha_of(d, k - 1);
-        const double tol2 = d.sc[S_TOL2];
-        if (tol2 >= 0.0) {
-            const double rr = sum_partials<RED_SUM>(d.P(P_RR), d.G);
-            if (rr <= tol2) {  // every workgroup reaches the same decision
-                x_update(d, p, alpha);
-                if (leader()) *d.state = conv_stamp(k);
-                return;
-            }
-        }
-        const double beta = rho / rho_slot(d, k - 1);  // :109
-        if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
-        FOR_ROWS(d.n, row) {
-            const double xo = x[row], si = s[row], po = p[row];
-            x[row] = xo + alpha * po;       // :95 of iteration k-1
-            p[row] = 1.0 * si + beta * po;  // :113
-        }
-        return;
-    }
-    ThreadRows t(d.n);
-    double xo = 0.0, si = 0.0, po = 0.0;
-    const int st = early_load(d.state);
-    if (t.valid()) {
-        xo = x[t.row];
-        si = s[t.row];
-        po = p[t.row];
-    }
-    const double tol2 = early_load(d.sc + S_TOL2);
-    const double rho_prev = early_load(d.sc + S_RHO0 + ((k - 1) & 1));
-    double sums[3];
-    sum_padded<3>({d.P(P_RS), d.P(P_DN), d.P(P_RR)}, sums);
+    const int st = *d.state;
     if (st != ST_RUN && st != conv_stamp(k)) return;
-    const double rho = sums[0];
-    const double alpha = rho_prev / sums[1];  // alpha_{k-1}, bitwise as alpha_of gives it
-    if (tol2 >= 0.0 && sums[2] <= tol2) {
-        x_update(d, p, alpha);
-        if (leader()) *d.state = conv_stamp(k);
-        return;
+    const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);  // :107
+    const double alpha = alpha_of(d, k - 1);
+    const double tol2 = d.sc[S_TOL2];
+    if (tol2 >= 0.0) {
+        const double rr = sum_partials<RED_SUM>(d.P(P_RR), d.G);
+        if (rr <= tol2) {  // every workgroup reaches the same decision
+            x_update(d, p, alpha);
+            if (leader()) *d.state = conv_stamp(k);
+            return;
+        }
     }
-    const double beta = rho / rho_prev;
+    const double beta = rho / rho_slot(d, k - 1);  // :109
     if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
-    while (t.valid()) {
-        const int row = t.row;
+    FOR_ROWS(d.n, row) {
+        const double xo = x[row], si = s[row], po = p[row];
         x[row] = xo + alpha * po;       // :95 of iteration k-1
         p[row] = 1.0 * si + beta * po;  // :113
-        t.next();
-        if (t.valid()) {
-            xo = x[t.row];
-            si = s[t.row];
-            po = p[t.row];
-        }
     }
 }
 
-// after the loop: the last iteration's x update, unless the solve stopped in
-// K0 (converged: already applied) or on a breakdown (no valid alpha)
-__global__ __launch_bounds__(kBlock) void k_x_flush(PcgDev d, const double *__restrict__ p, int k) {
+// (dn: iteration k's p.Ap partials)
+__global__ __launch_bounds__(kBlock) void k_x_flush(PcgDev d, const double *__restrict__ p, int k,
+                                                    const double *__restrict__ dn) {
     if (!running(d)) return;
-    x_update(d, p, alpha_of(d, k));
+    x_update(d, p, rho_slot(d, k) / sum_partials<RED_SUM>(dn, d.Gdn));
 }
 
 // K1: Ap = A p (empty rows keep Ap), one partial of p.Ap per 256-row chunk;
@@ -325,6 +297,80 @@ __global__ __launch_bounds__(kBlock) void k_iter_spmv_small(EllView A, PcgDev d,
     iter_spmv<W, true>(A, d, p);
 }
 
+// The direction p_k = 1.0 s + beta p_{k-1} read at any column j of the
+// product, computed exactly as the direction update computes it on the
+// solved rows (:113, no contraction: the same bits); past the solved rows
+// (ghost columns) the direction keeps its value.
+struct DirOperand {
+    const double *__restrict__ s, *__restrict__ p;
+    double beta;
+    int n;
+    __device__ inline double at(int j) const {
+        const double po = p[j], si = s[j < n ? j : 0];
+        return j < n ? 1.0 * si + beta * po : po;
+    }
+};
+
+// Small operators, k >= 1: K0 and K1 in one kernel over the product's grid
+// (one row per thread).  rho_k, the convergence test, x += alpha_{k-1} p_{k-1}
+// on the thread's row, p_k = 1.0 s + beta p_{k-1} into the other direction
+// buffer, Ap = A p_k with p_k formed at every column read, and the p_k.Ap
+// partials of iteration k (read by K2 of k while this kernel's own
+// partials of k-1 stay intact: two partial arrays by parity).
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_iter_dirprod(EllView A, PcgDev d, const double *__restrict__ pold,
+                                                         double *__restrict__ pnew, int k) {
+    const int st = early_load(d.state);
+    const int c = xcd_block(blockIdx.x, gridDim.x);
+    const int row = c * kBlock + threadIdx.x;
+    const bool own = row < d.n;
+    double xo = 0.0, si = 0.0, po = 0.0;
+    if (own) {
+        xo = d.x[row];
+        si = d.s[row];
+        po = pold[row];
+    }
+    const double tol2 = early_load(d.sc + S_TOL2);
+    const double rho_prev = early_load(d.sc + S_RHO0 + ((k - 1) & 1));
+    PaddedSums<3> ps;
+    ps.load({d.P(P_RS), dn_part(d, k - 1), d.P(P_RR)}, {d.G, d.Gdn, d.G});
+    // stencil layout: the operator's loads (none depends on beta) go out
+    // while the prologue's are in flight
+    constexpr int SW = W < 0 ? -W : 1;
+    StencilPre<SW, double> q;
+    if (W < 0 && own) stencil_row_pre<SW, double>(A, A.val, row, q);
+    double sums[3];
+    ps.reduce(sums);
+    if (st != ST_RUN && st != conv_stamp(k)) return;
+    const double alpha = rho_prev / sums[1];  // alpha_{k-1}, bitwise as K2 of k-1 formed it (:93)
+    if (tol2 >= 0.0 && sums[2] <= tol2) {     // every workgroup reaches the same decision
+        if (own) d.x[row] = xo + alpha * po;
+        if (leader()) *d.state = conv_stamp(k);
+        return;
+    }
+    const double beta = sums[0] / rho_prev;  // :107-109
+    if (leader()) d.sc[S_RHO0 + (k & 1)] = sums[0];
+    double pr = 0.0, sum = 0.0;
+    if (own) {
+        d.x[row] = xo + alpha * po;  // :95 of iteration k-1
+        pr = 1.0 * si + beta * po;   // :113
+        pnew[row] = pr;
+        const DirOperand op{d.s, pold, beta, d.n};
+        bool any;
+        if constexpr (W < 0)
+            any = stencil_row_fin<SW, double, double>(A, A.val, op, row, q, sum);  // :87
+        else
+            any = row_dot<W, double, double>(A, A.val, op, row, sum);
+        if (any)
+            d.Ap[row] = sum;
+        else
+            sum = d.Ap[row];  // empty row: Ap keeps its value
+    }
+    double acc = own ? pr * sum : 0.0;
+    acc = block_reduce<RED_SUM>(acc);
+    if (threadIdx.x == 0) dn_part(d, k)[c] = acc;
+}
+
 // the per-chunk partials, folded in index order into Gdn partials of P_DN
 __global__ __launch_bounds__(kBlock) void k_fold_dn(PcgDev d) {
     if (!running(d)) return;
@@ -379,7 +425,7 @@ __global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, int k) {
         }
         const double rho = early_load(d.sc + S_RHO0 + (k & 1));
         double sums[1];
-        sum_padded<1>({d.P(P_DN)}, sums);
+        sum_padded<1>({dn_part(d, k)}, {d.Gdn}, sums);
         const double dn = sums[0];
         if (st != ST_RUN) return;
         const double alpha = rho / dn;
@@ -432,50 +478,75 @@ __global__ __launch_bounds__(kBlock) void k_fin(PcgDev d) {
 // K0 (k >= 1): rho_k, the correction update xc += alpha_{k-1} p of the
 // previous iteration (`pending`: not yet folded by a defect correction),
 // beta (0 after a restart: the previous rho slot is +inf), p32 = s32 + beta p32
-template <bool SMALL>
 __global__ __launch_bounds__(kBlock) void k_m_dir(PcgDev d, float *__restrict__ p, int k, int pending) {
     float *__restrict__ xc = d.xc32;
     const float *__restrict__ s = d.s32;
-    if constexpr (!SMALL) {
-        if (!running(d)) return;
-        const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);
-        const float a32 = pending ? (float)alpha_of(d, k - 1) : 0.0f;
-        const float beta = (float)(rho / rho_slot(d, k - 1));
-        if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
-        FOR_ROWS(d.n, row) {
-            const float xo = xc[row], si = s[row], po = p[row];
-            if (pending) xc[row] = xo + a32 * po;
-            p[row] = si + beta * po;
-        }
-        return;
-    }
-    ThreadRows t(d.n);
-    float xo = 0.0f, si = 0.0f, po = 0.0f;
-    const int st = early_load(d.state);
-    if (t.valid()) {
-        xo = xc[t.row];
-        si = s[t.row];
-        po = p[t.row];
-    }
-    const double rho_prev = early_load(d.sc + S_RHO0 + ((k - 1) & 1));
-    double sums[2];
-    sum_padded<2>({d.P(P_RS), d.P(P_DN)}, sums);
-    if (st != ST_RUN) return;
-    const double rho = sums[0];
-    const float a32 = pending ? (float)(rho_prev / sums[1]) : 0.0f;  // alpha_{k-1}, as alpha_of gives it
-    const float beta = (float)(rho / rho_prev);
+    if (!running(d)) return;
+    const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);
+    const float a32 = pending ? (float)alpha_of(d, k - 1) : 0.0f;
+    const float beta = (float)(rho / rho_slot(d, k - 1));
     if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
-    while (t.valid()) {
-        const int row = t.row;
+    FOR_ROWS(d.n, row) {
+        const float xo = xc[row], si = s[row], po = p[row];
         if (pending) xc[row] = xo + a32 * po;
         p[row] = si + beta * po;
-        t.next();
-        if (t.valid()) {
-            xo = xc[t.row];
-            si = s[t.row];
-            po = p[t.row];
-        }
     }
+}
+
+// (as DirOperand, in FP32: the direction update's float expression)
+struct DirOperand32 {
+    const float *__restrict__ s, *__restrict__ p;
+    float beta;
+    int n;
+    __device__ inline float at(int j) const {
+        const float po = p[j], si = s[j < n ? j : 0];
+        return j < n ? si + beta * po : po;
+    }
+};
+
+// Small operators, k >= 1: k_m_dir and the FP32 product in one kernel (as
+// k_iter_dirprod)
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_m_dirprod(EllView A, PcgDev d, const float *__restrict__ pold,
+                                                      float *__restrict__ pnew, int k, int pending) {
+    const int st = early_load(d.state);
+    const int c = xcd_block(blockIdx.x, gridDim.x);
+    const int row = c * kBlock + threadIdx.x;
+    const bool own = row < d.n;
+    float xo = 0.0f, si = 0.0f, po = 0.0f;
+    if (own) {
+        xo = d.xc32[row];
+        si = d.s32[row];
+        po = pold[row];
+    }
+    const double rho_prev = early_load(d.sc + S_RHO0 + ((k - 1) & 1));
+    PaddedSums<2> ps;
+    ps.load({d.P(P_RS), dn_part(d, k - 1)}, {d.G, d.Gdn});
+    constexpr int SW = W < 0 ? -W : 1;
+    StencilPre<SW, float> q;
+    if (W < 0 && own) stencil_row_pre<SW, float>(A, A.val32, row, q);
+    double sums[2];
+    ps.reduce(sums);
+    if (st != ST_RUN) return;
+    const float a32 = pending ? (float)(rho_prev / sums[1]) : 0.0f;  // alpha_{k-1}, as K2 of k-1 formed it
+    const float beta = (float)(sums[0] / rho_prev);
+    if (leader()) d.sc[S_RHO0 + (k & 1)] = sums[0];
+    double acc = 0.0;
+    if (own) {
+        if (pending) d.xc32[row] = xo + a32 * po;
+        const float pr = si + beta * po;
+        pnew[row] = pr;
+        const DirOperand32 op{d.s32, pold, beta, d.n};
+        float sum;
+        if constexpr (W < 0)
+            stencil_row_fin<SW, float, float>(A, A.val32, op, row, q, sum);
+        else
+            row_dot<W, float, float>(A, A.val32, op, row, sum);
+        d.Ap32[row] = sum;
+        acc = (double)pr * (double)sum;
+    }
+    acc = block_reduce<RED_SUM>(acc);
+    if (threadIdx.x == 0) dn_part(d, k)[c] = acc;
 }
 
 // K1 in FP32 (SMALL as in iter_spmv)
@@ -564,7 +635,7 @@ __global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, int k) {
         }
         const double rho = early_load(d.sc + S_RHO0 + (k & 1));
         double sums[1];
-        sum_padded<1>({d.P(P_DN)}, sums);
+        sum_padded<1>({dn_part(d, k)}, {d.Gdn}, sums);
         const double dn = sums[0];
         if (st != ST_RUN) return;
         const double alpha = rho / dn;
@@ -603,8 +674,10 @@ __global__ __launch_bounds__(kBlock) void k_m_status(PcgDev d) {
 }
 
 // x += (xc + alpha_k p) (the last iteration's pending update first), xc = 0
-__global__ __launch_bounds__(kBlock) void k_m_fold(PcgDev d, const float *__restrict__ p, int k) {
-    const float a32 = (float)alpha_of(d, k);
+// (dn: iteration k's p.Ap partials)
+__global__ __launch_bounds__(kBlock) void k_m_fold(PcgDev d, const float *__restrict__ p, int k,
+                                                   const double *__restrict__ dn) {
+    const float a32 = (float)(rho_slot(d, k) / sum_partials<RED_SUM>(dn, d.Gdn));
     FOR_ROWS(d.n, row) {
         const float xcv = d.xc32[row] + a32 * p[row];
         d.x[row] = d.x[row] + (double)xcv;
@@ -712,10 +785,13 @@ struct gip_pcg {
     int n, n_all, grid;
     double *r, *p, *s, *Ap, *dinv;
     float *r32, *p32, *s32, *Ap32, *xc32, *dinv32;
-    double *az;  // A 1 (MIXED coarse step)
+    double *p2;   // second direction buffer of the fused iteration (small operators)
+    float *p32b;  // its FP32 counterpart
+    double *az;   // A 1 (MIXED coarse step)
     double *sc, *part, *chunk;
     int nchunk, gdn;
-    bool fold;  // nchunk > kMaxRedBlocks: the chunk partials need the fold kernel
+    bool fold;  // nchunk > kMaxRedBlocks: the chunk partials need the fold kernel;
+                // otherwise the fused two-kernel iteration runs
     int *state;    // state, iters
     int *h_state;  // pinned copy
     double *h_sc;  // pinned scalars
@@ -805,6 +881,7 @@ int gip_pcg_create(gip_ctx *c, const gip_mat *A, int32_t n_all, gip_pcg **out) {
     };
     dalloc((void **)&S->r, n * 8);
     dalloc((void **)&S->p, na * 8);
+    if (!S->fold) dalloc((void **)&S->p2, na * 8);
     dalloc((void **)&S->s, n * 8);
     dalloc((void **)&S->Ap, n * 8);
     dalloc((void **)&S->dinv, n * 8);
@@ -840,8 +917,8 @@ int gip_pcg_destroy(gip_pcg *S) {
     if (!S) return GIP_OK;
     hipSetDevice(S->ctx->device);
     hipStreamSynchronize(S->ctx->stream);
-    void *ptrs[] = {S->r,   S->p,    S->s,    S->Ap,     S->dinv, S->r32, S->p32,  S->s32,
-                    S->Ap32, S->xc32, S->dinv32, S->az, S->sc,   S->part, S->chunk, S->state};
+    void *ptrs[] = {S->r,    S->p,      S->s,  S->Ap, S->dinv, S->r32,   S->p32,  S->s32,  S->Ap32,
+                    S->xc32, S->dinv32, S->az, S->sc, S->part, S->chunk, S->state, S->p2, S->p32b};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (S->h_state) hipHostFree(S->h_state);
@@ -864,6 +941,10 @@ static int ensure_mixed(gip_pcg *S) {
     // there (the correction leaves the ghost values of x unchanged)
     GIP_HIP_TRY(hipMalloc(&S->p32, na * 4));
     GIP_HIP_TRY(hipMemsetAsync(S->p32, 0, na * 4, S->ctx->stream));
+    if (!S->fold) {
+        GIP_HIP_TRY(hipMalloc(&S->p32b, na * 4));
+        GIP_HIP_TRY(hipMemsetAsync(S->p32b, 0, na * 4, S->ctx->stream));
+    }
     if (S->n) {
         k_to32<<<(S->n + kBlock - 1) / kBlock, kBlock, 0, S->ctx->stream>>>(S->n, S->dinv, S->dinv32);
         GIP_LAUNCH_CHECK("dinv32");
@@ -912,8 +993,15 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
     EllView v = view(A);
     const int pollk = o->poll > 0 ? o->poll : 16;
     const int use_coarse = o->coarse >= 0 ? 1 : 0;
+    // direction and p.Ap partials of iteration k: one buffer and one partial
+    // array with the fold; alternating by parity in the fused iteration
+    auto pdir = [&](int k) { return (!S->fold && (k & 1)) ? S->p2 : S->p; };
+    auto pdir32 = [&](int k) { return (!S->fold && (k & 1)) ? S->p32b : S->p32; };
+    auto pdn = [&](int k) {
+        return (const double *)(S->part + (size_t)((!S->fold && (k & 1)) ? P_DN2 : P_DN) * kMaxRedBlocks);
+    };
     // setup, GIP_LinearSolver.h:55-80
-    DISPATCH_W(A, k_setup_resid, v, d, x0, S->n_all, S->p);
+    DISPATCH_W(A, k_setup_resid, v, d, x0, S->n_all, S->p, S->p2);
     k_setup_dir<<<S->grid, kBlock, 0, st>>>(d, S->p);
     k_setup_fin<<<1, kBlock, 0, st>>>(d, o->tol);
     GIP_LAUNCH_CHECK("pcg setup");
@@ -925,20 +1013,22 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
             int chunk = std::min(pollk, o->maxit - k);
             for (int j = 0; j < chunk; ++j, ++k) {
                 if (S->fold) {
-                    if (k > 0) k_iter_dir<false><<<S->grid, kBlock, 0, st>>>(d, S->p, k);
+                    if (k > 0) k_iter_dir<<<S->grid, kBlock, 0, st>>>(d, S->p, k);
                     DISPATCH_WG(A, S->nchunk, k_iter_spmv, v, d, S->p);
                     k_fold_dn<<<S->gdn, kBlock, 0, st>>>(d);
                     k_iter_update<false><<<S->grid, kBlock, 0, st>>>(d, k);
                 } else {
-                    if (k > 0) k_iter_dir<true><<<S->grid, kBlock, 0, st>>>(d, S->p, k);
-                    DISPATCH_WG(A, S->nchunk, k_iter_spmv_small, v, d, S->p);
+                    if (k > 0)
+                        DISPATCH_WG(A, S->nchunk, k_iter_dirprod, v, d, pdir(k - 1), pdir(k), k);
+                    else
+                        DISPATCH_WG(A, S->nchunk, k_iter_spmv_small, v, d, S->p);
                     k_iter_update<true><<<S->grid, kBlock, 0, st>>>(d, k);
                 }
             }
             GIP_LAUNCH_CHECK("pcg iteration");
             if ((rc = poll(S, false))) return rc;
         }
-        if (k > 0) k_x_flush<<<S->grid, kBlock, 0, st>>>(d, S->p, k - 1);
+        if (k > 0) k_x_flush<<<S->grid, kBlock, 0, st>>>(d, pdir(k - 1), k - 1, pdn(k - 1));
         k_fin<<<1, kBlock, 0, st>>>(d);
         GIP_LAUNCH_CHECK("pcg final test");
     } else {
@@ -957,13 +1047,15 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
             int chunk = std::min(pollk, o->maxit - k);
             for (int j = 0; j < chunk; ++j, ++k) {
                 if (S->fold) {
-                    if (k > 0) k_m_dir<false><<<S->grid, kBlock, 0, st>>>(d, S->p32, k, pending);
+                    if (k > 0) k_m_dir<<<S->grid, kBlock, 0, st>>>(d, S->p32, k, pending);
                     DISPATCH_WG(A, S->nchunk, k_m_spmv, v, d, S->p32);
                     k_fold_dn<<<S->gdn, kBlock, 0, st>>>(d);
                     k_m_update<false><<<S->grid, kBlock, 0, st>>>(d, k);
                 } else {
-                    if (k > 0) k_m_dir<true><<<S->grid, kBlock, 0, st>>>(d, S->p32, k, pending);
-                    DISPATCH_WG(A, S->nchunk, k_m_spmv_small, v, d, S->p32);
+                    if (k > 0)
+                        DISPATCH_WG(A, S->nchunk, k_m_dirprod, v, d, pdir32(k - 1), pdir32(k), k, pending);
+                    else
+                        DISPATCH_WG(A, S->nchunk, k_m_spmv_small, v, d, S->p32);
                     k_m_update<true><<<S->grid, kBlock, 0, st>>>(d, k);
                 }
                 pending = 1;
@@ -975,7 +1067,7 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
             const double rr32 = S->h_sc[S_RR], tol2 = S->h_sc[S_TOL2];
             const bool near_target = tol2 >= 0.0 && rr32 <= tol2;
             if (rr32 <= inner2 * last_rr || near_target || k >= o->maxit) {
-                k_m_fold<<<S->grid, kBlock, 0, st>>>(d, S->p32, k - 1);
+                k_m_fold<<<S->grid, kBlock, 0, st>>>(d, pdir32(k - 1), k - 1, pdn(k - 1));
                 pending = 0;
                 DISPATCH_W(A, k_m_resid, v, d);
                 k_m_coarse<<<S->grid, kBlock, 0, st>>>(d, use_coarse);

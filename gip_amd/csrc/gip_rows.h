@@ -56,20 +56,41 @@ __device__ inline int xcd_block(int b, int G) {
     return x * q + (x < r ? x : r) + idx;
 }
 
-// The input vector of a product.
+// The input vector of a product.  The row sums take any operand type with
+// at(j) (the PCG also forms a direction on the fly as its operand).
 template <class T>
 struct Operand {
     const T *__restrict__ p;
     __device__ inline T at(int j) const { return p[j]; }
 };
 
+// The tail of a row in a slice of kind 2 (empty rows, or rows of several
+// tail entries), located through the slice header's lane masks; false for
+// an empty row.
+template <class T, class V, class X>
+__device__ inline bool stencil_tail_hdr(const EllView &A, const V *__restrict__ val, const X &x, int s, int row,
+                                        T &sum) {
+    const StencilHdr *h = A.st_hdr + s;
+    const uint64_t bit = 1ull << (row & 63), below = bit - 1ull;
+    if (h->empty & bit) return false;  // empty row: left untouched
+    int t = h->tsoff;
+    for (int j = 0; j < kStencilMaxTail; ++j) {
+        const uint64_t mj = h->m[j];
+        if (!(mj & bit)) break;
+        const int e = t + __popcll(mj & below);
+        sum = sum + x.at(A.st_tcol[e]) * val[A.tail_base + e];
+        t += __popcll(mj);
+    }
+    return true;
+}
+
 // Stencil-layout row sum: the slots in order (column row + offset, clamped
 // into the vector for slots the row lacks, whose value is 0), then the
 // row's tail in stored order.  The wavefront's 64 rows are one slice, so
 // the slice's offsets and header are wavefront-uniform (scalar loads); a
 // tail entry's position follows from the header's lane masks.
-template <int SW, class T, class V>
-__device__ inline bool stencil_row(const EllView &A, const V *__restrict__ val, const Operand<T> &x, int row, T &sum) {
+template <int SW, class T, class V, class X>
+__device__ inline bool stencil_row(const EllView &A, const V *__restrict__ val, const X &x, int row, T &sum) {
     const int w = SW > 0 ? SW : A.st_w;
     const int s = __builtin_amdgcn_readfirstlane(row >> 6);
     const int32_t *off = A.st_off + (size_t)s * w;
@@ -103,29 +124,71 @@ __device__ inline bool stencil_row(const EllView &A, const V *__restrict__ val, 
             sum = sum + x.at(A.st_tcol[e]) * val[A.tail_base + e];
         }
     } else if (kind == 2) {
-        const StencilHdr *h = A.st_hdr + s;
-        const uint64_t bit = 1ull << (row & 63), below = bit - 1ull;
-        if (h->empty & bit) return false;  // empty row: left untouched
-        int t = h->tsoff;
-        for (int j = 0; j < kStencilMaxTail; ++j) {
-            const uint64_t mj = h->m[j];
-            if (!(mj & bit)) break;
-            const int e = t + __popcll(mj & below);
-            sum = sum + x.at(A.st_tcol[e]) * val[A.tail_base + e];
-            t += __popcll(mj);
+        return stencil_tail_hdr(A, val, x, s, row, sum);
+    }
+    return true;
+}
+
+// The operator's part of a stencil row (slot columns and values, and for a
+// slice of kind 0 or 1 its one possible tail entry), loaded apart from the
+// operand so that a kernel can request it before its prologue; the operand
+// reads follow in stencil_row_fin, in the order stencil_row uses.
+template <int SW, class V>
+struct StencilPre {
+    int c[SW];
+    V v[SW];
+    int kind, slice, tcol;  // tcol < 0: no tail entry (kinds 0 and 1)
+    V tval;
+};
+
+template <int SW, class V>
+__device__ inline void stencil_row_pre(const EllView &A, const V *__restrict__ val, int row, StencilPre<SW, V> &q) {
+    const int s = __builtin_amdgcn_readfirstlane(row >> 6);
+    const int32_t *off = A.st_off + (size_t)s * SW;
+    const V *vs = val + (size_t)s * SW * 64 + (row & 63);
+#pragma unroll
+    for (int k = 0; k < SW; ++k) {
+        const int cc = row + off[k];
+        q.c[k] = cc < 0 ? 0 : (cc > A.cmax ? A.cmax : cc);
+        q.v[k] = __builtin_nontemporal_load(&vs[64 * k]);
+    }
+    const StencilTail tl = A.st_tail[s];  // the 16-byte record in one load
+    q.slice = s;
+    q.kind = tl.kind;
+    q.tcol = -1;
+    q.tval = V(0);
+    if (q.kind == 1) {
+        const uint64_t bit = 1ull << (row & 63);
+        if (tl.m0 & bit) {
+            const int e = tl.tsoff + __popcll(tl.m0 & (bit - 1ull));
+            q.tcol = A.st_tcol[e];
+            q.tval = val[A.tail_base + e];
         }
+    }
+}
+
+template <int SW, class T, class V, class X>
+__device__ inline bool stencil_row_fin(const EllView &A, const V *__restrict__ val, const X &x, int row,
+                                       const StencilPre<SW, V> &q, T &sum) {
+    sum = T(0);
+#pragma unroll
+    for (int k = 0; k < SW; ++k) sum = sum + x.at(q.c[k]) * q.v[k];
+    if (q.kind == 1) {
+        if (q.tcol >= 0) sum = sum + x.at(q.tcol) * q.tval;
+    } else if (q.kind == 2) {
+        return stencil_tail_hdr(A, val, x, q.slice, row, sum);
     }
     return true;
 }
 
 // Row sum in stored order; returns false for an empty row.
-template <int W, class T, class V>
-__device__ inline bool row_dot(const EllView &A, const V *__restrict__ val, const Operand<T> &x, int row, T &sum) {
+template <int W, class T, class V, class X>
+__device__ inline bool row_dot(const EllView &A, const V *__restrict__ val, const X &x, int row, T &sum) {
     sum = T(0);
     if (W < 0) {
-        return stencil_row<(W < 0 ? -W : 0), T, V>(A, val, x, row, sum);
+        return stencil_row<(W < 0 ? -W : 0), T, V, X>(A, val, x, row, sum);
     } else if (W == 0 && A.fmt == GIP_FMT_STENCIL) {
-        return stencil_row<0, T, V>(A, val, x, row, sum);
+        return stencil_row<0, T, V, X>(A, val, x, row, sum);
     } else if (W > 0) {
         const int base = A.soff[row >> 6] + (row & 63);
         int c[W > 0 ? W : 1];

@@ -141,3 +141,26 @@ def test_batch_mixed_precision_close_to_fp64_oracle(dev, tmp_path_factory):
             print(f"[batch mixed inst {k} {f}] rel {err:.2e}")
             assert err <= 1e-7, (k, f, err)
     print("[batch layouts]", b.layouts())
+
+
+def test_batch_single_instance_on_c100k_matches_oracle(dev, c100k_root):
+    """The shipped C100K case (tetrahedral cells, merged to one partition) as
+    a one-instance batch: the device-resident, graph-captured step against
+    the oracle with the reference's fixed 100 PCG iterations, to 1e-10."""
+    cav = cavity.read(c100k_root, "C100K")
+    b = batch.Batch(dev, [cav], [1e-3], pcg_tol=1e-300, pcg_maxit=100, precision=hip.PCG_FP64, poll=100,
+                    use_graph=True)
+    orc = pyoracle.Dns(c100k_root, "C100K", gamma=1e-3)
+    for step in range(3):
+        b.step()
+        w, g = orc.step(), b.info(0)
+        print(f"[C100K batch step {step + 1}] massr {g['massr']:.9e} dt {g['dt']:.9e} | oracle {w['massr']:.9e}",
+              flush=True)
+        assert g["pcg_iters"] == w["pcg_iters"] == 100
+        for key in ("time", "dt"):
+            assert _rel(g[key], w[key]) <= 1e-10, (step, key, g[key], w[key])
+        assert _massr_close(g["massr"], w["massr"], 1e-10), (step, g["massr"], w["massr"])
+    for f in ("u", "v", "w", "p", "mf"):
+        a, r = b.field(0, f), orc.field(f)
+        assert np.linalg.norm(a - r) <= 1e-10 * np.linalg.norm(r), f
+    b.close()
