@@ -79,8 +79,9 @@ struct PcgDev {
     int n, G;      // solved rows, workgroups of the sweeping kernels (<= kMaxRedBlocks)
     int nchunk;    // 256-row chunks: workgroups of the product kernels
     int Gdn;       // p.Ap partials the next kernels read (<= kMaxRedBlocks)
-    double *chunk; // nchunk per-chunk partials of the product (the P_DN
-                   // partials themselves when no fold is needed)
+    double *chunk; // partials of the product: with the fold, kWaves per
+                   // 256-row chunk (one per wavefront); without it, one per
+                   // chunk, and these are the P_DN partials themselves
     double *x, *r, *s, *Ap;
     const double *b, *dinv;
     float *r32, *s32, *Ap32, *xc32;
@@ -112,6 +113,13 @@ __device__ inline double early_load(const double *p) {
 }
 __device__ inline int conv_stamp(int k) { return ST_CONV | (k << 2); }
 __device__ inline bool leader() { return blockIdx.x == 0 && threadIdx.x == 0; }
+// Sum over the wavefront (shuffle tree, no LDS or barrier); valid in lane 0.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = v + __shfl_down(v, off, 64);
+    return v;
+}
+constexpr int kWaves = kBlock / 64;  // wavefronts per workgroup
 __device__ inline double rho_slot(const PcgDev &d, int k) { return d.sc[S_RHO0 + (k & 1)]; }
 // p.Ap partials of iteration k in the fused (small-operator) iteration
 __device__ inline double *dn_part(const PcgDev &d, int k) { return d.P((k & 1) ? P_DN2 : P_DN); }
@@ -265,8 +273,9 @@ __device__ inline void iter_spmv(const EllView &A, const PcgDev &d, const double
                 sum = Ap[row];  // empty row: Ap keeps its value
             acc = pr * sum;
         }
-        acc = block_reduce<RED_SUM>(acc);
-        if (threadIdx.x == 0) d.chunk[c] = acc;
+        // one partial per wavefront: no barrier, each wavefront retires alone
+        acc = wave_sum(acc);
+        if ((threadIdx.x & 63) == 0) d.chunk[c * kWaves + (threadIdx.x >> 6)] = acc;
         return;
     }
     const int st = early_load(d.state);
@@ -371,11 +380,18 @@ __global__ __launch_bounds__(kBlock) void k_iter_dirprod(EllView A, PcgDev d, co
     if (threadIdx.x == 0) dn_part(d, k)[c] = acc;
 }
 
-// the per-chunk partials, folded in index order into Gdn partials of P_DN
+// the per-wavefront partials of the long products, folded in index order:
+// thread i sums chunk i's kWaves partials, each workgroup its 256 chunks
+// into one of the Gdn partials of P_DN
 __global__ __launch_bounds__(kBlock) void k_fold_dn(PcgDev d) {
     if (!running(d)) return;
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    double v = i < d.nchunk ? d.chunk[i] : 0.0;
+    double v = 0.0;
+    if (i < d.nchunk) {
+        const double *w = d.chunk + (size_t)i * kWaves;
+#pragma unroll
+        for (int j = 0; j < kWaves; ++j) v = v + w[j];
+    }
     v = block_reduce<RED_SUM>(v);
     if (threadIdx.x == 0) d.P(P_DN)[blockIdx.x] = v;
 }
@@ -553,7 +569,10 @@ __global__ __launch_bounds__(kBlock) void k_m_dirprod(EllView A, PcgDev d, const
 template <int W, bool SMALL>
 __device__ inline void m_spmv(const EllView &A, const PcgDev &d, const float *__restrict__ p) {
     if constexpr (!SMALL) {
-        if (!running(d)) return;
+        // no state test: the product's stores (Ap32, its partials) are read
+        // only by the fold and the update, which test it; and mixed
+        // iterations are queued only while the solve runs (convergence is
+        // decided at a defect correction, after which nothing is queued)
         const int c = xcd_block(blockIdx.x, gridDim.x);
         const int row = c * kBlock + threadIdx.x;
         double acc = 0.0;
@@ -565,8 +584,8 @@ __device__ inline void m_spmv(const EllView &A, const PcgDev &d, const float *__
             d.Ap32[row] = sum;
             acc = (double)pr * (double)sum;
         }
-        acc = block_reduce<RED_SUM>(acc);
-        if (threadIdx.x == 0) d.chunk[c] = acc;
+        acc = wave_sum(acc);
+        if ((threadIdx.x & 63) == 0) d.chunk[c * kWaves + (threadIdx.x >> 6)] = acc;
         return;
     }
     const int st = early_load(d.state);
@@ -887,7 +906,7 @@ int gip_pcg_create(gip_ctx *c, const gip_mat *A, int32_t n_all, gip_pcg **out) {
     dalloc((void **)&S->dinv, n * 8);
     dalloc((void **)&S->sc, S_COUNT * 8);
     dalloc((void **)&S->part, (size_t)P_COUNT * kMaxRedBlocks * 8);
-    if (S->fold) dalloc((void **)&S->chunk, (size_t)S->nchunk * 8);
+    if (S->fold) dalloc((void **)&S->chunk, (size_t)S->nchunk * kWaves * 8);
     dalloc((void **)&S->state, 2 * sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_state, 2 * sizeof(int), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_sc, S_COUNT * 8, hipHostMallocDefault);
