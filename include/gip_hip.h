@@ -153,8 +153,11 @@ int gip_dnrm_inf_dev(gip_ctx *ctx, int32_t n, const double *x, double *dresult);
 
 /* ---- fused Jacobi-PCG ----------------------------------------------------
  * GIP_LinearSolver::solve (GIP_LinearSolver.h:45-117) with every scalar
- * kept on the device: per iteration one SpMV+dot kernel and one
- * update+precondition+dot kernel; the host only polls a convergence flag.
+ * kept on the device: per iteration a direction kernel, a product kernel
+ * (with its p.Ap partials) and an update kernel (r, s = Dinv r and the
+ * partials of r.s and r.r); on operators of at most 2^19 rows the direction
+ * is formed inside the product (two kernels per iteration).  The host only
+ * reads the solver's state word every `poll` iterations.
  * precision GIP_PCG_FP64: the reference recurrences in FP64.
  * precision GIP_PCG_MIXED: FP32 operator values, vectors and SpMV inside an
  *   FP64 defect-correction loop: the FP64 true residual r = b - A x is
