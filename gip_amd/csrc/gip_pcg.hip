@@ -1025,11 +1025,19 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
     k_setup_fin<<<1, kBlock, 0, st>>>(d, o->tol);
     GIP_LAUNCH_CHECK("pcg setup");
     int corrections = 0, rc;
-    if ((rc = poll(S, true))) return rc;
+    // FP64 with a fixed iteration count (tol <= 0): nothing can stop the
+    // solve early but a breakdown, after which every queued kernel returns
+    // at once, so all iterations are queued and the host reads the state
+    // once, at the end (every 4096 iterations on long runs)
+    const bool fixed = !mixed && !(o->tol > 0.0);
+    if (fixed)
+        S->h_state[0] = ST_RUN;
+    else if ((rc = poll(S, true)))
+        return rc;
     if (!mixed) {
         int k = 0;  // iterations queued
         while (k < o->maxit && S->h_state[0] == ST_RUN) {
-            int chunk = std::min(pollk, o->maxit - k);
+            int chunk = std::min(fixed ? 4096 : pollk, o->maxit - k);
             for (int j = 0; j < chunk; ++j, ++k) {
                 if (S->fold) {
                     if (k > 0) k_iter_dir<<<S->grid, kBlock, 0, st>>>(d, S->p, k);
@@ -1045,7 +1053,7 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
                 }
             }
             GIP_LAUNCH_CHECK("pcg iteration");
-            if ((rc = poll(S, false))) return rc;
+            if (!fixed && (rc = poll(S, false))) return rc;
         }
         if (k > 0) k_x_flush<<<S->grid, kBlock, 0, st>>>(d, pdir(k - 1), k - 1, pdn(k - 1));
         k_fin<<<1, kBlock, 0, st>>>(d);
