@@ -108,14 +108,20 @@ def ns_leg(dns, cav, mode, steps, warmup):
     sim.sync()
     t0 = time.perf_counter()
     its = []
+    phases = {k: 0.0 for k in sim.PHASES}
     for k in range(steps):
         s = sim.step()
         its.append(s["pcg_iters"])
+        # device time per phase (events at the phase boundaries; read after
+        # the step, whose CFL phase has already synchronised)
+        for name, ms in sim.phase_ms().items():
+            phases[name] += ms / steps
     sim.sync()
     dt = time.perf_counter() - t0
     fields = {k: sim.field(k) for k in ("u", "v", "w", "p")}
     sim.close()
-    return {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps, "pcg_iters_mean": float(np.mean(its))}, fields
+    return {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps, "pcg_iters_mean": float(np.mean(its)),
+            "phase_ms": {k: round(v, 3) for k, v in phases.items()}}, fields
 
 
 def spmv32_bytes(nnz, rows, cols):

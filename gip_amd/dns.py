@@ -28,6 +28,7 @@ def lib():
         L.gip_dns_open.argtypes = [ctypes.c_char_p, ctypes.c_char_p, POINTER(Config), POINTER(c_void_p)]
         L.gip_dns_from_cavity.argtypes = [c_void_p, POINTER(Config), POINTER(c_void_p)]
         L.gip_dns_step.argtypes = [c_void_p, POINTER(StepInfo)]
+        L.gip_dns_phase_ms.argtypes = [c_void_p, POINTER(c_double)]
         L.gip_dns_get_field.argtypes = [c_void_p, ctypes.c_char_p, c_void_p, c_int32]
         L.gip_dns_set_field.argtypes = [c_void_p, ctypes.c_char_p, c_void_p, c_int32]
         L.gip_dns_sizes.argtypes = [c_void_p, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]
@@ -68,6 +69,14 @@ class Simulation:
         _check(lib().gip_dns_step(self.h, byref(s)))
         return {"time": s.time, "dt": s.dt, "massr": s.massr, "step": s.step, "pcg_iters": s.pcg_iters,
                 "pcg_status": s.pcg_status, "pcg_relres": s.pcg_relres, "pcg_corrections": s.pcg_corrections}
+
+    PHASES = ("predictor", "poisson", "correction", "mass_fluxes", "cfl")
+
+    def phase_ms(self):
+        """Device time of each phase of the last step (ms), keyed by PHASES."""
+        out = (c_double * 5)()
+        _check(lib().gip_dns_phase_ms(self.h, out))
+        return dict(zip(self.PHASES, list(out)))
 
     def field(self, name):
         n = self.n_faces if name == "mf" else self.n_all

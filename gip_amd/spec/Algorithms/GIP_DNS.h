@@ -13,12 +13,15 @@
 // Additions: the constants the reference hardcodes (GIP_DNS.h:132-137,
 // :180-185) come from configure(); setUp(const gip_cavity*) takes the
 // operators from memory instead of InputFiles; step() runs one step and
-// reports its numbers.  Files are read for one partition
-// (<name>_1p/...-0), relative to root().
+// reports its numbers; an optional hook is called at the six phase
+// boundaries of a step (0 before the predictor ... 5 after the CFL phase),
+// for timing.  Files are read for one partition (<name>_1p/...-0),
+// relative to root().
 #ifndef _gip_dns_
 #define _gip_dns_
 
 #include <cstdio>
+#include <functional>
 #include <iostream>
 #include <stdexcept>
 #include <string>
@@ -47,6 +50,8 @@ class GIP_DNS {
     void setUp(const gip_cavity *cav);
     void integrate();
     void step(GIP_StepInfo *info);
+    // called with 0..5 at the phase boundaries of every step (empty: none)
+    void setPhaseHook(std::function<void(int)> h) { phaseHook = std::move(h); }
 
     Vector &field(const std::string &f);
     int innerSize() { return topoCols.getInnerSize(); }
@@ -75,6 +80,10 @@ class GIP_DNS {
     std::string path(const char *kind, const char *group, const char *op, const char *ext);
     void allocateFields();
     void setUpSolver();
+    std::function<void(int)> phaseHook;
+    void mark(int phase) {
+        if (phaseHook) phaseHook(phase);
+    }
     void PredictorVelocity();
     void PoissonEquation();
     void VelocityCorrection();
@@ -193,11 +202,17 @@ void GIP_DNS<Matrix, Vector, Node, NonLinear>::step(GIP_StepInfo *info) {
         sigma = 1.0 / dt;
         started = true;
     }
+    mark(0);
     PredictorVelocity();
+    mark(1);
     PoissonEquation();
+    mark(2);
     VelocityCorrection();
+    mark(3);
     MassFluxes();
+    mark(4);
     CFLCondition(info);
+    mark(5);
     ++it;
 }
 

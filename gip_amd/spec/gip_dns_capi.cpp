@@ -1,6 +1,7 @@
 // C ABI of GIP_DNS instantiated with the HIP specification (include/gip_dns.h).
 #include <cstring>
 #include <exception>
+#include <memory>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -13,7 +14,24 @@ typedef GIP_DNS<GIP_MatrixHIP, GIP_VectorHIP, GIP_ArchHIP, GIP_NonLinearHIP> Hip
 
 struct gip_dns {
     HipDNS sim;
+    gip_event *ev[6] = {};  // device events at the phase boundaries of the last step
+    bool timed = false;
     explicit gip_dns(const char *n) : sim(n) {}
+    ~gip_dns() {
+        for (gip_event *e : ev)
+            if (e) gip_event_destroy(e);
+    }
+    // events recorded on the node's stream at each phase boundary: no
+    // synchronisation inside the step; read after it
+    void attach_timer() {
+        gip_ctx *c = sim.node().ctx();
+        for (gip_event *&e : ev)
+            if (gip_event_create(c, &e) != GIP_OK) throw std::runtime_error("event create failed");
+        sim.setPhaseHook([this, c](int ph) {
+            gip_event_record(c, ev[ph]);
+            if (ph == 5) timed = true;
+        });
+    }
 };
 
 static thread_local std::string g_err;
@@ -52,11 +70,12 @@ const char *gip_dns_last_error(void) { return g_err.c_str(); }
 int gip_dns_open(const char *root, const char *name, const gip_dns_config *cfg, gip_dns **out) {
     if (!name || !out) return fail(GIP_EINVAL, "name and out required");
     try {
-        gip_dns *s = new gip_dns(name);
+        std::unique_ptr<gip_dns> s(new gip_dns(name));
         s->sim.setRoot(root ? root : "");
         s->sim.configure(params_of(cfg));
         s->sim.setUp();
-        *out = s;
+        s->attach_timer();
+        *out = s.release();
         return GIP_OK;
     } catch (const std::exception &e) {
         return fail(GIP_EINVAL, e.what());
@@ -66,10 +85,11 @@ int gip_dns_open(const char *root, const char *name, const gip_dns_config *cfg, 
 int gip_dns_from_cavity(const gip_cavity *cav, const gip_dns_config *cfg, gip_dns **out) {
     if (!cav || !out) return fail(GIP_EINVAL, "cavity and out required");
     try {
-        gip_dns *s = new gip_dns("memory");
+        std::unique_ptr<gip_dns> s(new gip_dns("memory"));
         s->sim.configure(params_of(cfg));
         s->sim.setUp(cav);
-        *out = s;
+        s->attach_timer();
+        *out = s.release();
         return GIP_OK;
     } catch (const std::exception &e) {
         return fail(GIP_EINVAL, e.what());
@@ -130,6 +150,18 @@ int gip_dns_sizes(gip_dns *s, int32_t *inner, int32_t *all, int32_t *faces) {
 int gip_dns_sync(gip_dns *s) {
     if (!s) return fail(GIP_EINVAL, "null simulation");
     return gip_sync(s->sim.node().ctx());
+}
+
+int gip_dns_phase_ms(gip_dns *s, double *ms) {
+    if (!s || !ms) return fail(GIP_EINVAL, "null argument");
+    if (!s->timed) return fail(GIP_EINVAL, "no step taken yet");
+    for (int i = 0; i < 5; ++i) {
+        float t = 0.0f;
+        int rc = gip_event_elapsed_ms(s->ev[i], s->ev[i + 1], &t);
+        if (rc) return fail(rc, "event elapsed failed");
+        ms[i] = t;
+    }
+    return GIP_OK;
 }
 
 int gip_dns_close(gip_dns *s) {

@@ -46,6 +46,13 @@ int gip_dns_open(const char *root, const char *name, const gip_dns_config *cfg, 
 /* Case from an in-memory cavity (gip_cavity_generate / gip_cavity_read). */
 int gip_dns_from_cavity(const gip_cavity *cav, const gip_dns_config *cfg, gip_dns **out);
 int gip_dns_step(gip_dns *s, gip_dns_step_info *info);
+/* Device time of the five phases of the last step, in ms: predictor,
+ * Poisson (right-hand side, PCG, ghost shuffle), velocity correction, mass
+ * fluxes, CFL (GIP_DNS.h:308-414), from events recorded on the stream at
+ * the phase boundaries (no synchronisation inside the step).  Not in the
+ * reference, which has no per-phase timer; GIP_Timer (GIP_Timer.h:6-26)
+ * times whole channels. */
+int gip_dns_phase_ms(gip_dns *s, double *ms);
 /* Copy a field ("u","v","w","p","mf","p0","ax0","ay0","az0","rhs") to host memory; n = its length. */
 int gip_dns_get_field(gip_dns *s, const char *field, double *host, int32_t n);
 int gip_dns_set_field(gip_dns *s, const char *field, const double *host, int32_t n);

@@ -8,6 +8,7 @@ counts within +-1; the README golden lines for C100K.
 """
 import json
 import subprocess
+import time
 
 import numpy as np
 import pytest
@@ -130,6 +131,29 @@ def test_dns_tolerance_mode_iteration_counts(cav46_dir, mode):
         err = _field_rel(sim.field(f), want[f])
         print(f"[tol-mode field {f}] rel L2 {err:.3e}", flush=True)
         assert err <= 1e-7, (f, err)
+
+
+def test_dns_phase_times(cav46_dir):
+    """Per-phase device times of a step (events at the phase boundaries):
+    all positive, the Poisson phase (100 fixed PCG iterations) the largest,
+    and together no longer than the step's wall time."""
+    sim = dns.Simulation(root=cav46_dir, name="cav46", gamma=1e-2, pcg_mode="fused")
+    with pytest.raises(dns.DnsError):
+        sim.phase_ms()  # no step yet
+    sim.step()
+    for _ in range(3):
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step()
+        sim.sync()
+        wall_ms = 1e3 * (time.perf_counter() - t0)
+        ph = sim.phase_ms()
+        print(f"[phases] {ph} wall {wall_ms:.3f} ms", flush=True)
+        assert set(ph) == set(dns.Simulation.PHASES)
+        assert all(v > 0.0 for v in ph.values())
+        assert ph["poisson"] == max(ph.values())
+        assert sum(ph.values()) <= 1.02 * wall_ms
+    sim.close()
 
 
 def test_dns_mixed_precision_error_per_field(cav46_dir):
