@@ -8,8 +8,9 @@ one Jacobi-PCG solve of that system to a 1e-10 FP64 relative true residual
 ||b - A x||_2/||b||_2 with the mixed-precision solver (FP32 operator,
 vectors and SpMV inside the FP64 defect-correction loop); `value` is its
 time in ms.  Reported beside it: the same solve with the FP64 solver, the
-FP64 SpMV HBM rate on the Laplacian (the dominant kernel, `roofline`), the
-time-step rate of full Navier–Stokes steps at 256^3 (Re=1000, PCG to 1e-10,
+FP64 SpMV HBM rate on the Laplacian (`roofline`), the FP32 product of the
+timed mixed solves (the solve's largest kernel, timed inside the solves:
+`roofline_solve`), the time-step rate of full Navier–Stokes steps at 256^3 (Re=1000, PCG to 1e-10,
 FP64 vs mixed), and the reference CPU path's PCG on the same system
 (`cpu_baseline`).  Inputs are resident in HBM before the timed region.
 
@@ -27,8 +28,9 @@ Other suites (one JSON line each, not the default):
     --suite clustered 128^3 tanh-clustered mesh (beta 1.5), Re=1000, 100 steps,
                       PCG to 1e-10: steps/s FP64 vs mixed, the L SpMV rate and
                       the FP64/mixed PCG on a zero-mean N(0,1) right-hand side
-    --probe spmv      a few launches of the FP64 SpMV and of a plain copy of
-                      known size, for rocprofv3 --pmc traffic runs
+    --probe spmv      a few launches of the FP64 SpMV, eight iterations of each
+                      PCG, and a plain copy of known size, for rocprofv3 --pmc
+                      traffic runs
 """
 from __future__ import annotations
 
@@ -85,20 +87,25 @@ def pcg_leg(hip, dev, A, b_dev, n, n_all, precision, steps, warmup, maxit, bnorm
     z = dev.array(n_all)
     hip.fill(z, 0.0)
     e0, e1 = hip.Event(dev), hip.Event(dev)
-    times, res = [], None
+    times, prod, res = [], [], None
     for k in range(warmup + steps):
+        # time_products: one product launch per host poll (every 32
+        # iterations) between two events on the solver's stream
         e0.record()
-        res = solver.solve(b_dev, x, z, maxit=maxit, tol=1e-10, precision=precision, inner_tol=0.1, poll=32)
+        res = solver.solve(b_dev, x, z, maxit=maxit, tol=1e-10, precision=precision, inner_tol=0.1, poll=32,
+                           time_products=True)
         e1.record()
         ms = e0.elapsed_ms(e1)
         if k >= warmup:
             times.append(ms)
+            prod.append(res.product_ms)
         log(f"  pcg {'mixed' if precision else 'fp64'} solve {k}: {ms:.1f} ms, {res.iters} it, "
             f"{res.corrections} corr, status {res.status}")
     rel = true_relres(hip, dev, A, b_dev, x, n, n_all, bnorm)
     solver.destroy()
     return {"ms": float(np.mean(times)), "ms_min": float(np.min(times)), "iters": res.iters,
-            "corrections": res.corrections, "true_relres": float(rel), "status": res.status}
+            "corrections": res.corrections, "true_relres": float(rel), "status": res.status,
+            "product_us": 1e3 * float(np.median(prod)), "product_samples_per_solve": res.product_samples}
 
 
 def ns_leg(dns, cav, mode, steps, warmup):
@@ -129,11 +136,12 @@ def spmv32_bytes(nnz, rows, cols):
     return 8 * nnz + 4 * (rows + 1) + 4 * rows + 4 * cols
 
 
-def measured_traffic(n, layout):
-    """HBM bytes per launch of the FP64 SpMV at n^3 in this layout, from the
-    committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by a
-    copy kernel of known size)."""
-    f = ROOT / "profiles" / f"spmv_{layout}_fp64_{n}_traffic.txt"
+def measured_traffic(n, layout, which="fp64"):
+    """HBM bytes per launch of the FP64 SpMV at n^3 in this layout ("fp32_solve":
+    the FP32 product inside the mixed PCG), from the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known
+    size)."""
+    f = ROOT / "profiles" / f"spmv_{layout}_{which}_{n}_traffic.txt"
     if not f.exists():
         return None
     for line in f.read_text().splitlines():
@@ -299,15 +307,25 @@ def main():
                      "kernel": {4: f"k_stencil<0,-{info.get('width', 7)}> (FP64 stencil-layout SpMV)",
                                 2: "k_ell<0,7> (FP64 sliced-ELL SpMV)"}.get(info["format"], "FP64 SpMV")
                                + f", {n}^3 Laplacian"},
+        # the dominant kernel of the timed mixed solve: its FP32 product,
+        # timed in the solve (events around one launch per host poll)
+        "roofline_solve": {
+            "bound": "hbm", "achieved": round(alg32 / (mixed["product_us"] * 1e-6) / 1e9, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(alg32 / (mixed["product_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(n, hip.FORMAT_NAMES.get(info["format"], "?"), "fp32_solve"),
+            "algorithmic_bytes": alg32, "us_per_launch": round(mixed["product_us"], 2),
+            "kernel": f"k_m_spmv (FP32 product of the mixed PCG, timed inside the solve), {n}^3 Laplacian"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
 
 def probe_spmv(args):
-    """Five FP64 SpMVs on the n^3 Laplacian and five copies of 2^27 doubles
-    (1 GiB read + 1 GiB written each): the launches a rocprofv3 --pmc run
-    counts, the copy calibrating FETCH_SIZE/WRITE_SIZE against known bytes."""
+    """Five FP64 SpMVs on the n^3 Laplacian, eight iterations of the FP64 and
+    of the mixed PCG on it (their in-solve products), and five copies of 2^27
+    doubles (1 GiB read + 1 GiB written each): the launches a rocprofv3 --pmc
+    run counts, the copy calibrating FETCH_SIZE/WRITE_SIZE against known
+    bytes."""
     from gip_amd import cavity, hip
     cav = cavity.generate(args.n, gamma=1e-3, ops=cavity.OP_BITS["L"])
     L = cav.op("L")
@@ -318,6 +336,15 @@ def probe_spmv(args):
     y = dev.array(cav.n_all)
     for _ in range(5):
         A.spmv(x, y)
+    # 8 iterations of each PCG (their products: k_iter_spmv, k_m_spmv)
+    A32 = hip.Matrix.from_csr(dev, L, fp32=True)
+    solver = hip.PCG(A32, cav.n_all)
+    b = dev.to_device(np.concatenate([cavity.rhs_normal(cav.n_inner, 1), np.zeros(cav.n_all - cav.n_inner)]))
+    z = dev.to_device(np.zeros(cav.n_all))
+    for prec in (hip.PCG_FP64, hip.PCG_MIXED):
+        solver.solve(b, y, z, maxit=8, tol=1e-10, precision=prec, poll=32)
+    solver.destroy()
+    A32.destroy()
     nc = 1 << 27
     a, b = dev.array(nc), dev.array(nc)
     hip.fill(a, 1.0)

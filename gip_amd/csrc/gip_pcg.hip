@@ -45,8 +45,10 @@
 // (eigenvalue ~ pin/N), which FP32 arithmetic cannot resolve; left to the
 // FP32 iteration it re-enters through round-off and the corrections
 // diverge.  For an SPD operator the step never increases the A-norm error.
+#include <algorithm>
 #include <cmath>
 #include <new>
+#include <vector>
 
 #include "gip_blas1.h"
 #include "gip_internal.h"
@@ -814,6 +816,7 @@ struct gip_pcg {
     int *state;    // state, iters
     int *h_state;  // pinned copy
     double *h_sc;  // pinned scalars
+    hipEvent_t tev0, tev1;  // time_products: the sampled product launch
 };
 
 // Multiple of 8 (one share per XCD), at most kMaxRedBlocks (one partial each).
@@ -942,6 +945,8 @@ int gip_pcg_destroy(gip_pcg *S) {
         if (p) hipFree(p);
     if (S->h_state) hipHostFree(S->h_state);
     if (S->h_sc) hipHostFree(S->h_sc);
+    if (S->tev0) hipEventDestroy(S->tev0);
+    if (S->tev1) hipEventDestroy(S->tev1);
     delete S;
     return GIP_OK;
 }
@@ -1019,6 +1024,31 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
     auto pdn = [&](int k) {
         return (const double *)(S->part + (size_t)((!S->fold && (k & 1)) ? P_DN2 : P_DN) * kMaxRedBlocks);
     };
+    // time_products: events around one product launch (k >= 1) per host
+    // poll, read after the poll's synchronisation
+    const bool timing = o->time_products != 0;
+    if (timing && !S->tev0) {
+        GIP_HIP_TRY(hipEventCreate(&S->tev0));
+        GIP_HIP_TRY(hipEventCreate(&S->tev1));
+    }
+    std::vector<float> samples;
+    bool armed = false;  // a sampled pair is recorded and not read yet
+    auto sample_begin = [&](int k) {
+        const bool on = timing && !armed && k > 0;
+        if (on) hipEventRecord(S->tev0, st);
+        return on;
+    };
+    auto sample_end = [&](bool on) {
+        if (on) {
+            hipEventRecord(S->tev1, st);
+            armed = true;
+        }
+    };
+    auto sample_read = [&]() {  // after a host synchronisation
+        float ms = 0.0f;
+        if (armed && hipEventElapsedTime(&ms, S->tev0, S->tev1) == hipSuccess) samples.push_back(ms);
+        armed = false;
+    };
     // setup, GIP_LinearSolver.h:55-80
     DISPATCH_W(A, k_setup_resid, v, d, x0, S->n_all, S->p, S->p2);
     k_setup_dir<<<S->grid, kBlock, 0, st>>>(d, S->p);
@@ -1041,19 +1071,26 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
             for (int j = 0; j < chunk; ++j, ++k) {
                 if (S->fold) {
                     if (k > 0) k_iter_dir<<<S->grid, kBlock, 0, st>>>(d, S->p, k);
+                    const bool smp = sample_begin(k);
                     DISPATCH_WG(A, S->nchunk, k_iter_spmv, v, d, S->p);
+                    sample_end(smp);
                     k_fold_dn<<<S->gdn, kBlock, 0, st>>>(d);
                     k_iter_update<false><<<S->grid, kBlock, 0, st>>>(d, k);
                 } else {
+                    const bool smp = sample_begin(k);
                     if (k > 0)
                         DISPATCH_WG(A, S->nchunk, k_iter_dirprod, v, d, pdir(k - 1), pdir(k), k);
                     else
                         DISPATCH_WG(A, S->nchunk, k_iter_spmv_small, v, d, S->p);
+                    sample_end(smp);
                     k_iter_update<true><<<S->grid, kBlock, 0, st>>>(d, k);
                 }
             }
             GIP_LAUNCH_CHECK("pcg iteration");
-            if (!fixed && (rc = poll(S, false))) return rc;
+            if (!fixed) {
+                if ((rc = poll(S, false))) return rc;
+                sample_read();
+            }
         }
         if (k > 0) k_x_flush<<<S->grid, kBlock, 0, st>>>(d, pdir(k - 1), k - 1, pdn(k - 1));
         k_fin<<<1, kBlock, 0, st>>>(d);
@@ -1075,14 +1112,18 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
             for (int j = 0; j < chunk; ++j, ++k) {
                 if (S->fold) {
                     if (k > 0) k_m_dir<<<S->grid, kBlock, 0, st>>>(d, S->p32, k, pending);
+                    const bool smp = sample_begin(k);
                     DISPATCH_WG(A, S->nchunk, k_m_spmv, v, d, S->p32);
+                    sample_end(smp);
                     k_fold_dn<<<S->gdn, kBlock, 0, st>>>(d);
                     k_m_update<false><<<S->grid, kBlock, 0, st>>>(d, k);
                 } else {
+                    const bool smp = sample_begin(k);
                     if (k > 0)
                         DISPATCH_WG(A, S->nchunk, k_m_dirprod, v, d, pdir32(k - 1), pdir32(k), k, pending);
                     else
                         DISPATCH_WG(A, S->nchunk, k_m_spmv_small, v, d, S->p32);
+                    sample_end(smp);
                     k_m_update<true><<<S->grid, kBlock, 0, st>>>(d, k);
                 }
                 pending = 1;
@@ -1090,6 +1131,7 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
             k_m_status<<<1, kBlock, 0, st>>>(d);
             GIP_LAUNCH_CHECK("mixed iteration");
             if ((rc = poll(S, true))) return rc;
+            sample_read();
             if (S->h_state[0] != ST_RUN) break;
             const double rr32 = S->h_sc[S_RR], tol2 = S->h_sc[S_TOL2];
             const bool near_target = tol2 >= 0.0 && rr32 <= tol2;
@@ -1107,6 +1149,7 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
         }
     }
     if ((rc = poll(S, true))) return rc;
+    sample_read();
     int status = GIP_OK;
     const int fin = S->h_state[0] & ST_MASK;
     if (fin == ST_BREAK)
@@ -1119,6 +1162,12 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
         const double bb = S->h_sc[S_BB], rr = S->h_sc[S_RR];
         stats->relres = bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
         stats->status = status;
+        stats->product_samples = (int32_t)samples.size();
+        stats->product_ms = 0.0;
+        if (!samples.empty()) {  // median (a launch that found the solve finished is short)
+            std::nth_element(samples.begin(), samples.begin() + samples.size() / 2, samples.end());
+            stats->product_ms = samples[samples.size() / 2];
+        }
     }
     return status == GIP_EBREAKDOWN ? set_error(GIP_EBREAKDOWN, "PCG breakdown (p.Ap == 0 or non-finite step)") : GIP_OK;
 }

@@ -23,11 +23,12 @@ OK, EINVAL, EHIP, ENOMEM, EBREAKDOWN, ENOCONV = 0, -1, -2, -3, -4, -5
 
 class PcgOpts(ctypes.Structure):
     _fields_ = [("maxit", c_int32), ("tol", c_double), ("precond", c_int32), ("precision", c_int32),
-                ("inner_tol", c_double), ("poll", c_int32), ("coarse", c_int32)]
+                ("inner_tol", c_double), ("poll", c_int32), ("coarse", c_int32), ("time_products", c_int32)]
 
 
 class PcgStats(ctypes.Structure):
-    _fields_ = [("iters", c_int32), ("corrections", c_int32), ("relres", c_double), ("status", c_int32)]
+    _fields_ = [("iters", c_int32), ("corrections", c_int32), ("relres", c_double), ("status", c_int32),
+                ("product_ms", c_double), ("product_samples", c_int32)]
 
 
 class TuneReport(ctypes.Structure):
@@ -346,6 +347,8 @@ class PcgResult:
     corrections: int
     relres: float
     status: int
+    product_ms: float = 0.0  # time_products: median sampled product launch (ms)
+    product_samples: int = 0
 
 
 class PCG:
@@ -359,15 +362,16 @@ class PCG:
         self.dev._adopt(self)
 
     def solve(self, b, x, x0, *, maxit=100, tol=0.0, precond=True, precision=PCG_FP64, inner_tol=0.1,
-              poll=16, coarse=True, raise_on_breakdown=True):
+              poll=16, coarse=True, raise_on_breakdown=True, time_products=False):
         if b.n < self.A.nrows or x.n < self.n_all or x0.n < self.n_all:
             raise ValueError("PCG vectors shorter than the solver's layout")
-        o = PcgOpts(maxit, tol, 1 if precond else 0, precision, inner_tol, poll, 0 if coarse else -1)
+        o = PcgOpts(maxit, tol, 1 if precond else 0, precision, inner_tol, poll, 0 if coarse else -1,
+                    1 if time_products else 0)
         st = PcgStats()
         rc = lib().gip_pcg_solve(self.ptr, b.ptr, x.ptr, x0.ptr, byref(o), byref(st))
         if rc != 0 and (raise_on_breakdown or rc != EBREAKDOWN):
             check(rc)
-        return PcgResult(st.iters, st.corrections, st.relres, st.status)
+        return PcgResult(st.iters, st.corrections, st.relres, st.status, st.product_ms, st.product_samples)
 
     def destroy(self):
         if self.ptr and self.dev.ctx:

@@ -179,12 +179,19 @@ typedef struct gip_pcg_opts {
                           of the solved rows (the near-null mode of a pinned
                           Neumann Laplacian, which FP32 cannot resolve):
                           0 on (default), -1 off                                  */
+    int32_t time_products; /* 1: time one product launch per host poll with events on
+                              the solver's stream (read at the poll's synchronisation);
+                              the median is returned in gip_pcg_stats.product_ms      */
 } gip_pcg_opts;
 typedef struct gip_pcg_stats {
     int32_t iters;       /* iterations done                                     */
     int32_t corrections; /* FP64 true-residual corrections (MIXED)              */
     double relres;       /* ||r||_2/||b||_2 at exit (recursive for FP64)        */
     int32_t status;      /* GIP_OK, GIP_EBREAKDOWN or GIP_ENOCONV               */
+    double product_ms;   /* time_products: median duration of the sampled product
+                            launches (the SpMV kernel of an iteration; with the
+                            direction fused in on operators of <= 2^19 rows)     */
+    int32_t product_samples; /* launches sampled                                  */
 } gip_pcg_stats;
 int gip_pcg_create(gip_ctx *ctx, const gip_mat *A, int32_t n_all, gip_pcg **solver);
 /* Solves rows [0,nrows(A)) of A x = b; x0 is copied into x first; vectors are

@@ -308,6 +308,28 @@ def test_pcg_tolerance_iteration_count_within_one(dev, cav46, tol):
     assert np.linalg.norm(dx.download() - want) <= bound * np.linalg.norm(want)
 
 
+@pytest.mark.parametrize("precision", [hip.PCG_FP64, hip.PCG_MIXED])
+def test_pcg_product_timing_changes_nothing(dev, cav46, precision):
+    """time_products records events around one product launch per host
+    poll: the iterates and counts are bitwise those of an untimed solve, and
+    the sampled durations are positive."""
+    L, n, nall, b, A = _pcg_setup(dev, cav46)
+    solver = hip.PCG(A, nall)
+    x0 = dev.to_device(np.zeros(nall))
+    outs = []
+    for timed in (False, True):
+        dx = dev.array(nall)
+        r = solver.solve(dev.to_device(b), dx, x0, maxit=5000, tol=1e-9, precision=precision, poll=8,
+                         time_products=timed)
+        outs.append((r, dx.download()))
+    (r0, x_plain), (r1, x_timed) = outs
+    print(f"[timed products] {r1.product_samples} samples, median {1e3 * r1.product_ms:.2f} us", flush=True)
+    assert r0.status == r1.status == 0 and r0.iters == r1.iters and r0.corrections == r1.corrections
+    assert np.array_equal(x_plain, x_timed)
+    assert r0.product_samples == 0 and r0.product_ms == 0.0
+    assert r1.product_samples >= r1.iters // 8 - 1 and r1.product_ms > 0.0
+
+
 def test_pcg_mixed_precision_reaches_fp64_residual(dev, cav46):
     L, n, nall, b, A = _pcg_setup(dev, cav46)
     x0 = np.zeros(nall)
