@@ -208,6 +208,27 @@ __device__ inline void x_update(const PcgDev &d, const double *__restrict__ p, d
     }
 }
 
+// A product operand that also keeps the value it reads at the row's own
+// column (the p_i of p.Ap, which the row's diagonal entry reads anyway), so
+// that the product does not load it a second time (FP64 product: 1.8%
+// faster at 256^3; the FP32 product keeps the separate load, which measured
+// faster there).
+template <class T>
+struct SelfOperand {
+    const T *__restrict__ p;
+    int row;
+    mutable T self;
+    mutable bool found;
+    __device__ inline T at(int j) const {
+        const T v = p[j];
+        if (j == row) {
+            self = v;
+            found = true;
+        }
+        return v;
+    }
+};
+
 // Kernel prologues.  SMALL (operators of at most 2^19 rows, where every
 // thread holds at most one row and a kernel lasts a few microseconds): the
 // state word, the scalars, the partials and the row's operands are requested
@@ -265,14 +286,14 @@ __device__ inline void iter_spmv(const EllView &A, const PcgDev &d, const double
         const int row = c * kBlock + threadIdx.x;
         double acc = 0.0;
         if (row < d.n) {
-            Operand<double> po{p};
+            const SelfOperand<double> po{p, row, 0.0, false};
             double *__restrict__ Ap = d.Ap;
-            const double pr = p[row];
             double sum;
             if (row_dot<W, double, double>(A, A.val, po, row, sum))
                 Ap[row] = sum;
             else
                 sum = Ap[row];  // empty row: Ap keeps its value
+            const double pr = po.found ? po.self : p[row];
             acc = pr * sum;
         }
         // one partial per wavefront: no barrier, each wavefront retires alone
@@ -579,6 +600,7 @@ __device__ inline void m_spmv(const EllView &A, const PcgDev &d, const float *__
         const int row = c * kBlock + threadIdx.x;
         double acc = 0.0;
         if (row < d.n) {
+            // (a SelfOperand here measured 9% slower than the separate load)
             Operand<float> po{p};
             const float pr = p[row];
             float sum;
