@@ -138,3 +138,35 @@ def test_pcg_either_side_of_the_partial_fold(dev, n):
     r = solver.solve(dev.to_device(b), dx, dev.to_device(x0), maxit=1000, tol=1e-10, precision=hip.PCG_MIXED)
     x = dx.download()
     assert r.status == 0 and np.linalg.norm(b - T @ x) <= 1.01e-10 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("precision", [hip.PCG_FP64, hip.PCG_MIXED])
+def test_pcg_long_path_layouts_give_identical_iterates(dev, precision):
+    """81^3 cavity (531,441 rows: the three-kernel iteration with the fold):
+    stencil, ELL and CSR operators sum each row in stored order, so the
+    iterates agree bitwise whichever layout holds L; the FP64 ones also
+    against the oracle to 1e-10."""
+    c = cavity.generate(81, gamma=1e-3, ops=cavity.OP_BITS["L"])
+    L = c.op("L")
+    n, nall = c.n_inner, c.n_all
+    assert n > 2 ** 19
+    b = np.zeros(nall)
+    b[:n] = cavity.rhs_normal(n, 3)
+    z = dev.to_device(np.zeros(nall))
+    outs = []
+    for fmt in (hip.FMT_STENCIL, hip.FMT_ELL, hip.FMT_CSR):
+        A = hip.Matrix.from_csr(dev, L, fmt=fmt, fp32=True)
+        assert A.info()["format"] == fmt
+        solver = hip.PCG(A, nall)
+        dx = dev.array(nall)
+        r = solver.solve(dev.to_device(b), dx, z, maxit=60, precision=precision)
+        outs.append((dx.download(), r.iters, r.corrections))
+        solver.destroy()
+        A.destroy()
+    for x, it, corr in outs[1:]:
+        assert it == outs[0][1] and corr == outs[0][2]
+        assert np.array_equal(x, outs[0][0])
+    if precision == hip.PCG_FP64:
+        want, st, it, rel = pyoracle.pcg(pyoracle.Csr.of(L), b, np.zeros(nall), n, 60)
+        assert outs[0][1] == it == 60
+        assert np.linalg.norm(outs[0][0] - want) <= 1e-10 * np.linalg.norm(want)
