@@ -406,6 +406,47 @@ def test_pcg_breakdown_is_reported(dev, cav46):
     assert r.iters == 0 and r.status == 0
 
 
+def test_pcg_argument_errors_and_statuses(dev, cav46):
+    """The PCG's C ABI rejects bad arguments with GIP_EINVAL (nothing is
+    launched) and reports a solve that stops at maxit above its tolerance as
+    GIP_ENOCONV in its status."""
+    from ctypes import byref, c_void_p
+    L = cav46.op("L")
+    n, nall = cav46.n_inner, cav46.n_all
+    b = np.zeros(nall)
+    b[:n] = cavity.rhs_normal(n, 2)
+    lib = hip.lib()
+    A64 = hip.Matrix.from_csr(dev, L)  # no FP32 copy
+    solver = hip.PCG(A64, nall)
+    db, dx, z = dev.to_device(b), dev.array(nall), dev.to_device(np.zeros(nall))
+    st = hip.PcgStats()
+    good = hip.PcgOpts(10, 0.0, 1, hip.PCG_FP64, 0.1, 16, 0, 0)
+    assert lib.gip_pcg_solve(solver.ptr, None, dx.ptr, z.ptr, byref(good), byref(st)) == hip.EINVAL
+    assert lib.gip_pcg_solve(solver.ptr, db.ptr, dx.ptr, z.ptr, None, byref(st)) == hip.EINVAL
+    neg = hip.PcgOpts(-1, 0.0, 1, hip.PCG_FP64, 0.1, 16, 0, 0)
+    assert lib.gip_pcg_solve(solver.ptr, db.ptr, dx.ptr, z.ptr, byref(neg), byref(st)) == hip.EINVAL
+    mixed = hip.PcgOpts(10, 1e-8, 1, hip.PCG_MIXED, 0.1, 16, 0, 0)
+    assert lib.gip_pcg_solve(solver.ptr, db.ptr, dx.ptr, z.ptr, byref(mixed), byref(st)) == hip.EINVAL
+    assert b"FP32" in lib.gip_last_error()
+    out = c_void_p()
+    assert lib.gip_pcg_create(dev.ctx, A64.ptr, n - 1, byref(out)) == hip.EINVAL  # vectors shorter than the rows
+    # a tolerance not reached within maxit: status ENOCONV, the call itself succeeds
+    r = solver.solve(db, dx, z, maxit=5, tol=1e-12)
+    assert r.status == hip.ENOCONV and r.iters == 5
+    # Jacobi needs every diagonal entry: drop row 0's diagonal
+    rp, col, val = L.rowptr, L.col, L.val
+    rows = np.repeat(np.arange(rp.size - 1), np.diff(rp))
+    keep = ~((rows == 0) & (col == 0))
+    rp2 = np.concatenate([[0], np.cumsum(np.bincount(rows[keep], minlength=rp.size - 1))]).astype(np.int32)
+    A_nodiag = hip.Matrix(dev, rp2, col[keep], val[keep], ncols=nall)
+    s2 = hip.PCG(A_nodiag, nall)
+    with pytest.raises(hip.HipError) as e:
+        s2.solve(db, dx, z, maxit=5)
+    assert e.value.code == hip.EINVAL
+    r = s2.solve(db, dx, z, maxit=5, precond=False)  # unpreconditioned CG runs
+    assert r.iters == 5
+
+
 def test_invalid_arguments_are_rejected(dev):
     with pytest.raises(hip.HipError) as e:
         hip.Matrix(dev, np.array([0, 2]), np.array([0, -1]), np.array([1.0, 2.0]))
