@@ -3,18 +3,16 @@
 #include <cstdio>
 #include <cstdlib>
 #include <iostream>
+#include <stdexcept>
 
 void GIP_Topo::postConstruct(std::string filename) {
     FILE *fp = std::fopen(filename.c_str(), "r");
-    if (!fp) {
-        std::cerr << "GIP_Topo: cannot open topology " << filename << std::endl;
-        std::exit(2);
-    }
+    if (!fp) throw std::runtime_error("GIP_Topo: cannot open topology " + filename);
     int s[9];
     for (int i = 0; i < 9; ++i)
         if (std::fscanf(fp, " %d", &s[i]) != 1) {
-            std::cerr << "GIP_Topo: malformed topology " << filename << std::endl;
-            std::exit(2);
+            std::fclose(fp);
+            throw std::runtime_error("GIP_Topo: malformed topology " + filename);
         }
     std::fclose(fp);
     ownedsize = s[3];
@@ -24,11 +22,9 @@ void GIP_Topo::postConstruct(std::string filename) {
     boundaries = s[7];
     halos = s[8];
     innersize = interiors + interfaces;
-    if (s[0] != 0 || s[1] != 0 || allsize != ownedsize) {
-        std::cerr << "GIP_Topo: " << filename
-                  << " describes a partition with a halo; this backend runs one partition per device" << std::endl;
-        std::exit(2);
-    }
+    if (s[0] != 0 || s[1] != 0 || allsize != ownedsize)
+        throw std::runtime_error("GIP_Topo: " + filename +
+                                 " describes a partition with a halo; this backend runs one partition per device");
 }
 
 void GIP_Topo::postConstructSizes(int inner, int ghosts) {

@@ -7,6 +7,7 @@
 // working directory) and prints one line per step in the reference's format.
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <iostream>
 
 #include "GIP_DNS.h"
@@ -51,10 +52,15 @@ int main(int argc, char **argv) {
             return 1;
         }
     }
-    GIP_DNS<Matrix, Vector, Architecture, NonLinear> driven(argv[1]);
-    driven.setRoot(root);
-    driven.configure(p);
-    driven.setUp();
-    driven.integrate();
+    try {
+        GIP_DNS<Matrix, Vector, Architecture, NonLinear> driven(argv[1]);
+        driven.setRoot(root);
+        driven.configure(p);
+        driven.setUp();
+        driven.integrate();
+    } catch (const std::exception &e) {  // unreadable or unsupported input files
+        std::cerr << e.what() << std::endl;
+        return 2;
+    }
     return 0;
 }

@@ -4,6 +4,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
+#include <stdexcept>
+#include <string>
 
 GIP_MatrixHIP::~GIP_MatrixHIP() {
     if (pcg) gip_pcg_destroy(pcg);
@@ -48,10 +50,8 @@ void GIP_MatrixHIP::checkShapes(GIP_Vector *x, int outLen) {
 
 void GIP_MatrixHIP::postConstruct(std::string name, GIP_Topo *topo, GIP_Arch *arch) {
     gip_csr A;
-    if (gip_csr_read(name.c_str(), &A) != 0) {
-        std::cerr << " Error opening operator file " << name << ": " << gip_cavity_last_error() << std::endl;
-        std::exit(2);
-    }
+    if (gip_csr_read(name.c_str(), &A) != 0)
+        throw std::runtime_error("error opening operator file " + name + ": " + gip_cavity_last_error());
     postConstruct(A, topo, arch);
     gip_csr_release(&A);
 }

@@ -3,6 +3,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <iostream>
+#include <stdexcept>
+#include <string>
 
 #include "gip_cavity.h"
 
@@ -32,13 +34,11 @@ void GIP_VectorHIP::postConstruct(std::string name, GIP_Topo *topo, GIP_Arch *ar
     postConstruct(topo, arch);
     double *vals = nullptr;
     int32_t n = 0;
-    if (gip_vec_read(name.c_str(), &vals, &n) != 0) {
-        std::cerr << " Error opening scalarfield file " << name << ": " << gip_cavity_last_error() << std::endl;
-        std::exit(2);
-    }
+    if (gip_vec_read(name.c_str(), &vals, &n) != 0)
+        throw std::runtime_error("error opening scalarfield file " + name + ": " + gip_cavity_last_error());
     if (n > size) {
-        std::cerr << " scalarfield " << name << " longer than the topology" << std::endl;
-        std::exit(2);
+        std::free(vals);
+        throw std::runtime_error("scalarfield " + name + " longer than the topology");
     }
     for (int i = 0; i < n; ++i) vec[i] = vals[i];
     std::free(vals);

@@ -164,3 +164,16 @@ def test_batch_single_instance_on_c100k_matches_oracle(dev, c100k_root):
         a, r = b.field(0, f), orc.field(f)
         assert np.linalg.norm(a - r) <= 1e-10 * np.linalg.norm(r), f
     b.close()
+
+
+def test_batch_argument_errors(dev):
+    """gip_batch_create rejects a non-positive tolerance and a cavity that
+    lacks the integrator's operators."""
+    full = cavity.generate(8, gamma=1e-2)
+    with pytest.raises(hip.HipError) as e:
+        batch.Batch(dev, [full], [1e-2], pcg_tol=0.0)
+    assert e.value.code == hip.EINVAL
+    lap_only = cavity.generate(8, gamma=1e-2, ops=cavity.OP_BITS["L"])
+    with pytest.raises(hip.HipError) as e:
+        batch.Batch(dev, [full, lap_only], [1e-2, 1e-2], pcg_tol=1e-8)
+    assert e.value.code == hip.EINVAL

@@ -220,3 +220,19 @@ def test_drivencavityhip_executable(cav46_dir):
     for g, w in zip(got, want):
         for key in ("time", "dt", "massr"):
             assert _rel(g[key], w[key]) <= REL
+
+
+def test_dns_argument_errors(tmp_path, cav46_dir):
+    """The integrator's C ABI reports a missing case, an unknown field and a
+    field of the wrong length as errors instead of failing later."""
+    with pytest.raises(dns.DnsError):
+        dns.Simulation(root=tmp_path, name="absent", gamma=1e-2)
+    sim = dns.Simulation(root=cav46_dir, name="cav46", gamma=1e-2, pcg_mode="fused")
+    with pytest.raises(dns.DnsError):
+        sim.field("nonexistent")
+    with pytest.raises(dns.DnsError):
+        sim.set_field("u", np.zeros(sim.n_all + 1))
+    u = sim.field("u")
+    sim.set_field("u", u)  # the right length round-trips
+    assert np.array_equal(sim.field("u"), u)
+    sim.close()
