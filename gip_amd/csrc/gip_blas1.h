@@ -54,6 +54,32 @@ __device__ inline double block_reduce(double v) {
     return v;
 }
 
+// M sums over the workgroup at once, valid in thread 0: each value takes
+// block_reduce's shuffle tree and final order (so each total is bitwise
+// block_reduce<RED_SUM>'s), and all M share one LDS exchange and its pair
+// of barriers.
+template <int M>
+__device__ inline void block_reduce_n(double (&v)[M]) {
+    __shared__ double sh[M][kBlock / 64];
+    __syncthreads();  // protect sh against a previous call
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[m] = v[m] + __shfl_down(v[m], off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int m = 0; m < M; ++m) sh[m][wid] = v[m];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            v[m] = sh[m][0];
+#pragma unroll
+            for (int k = 1; k < kBlock / 64; ++k) v[m] = v[m] + sh[m][k];
+        }
+}
+
 // Publish this workgroup's value (thread 0's v) and, in the workgroup that
 // finishes last, combine all partials in index order into *out (thread 0).
 // Returns true in every thread of that last workgroup.  The counter is
@@ -133,14 +159,17 @@ struct PaddedSums {
     }
     __device__ inline void reduce(double (&out)[M]) {
         __shared__ double s_all[M];
+        double acc[M];
 #pragma unroll
         for (int m = 0; m < M; ++m) {
-            double acc = 0.0;
+            acc[m] = 0.0;
 #pragma unroll
-            for (int k = 0; k < K; ++k) acc = acc + v[m][k];
-            acc = block_reduce<RED_SUM>(acc);
-            if (threadIdx.x == 0) s_all[m] = acc;
+            for (int k = 0; k < K; ++k) acc[m] = acc[m] + v[m][k];
         }
+        block_reduce_n<M>(acc);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int m = 0; m < M; ++m) s_all[m] = acc[m];
         __syncthreads();
 #pragma unroll
         for (int m = 0; m < M; ++m) out[m] = s_all[m];

@@ -75,7 +75,10 @@ enum { ST_RUN = 0, ST_CONV = 1, ST_BREAK = 2, ST_MASK = 3 };
 // P_DN2: p.Ap partials of odd iterations on operators small enough for the
 // fused direction-product kernel (which reads the previous iteration's
 // partials while writing its own).
-enum { P_DN = 0, P_RS = 1, P_RR = 2, P_BB = 3, P_RS2 = 4, P_RR2 = 5, P_SUMR = 6, P_DN2 = 7, P_COUNT = 8 };
+// P_SA, P_SA2: partials of 1.Ap of the mixed inner iteration (the constant
+// vector's deflation), by iteration parity on small operators.
+enum { P_DN = 0, P_RS = 1, P_RR = 2, P_BB = 3, P_RS2 = 4, P_RR2 = 5, P_SUMR = 6, P_DN2 = 7, P_SA = 8, P_SA2 = 9,
+       P_COUNT = 10 };
 
 struct PcgDev {
     int n, G;      // solved rows, workgroups of the sweeping kernels (<= kMaxRedBlocks)
@@ -84,10 +87,13 @@ struct PcgDev {
     double *chunk; // partials of the product: with the fold, kWaves per
                    // 256-row chunk (one per wavefront); without it, one per
                    // chunk, and these are the P_DN partials themselves
+    double *chunk_s;  // long operators: per-wavefront partials of 1.Ap (MIXED deflation)
     double *x, *r, *s, *Ap;
     const double *b, *dinv;
     float *r32, *s32, *Ap32, *xc32;
     const float *dinv32;
+    const float *az32;  // A 1 in FP32 (MIXED deflation)
+    int deflate;        // MIXED: deflate the constant vector in the inner iteration
     const double *az;  // A 1 on the solved rows (MIXED)
     double *sc;        // S_COUNT scalars
     int *state, *iters;
@@ -125,6 +131,20 @@ constexpr int kWaves = kBlock / 64;  // wavefronts per workgroup
 __device__ inline double rho_slot(const PcgDev &d, int k) { return d.sc[S_RHO0 + (k & 1)]; }
 // p.Ap partials of iteration k in the fused (small-operator) iteration
 __device__ inline double *dn_part(const PcgDev &d, int k) { return d.P((k & 1) ? P_DN2 : P_DN); }
+// 1.Ap partials of iteration k of the mixed fused iteration
+__device__ inline double *sa_part(const PcgDev &d, int k) { return d.P((k & 1) ? P_SA2 : P_SA); }
+
+// Deflation of the constant vector in the mixed inner iteration (deflated
+// CG, Saad, Yeung, Erhel and Guyomarc'h, SIAM J. Sci. Comput. 21, 2000,
+// with W = 1): the iteration uses w = P A p = A p - A1 (1.Ap)/(1.A1) in
+// place of A p, so alpha = rho / (p.Ap - (1.Ap)^2 / (1.A1)) (1.Ap = A1.p:
+// A symmetric on the solved rows).  The defect correction already applies
+// the matching coarse step (x += c 1).  Off when 1.A1 is not positive and
+// finite, like the coarse step.
+__device__ inline bool deflating(const PcgDev &d, double zaz) { return d.deflate && zaz > 0.0 && isfinite(zaz); }
+__device__ inline double m_denom(double dn, double sap, double zaz, bool dfl) {
+    return dfl ? dn - sap * sap / zaz : dn;
+}
 
 // ---- setup ---------------------------------------------------------------
 
@@ -168,12 +188,13 @@ __global__ __launch_bounds__(kBlock) void k_setup_dir(PcgDev d, double *__restri
         bb = bb + d.b[row] * d.b[row];
         rr = rr + ri * ri;
     }
-    rs = block_reduce<RED_SUM>(rs);
-    if (threadIdx.x == 0) d.P(P_RS)[blockIdx.x] = rs;
-    bb = block_reduce<RED_SUM>(bb);
-    if (threadIdx.x == 0) d.P(P_BB)[blockIdx.x] = bb;
-    rr = block_reduce<RED_SUM>(rr);
-    if (threadIdx.x == 0) d.P(P_RR)[blockIdx.x] = rr;
+    double red[3] = {rs, bb, rr};
+    block_reduce_n<3>(red);
+    if (threadIdx.x == 0) {
+        d.P(P_RS)[blockIdx.x] = red[0];
+        d.P(P_BB)[blockIdx.x] = red[1];
+        d.P(P_RR)[blockIdx.x] = red[2];
+    }
 }
 
 // one workgroup: rho_0, b.b, r.r, the tolerance, the state word and the iteration count
@@ -409,14 +430,23 @@ __global__ __launch_bounds__(kBlock) void k_iter_dirprod(EllView A, PcgDev d, co
 __global__ __launch_bounds__(kBlock) void k_fold_dn(PcgDev d) {
     if (!running(d)) return;
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    double v = 0.0;
+    double v = 0.0, sa = 0.0;
     if (i < d.nchunk) {
         const double *w = d.chunk + (size_t)i * kWaves;
 #pragma unroll
         for (int j = 0; j < kWaves; ++j) v = v + w[j];
+        if (d.deflate) {
+            const double *ws = d.chunk_s + (size_t)i * kWaves;
+#pragma unroll
+            for (int j = 0; j < kWaves; ++j) sa = sa + ws[j];
+        }
     }
     v = block_reduce<RED_SUM>(v);
     if (threadIdx.x == 0) d.P(P_DN)[blockIdx.x] = v;
+    if (d.deflate) {
+        sa = block_reduce<RED_SUM>(sa);
+        if (threadIdx.x == 0) d.P(P_SA)[blockIdx.x] = sa;
+    }
 }
 
 // K2: alpha_k, r += (-alpha) Ap, s = Dinv r, partials of r.s and r.r; each
@@ -495,10 +525,12 @@ __global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, int k) {
             }
         }
     }
-    rs = block_reduce<RED_SUM>(rs);
-    if (threadIdx.x == 0) d.P(P_RS)[blockIdx.x] = rs;
-    rr = block_reduce<RED_SUM>(rr);
-    if (threadIdx.x == 0) d.P(P_RR)[blockIdx.x] = rr;
+    double red[2] = {rs, rr};
+    block_reduce_n<2>(red);
+    if (threadIdx.x == 0) {
+        d.P(P_RS)[blockIdx.x] = red[0];
+        d.P(P_RR)[blockIdx.x] = red[1];
+    }
     if (leader()) *d.iters = k + 1;
 }
 
@@ -522,7 +554,14 @@ __global__ __launch_bounds__(kBlock) void k_m_dir(PcgDev d, float *__restrict__ 
     const float *__restrict__ s = d.s32;
     if (!running(d)) return;
     const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);
-    const float a32 = pending ? (float)alpha_of(d, k - 1) : 0.0f;
+    float a32 = 0.0f;
+    if (pending) {  // alpha_{k-1}, as K2 of k-1 formed it
+        const double zaz = d.sc[S_ZAZ];
+        const bool dfl = deflating(d, zaz);
+        const double dn = sum_partials<RED_SUM>(d.P(P_DN), d.Gdn);
+        const double sap = dfl ? sum_partials<RED_SUM>(d.P(P_SA), d.Gdn) : 0.0;
+        a32 = (float)(rho_slot(d, k - 1) / m_denom(dn, sap, zaz, dfl));
+    }
     const float beta = (float)(rho / rho_slot(d, k - 1));
     if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
     FOR_ROWS(d.n, row) {
@@ -559,18 +598,21 @@ __global__ __launch_bounds__(kBlock) void k_m_dirprod(EllView A, PcgDev d, const
         po = pold[row];
     }
     const double rho_prev = early_load(d.sc + S_RHO0 + ((k - 1) & 1));
-    PaddedSums<2> ps;
-    ps.load({d.P(P_RS), dn_part(d, k - 1)}, {d.G, d.Gdn});
+    const double zaz = early_load(d.sc + S_ZAZ);
+    PaddedSums<3> ps;
+    ps.load({d.P(P_RS), dn_part(d, k - 1), sa_part(d, k - 1)}, {d.G, d.Gdn, d.Gdn});
     constexpr int SW = W < 0 ? -W : 1;
     StencilPre<SW, float> q;
     if (W < 0 && own) stencil_row_pre<SW, float>(A, A.val32, row, q);
-    double sums[2];
+    double sums[3];
     ps.reduce(sums);
     if (st != ST_RUN) return;
-    const float a32 = pending ? (float)(rho_prev / sums[1]) : 0.0f;  // alpha_{k-1}, as K2 of k-1 formed it
+    const bool dfl = deflating(d, zaz);
+    // alpha_{k-1}, as K2 of k-1 formed it
+    const float a32 = pending ? (float)(rho_prev / m_denom(sums[1], sums[2], zaz, dfl)) : 0.0f;
     const float beta = (float)(sums[0] / rho_prev);
     if (leader()) d.sc[S_RHO0 + (k & 1)] = sums[0];
-    double acc = 0.0;
+    double acc = 0.0, sa = 0.0;
     if (own) {
         if (pending) d.xc32[row] = xo + a32 * po;
         const float pr = si + beta * po;
@@ -583,9 +625,14 @@ __global__ __launch_bounds__(kBlock) void k_m_dirprod(EllView A, PcgDev d, const
             row_dot<W, float, float>(A, A.val32, op, row, sum);
         d.Ap32[row] = sum;
         acc = (double)pr * (double)sum;
+        sa = (double)sum;
     }
-    acc = block_reduce<RED_SUM>(acc);
-    if (threadIdx.x == 0) dn_part(d, k)[c] = acc;
+    double red[2] = {acc, sa};
+    block_reduce_n<2>(red);
+    if (threadIdx.x == 0) {
+        dn_part(d, k)[c] = red[0];
+        if (dfl) sa_part(d, k)[c] = red[1];
+    }
 }
 
 // K1 in FP32 (SMALL as in iter_spmv)
@@ -598,7 +645,7 @@ __device__ inline void m_spmv(const EllView &A, const PcgDev &d, const float *__
         // decided at a defect correction, after which nothing is queued)
         const int c = xcd_block(blockIdx.x, gridDim.x);
         const int row = c * kBlock + threadIdx.x;
-        double acc = 0.0;
+        double acc = 0.0, sa = 0.0;
         if (row < d.n) {
             // (a SelfOperand here measured 9% slower than the separate load)
             Operand<float> po{p};
@@ -607,9 +654,15 @@ __device__ inline void m_spmv(const EllView &A, const PcgDev &d, const float *__
             row_dot<W, float, float>(A, A.val32, po, row, sum);
             d.Ap32[row] = sum;
             acc = (double)pr * (double)sum;
+            sa = (double)sum;
         }
         acc = wave_sum(acc);
-        if ((threadIdx.x & 63) == 0) d.chunk[c * kWaves + (threadIdx.x >> 6)] = acc;
+        const int wv = c * kWaves + (threadIdx.x >> 6);
+        if ((threadIdx.x & 63) == 0) d.chunk[wv] = acc;
+        if (d.deflate) {
+            sa = wave_sum(sa);
+            if ((threadIdx.x & 63) == 0) d.chunk_s[wv] = sa;
+        }
         return;
     }
     const int st = early_load(d.state);
@@ -622,13 +675,18 @@ __device__ inline void m_spmv(const EllView &A, const PcgDev &d, const float *__
         row_dot<W, float, float>(A, A.val32, po, row, sum);
     }
     if (st != ST_RUN) return;
-    double acc = 0.0;
+    double acc = 0.0, sa = 0.0;
     if (row < d.n) {
         d.Ap32[row] = sum;
         acc = (double)pr * (double)sum;
+        sa = (double)sum;
     }
-    acc = block_reduce<RED_SUM>(acc);
-    if (threadIdx.x == 0) d.chunk[c] = acc;
+    double red[2] = {acc, sa};
+    block_reduce_n<2>(red);
+    if (threadIdx.x == 0) {
+        d.chunk[c] = red[0];
+        if (d.deflate) sa_part(d, 0)[c] = red[1];
+    }
 }
 
 template <int W>
@@ -649,18 +707,24 @@ __global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, int k) {
     double rs = 0.0, rr = 0.0;
     if constexpr (!SMALL) {
         if (!running(d)) return;
-        const double dn = sum_partials<RED_SUM>(d.P(P_DN), d.Gdn);
+        const double zaz = d.sc[S_ZAZ];
+        const bool dfl = deflating(d, zaz);
+        const double sap = dfl ? sum_partials<RED_SUM>(d.P(P_SA), d.Gdn) : 0.0;
+        const double dn = m_denom(sum_partials<RED_SUM>(d.P(P_DN), d.Gdn), sap, zaz, dfl);
         const double alpha = rho_slot(d, k) / dn;
         if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) {
             if (leader()) *d.state = ST_BREAK;
             return;
         }
-        const float a32 = (float)alpha;
+        const float a32 = (float)alpha, c32 = dfl ? (float)(sap / zaz) : 0.0f;
+        const float *__restrict__ az = d.az32;
         FOR_ROWS(d.n, row) {
             // every operand first, so the row costs one memory round trip
             const float ro = r[row], apr = Ap[row];
             const float di = precond ? dinv[row] : 0.0f;
-            const float ri = ro - a32 * apr;
+            const float azr = dfl ? az[row] : 0.0f;
+            const float wr = dfl ? apr - c32 * azr : apr;  // (P A p)_row
+            const float ri = ro - a32 * wr;
             r[row] = ri;
             const float si = precond ? ri * di : ri;
             s[row] = si;
@@ -669,27 +733,33 @@ __global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, int k) {
         }
     } else {
         ThreadRows t(d.n);
-        float ro = 0.0f, apr = 0.0f, di = 0.0f;
+        float ro = 0.0f, apr = 0.0f, di = 0.0f, azr = 0.0f;
+        const float *__restrict__ az = d.az32;
         const int st = early_load(d.state);
+        const bool want_az = d.deflate != 0;
         if (t.valid()) {
             ro = r[t.row];
             apr = Ap[t.row];
             if (precond) di = dinv[t.row];
+            if (want_az) azr = az[t.row];
         }
         const double rho = early_load(d.sc + S_RHO0 + (k & 1));
-        double sums[1];
-        sum_padded<1>({dn_part(d, k)}, {d.Gdn}, sums);
-        const double dn = sums[0];
+        const double zaz = early_load(d.sc + S_ZAZ);
+        double sums[2];
+        sum_padded<2>({dn_part(d, k), sa_part(d, k)}, {d.Gdn, d.Gdn}, sums);
         if (st != ST_RUN) return;
+        const bool dfl = deflating(d, zaz);
+        const double dn = m_denom(sums[0], sums[1], zaz, dfl);
         const double alpha = rho / dn;
         if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) {
             if (leader()) *d.state = ST_BREAK;
             return;
         }
-        const float a32 = (float)alpha;
+        const float a32 = (float)alpha, c32 = dfl ? (float)(sums[1] / zaz) : 0.0f;
         while (t.valid()) {
             const int row = t.row;
-            const float ri = ro - a32 * apr;
+            const float wr = dfl ? apr - c32 * azr : apr;  // (P A p)_row
+            const float ri = ro - a32 * wr;
             r[row] = ri;
             const float si = precond ? ri * di : ri;
             s[row] = si;
@@ -700,13 +770,16 @@ __global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, int k) {
                 ro = r[t.row];
                 apr = Ap[t.row];
                 if (precond) di = dinv[t.row];
+                if (want_az) azr = az[t.row];
             }
         }
     }
-    rs = block_reduce<RED_SUM>(rs);
-    if (threadIdx.x == 0) d.P(P_RS)[blockIdx.x] = rs;
-    rr = block_reduce<RED_SUM>(rr);
-    if (threadIdx.x == 0) d.P(P_RR)[blockIdx.x] = rr;
+    double red[2] = {rs, rr};
+    block_reduce_n<2>(red);
+    if (threadIdx.x == 0) {
+        d.P(P_RS)[blockIdx.x] = red[0];
+        d.P(P_RR)[blockIdx.x] = red[1];
+    }
     if (leader()) *d.iters = k + 1;
 }
 
@@ -717,10 +790,13 @@ __global__ __launch_bounds__(kBlock) void k_m_status(PcgDev d) {
 }
 
 // x += (xc + alpha_k p) (the last iteration's pending update first), xc = 0
-// (dn: iteration k's p.Ap partials)
+// (dn, sa: iteration k's p.Ap and 1.Ap partials)
 __global__ __launch_bounds__(kBlock) void k_m_fold(PcgDev d, const float *__restrict__ p, int k,
-                                                   const double *__restrict__ dn) {
-    const float a32 = (float)(rho_slot(d, k) / sum_partials<RED_SUM>(dn, d.Gdn));
+                                                   const double *__restrict__ dn, const double *__restrict__ sa) {
+    const double zaz = d.sc[S_ZAZ];
+    const bool dfl = deflating(d, zaz);
+    const double sap = dfl ? sum_partials<RED_SUM>(sa, d.Gdn) : 0.0;
+    const float a32 = (float)(rho_slot(d, k) / m_denom(sum_partials<RED_SUM>(dn, d.Gdn), sap, zaz, dfl));
     FOR_ROWS(d.n, row) {
         const float xcv = d.xc32[row] + a32 * p[row];
         d.x[row] = d.x[row] + (double)xcv;
@@ -769,10 +845,12 @@ __global__ __launch_bounds__(kBlock) void k_m_coarse(PcgDev d, int use) {
         rs = rs + (double)r32 * (double)s32;
         rr = rr + ri * ri;
     }
-    rs = block_reduce<RED_SUM>(rs);
-    if (threadIdx.x == 0) d.P(P_RS2)[blockIdx.x] = rs;
-    rr = block_reduce<RED_SUM>(rr);
-    if (threadIdx.x == 0) d.P(P_RR2)[blockIdx.x] = rr;
+    double red[2] = {rs, rr};
+    block_reduce_n<2>(red);
+    if (threadIdx.x == 0) {
+        d.P(P_RS2)[blockIdx.x] = red[0];
+        d.P(P_RR2)[blockIdx.x] = red[1];
+    }
 }
 
 // One workgroup after k_m_resid: the replaced residual becomes the
@@ -831,6 +909,7 @@ struct gip_pcg {
     double *p2;   // second direction buffer of the fused iteration (small operators)
     float *p32b;  // its FP32 counterpart
     double *az;   // A 1 (MIXED coarse step)
+    float *az32;  // its FP32 copy (MIXED deflation)
     double *sc, *part, *chunk;
     int nchunk, gdn;
     bool fold;  // nchunk > kMaxRedBlocks: the chunk partials need the fold kernel;
@@ -873,6 +952,9 @@ static PcgDev make_dev(gip_pcg *S, const double *b, double *x, int precond) {
     d.nchunk = S->nchunk;
     d.Gdn = S->gdn;
     d.chunk = S->fold ? S->chunk : S->part + P_DN * kMaxRedBlocks;
+    d.chunk_s = S->fold ? S->chunk + (size_t)S->nchunk * kWaves : nullptr;
+    d.az32 = S->az32;
+    d.deflate = 0;
     d.x = x;
     d.r = S->r;
     d.s = S->s;
@@ -931,7 +1013,7 @@ int gip_pcg_create(gip_ctx *c, const gip_mat *A, int32_t n_all, gip_pcg **out) {
     dalloc((void **)&S->dinv, n * 8);
     dalloc((void **)&S->sc, S_COUNT * 8);
     dalloc((void **)&S->part, (size_t)P_COUNT * kMaxRedBlocks * 8);
-    if (S->fold) dalloc((void **)&S->chunk, (size_t)S->nchunk * kWaves * 8);
+    if (S->fold) dalloc((void **)&S->chunk, (size_t)S->nchunk * kWaves * 2 * 8);  // p.Ap, then 1.Ap
     dalloc((void **)&S->state, 2 * sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_state, 2 * sizeof(int), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **)&S->h_sc, S_COUNT * 8, hipHostMallocDefault);
@@ -962,7 +1044,7 @@ int gip_pcg_destroy(gip_pcg *S) {
     hipSetDevice(S->ctx->device);
     hipStreamSynchronize(S->ctx->stream);
     void *ptrs[] = {S->r,    S->p,      S->s,  S->Ap, S->dinv, S->r32,   S->p32,  S->s32,  S->Ap32,
-                    S->xc32, S->dinv32, S->az, S->sc, S->part, S->chunk, S->state, S->p2, S->p32b};
+                    S->xc32, S->dinv32, S->az, S->sc, S->part, S->chunk, S->state, S->p2, S->p32b, S->az32};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (S->h_state) hipHostFree(S->h_state);
@@ -983,6 +1065,7 @@ static int ensure_mixed(gip_pcg *S) {
     float **bufs[] = {&S->r32, &S->s32, &S->Ap32, &S->xc32, &S->dinv32};
     for (float **b : bufs) GIP_HIP_TRY(hipMalloc(b, n * 4));
     GIP_HIP_TRY(hipMalloc(&S->az, n * 8));
+    GIP_HIP_TRY(hipMalloc(&S->az32, n * 4));
     // the FP32 direction is an SpMV input: ghost columns included, and zero
     // there (the correction leaves the ghost values of x unchanged)
     GIP_HIP_TRY(hipMalloc(&S->p32, na * 4));
@@ -1009,6 +1092,10 @@ static int coarse_setup(gip_pcg *S) {
     GIP_LAUNCH_CHECK("coarse vector");
     int rc = spmv_launch(c, S->A, z, S->az, 1.0, 0.0, 0, nullptr);
     if (!rc) rc = launch_reduce(c, RED_SUM, S->n, S->az, z, S->sc + S_ZAZ);
+    if (!rc && S->n) {
+        k_to32<<<(S->n + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(S->n, S->az, S->az32);
+        GIP_LAUNCH_CHECK("coarse vector fp32");
+    }
     return rc;
 }
 
@@ -1039,12 +1126,16 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
     EllView v = view(A);
     const int pollk = o->poll > 0 ? o->poll : 16;
     const int use_coarse = o->coarse >= 0 ? 1 : 0;
+    d.deflate = mixed && use_coarse;
     // direction and p.Ap partials of iteration k: one buffer and one partial
     // array with the fold; alternating by parity in the fused iteration
     auto pdir = [&](int k) { return (!S->fold && (k & 1)) ? S->p2 : S->p; };
     auto pdir32 = [&](int k) { return (!S->fold && (k & 1)) ? S->p32b : S->p32; };
     auto pdn = [&](int k) {
         return (const double *)(S->part + (size_t)((!S->fold && (k & 1)) ? P_DN2 : P_DN) * kMaxRedBlocks);
+    };
+    auto psa = [&](int k) {
+        return (const double *)(S->part + (size_t)((!S->fold && (k & 1)) ? P_SA2 : P_SA) * kMaxRedBlocks);
     };
     // time_products: events around one product launch (k >= 1) per host
     // poll, read after the poll's synchronisation
@@ -1158,7 +1249,7 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
             const double rr32 = S->h_sc[S_RR], tol2 = S->h_sc[S_TOL2];
             const bool near_target = tol2 >= 0.0 && rr32 <= tol2;
             if (rr32 <= inner2 * last_rr || near_target || k >= o->maxit) {
-                k_m_fold<<<S->grid, kBlock, 0, st>>>(d, pdir32(k - 1), k - 1, pdn(k - 1));
+                k_m_fold<<<S->grid, kBlock, 0, st>>>(d, pdir32(k - 1), k - 1, pdn(k - 1), psa(k - 1));
                 pending = 0;
                 DISPATCH_W(A, k_m_resid, v, d);
                 k_m_coarse<<<S->grid, kBlock, 0, st>>>(d, use_coarse);
