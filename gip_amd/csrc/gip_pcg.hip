@@ -45,6 +45,9 @@
 // (eigenvalue ~ pin/N), which FP32 arithmetic cannot resolve; left to the
 // FP32 iteration it re-enters through round-off and the corrections
 // diverge.  For an SPD operator the step never increases the A-norm error.
+// The FP32 iteration is deflated by the same vector (A p replaced by
+// A p - A1 (1.Ap)/(1.A1)), so its residual stays orthogonal to 1 and the
+// constant mode is left to the FP64 step.
 #include <algorithm>
 #include <cmath>
 #include <new>

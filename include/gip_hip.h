@@ -177,7 +177,9 @@ typedef struct gip_pcg_opts {
     int32_t coarse;    /* MIXED: at every FP64 correction, the Galerkin step
                           x += c 1, c = (1.r)/(1.A1), along the constant vector
                           of the solved rows (the near-null mode of a pinned
-                          Neumann Laplacian, which FP32 cannot resolve):
+                          Neumann Laplacian, which FP32 cannot resolve), and
+                          the FP32 inner iteration deflated by that vector
+                          (A p replaced by A p - A1 (1.Ap)/(1.A1)):
                           0 on (default), -1 off                                  */
     int32_t time_products; /* 1: time one product launch per host poll with events on
                               the solver's stream (read at the poll's synchronisation);
