@@ -208,6 +208,34 @@ __device__ inline double sum_partials(const double *__restrict__ part, int n) {
     return s_all;
 }
 
+// M arrays of n partials summed together (n <= kMaxRedBlocks): only the
+// slots below n are read, in a workgroup-uniform loop of ceil(n / 256)
+// rounds, the M arrays' loads of a round issued together; the same total in
+// every workgroup (thread t takes t, t+256, ..., then the fixed tree).
+template <int M>
+__device__ inline void sum_partials_n(const double *const (&part)[M], int n, double (&out)[M]) {
+    __shared__ double s_all[M];
+    double acc[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) acc[m] = 0.0;
+    const int rounds = (n + kBlock - 1) / kBlock;
+    for (int k = 0; k < rounds; ++k) {
+        const int i = threadIdx.x + k * kBlock;
+        double v[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) v[m] = i < n ? part[m][i] : 0.0;
+#pragma unroll
+        for (int m = 0; m < M; ++m) acc[m] = acc[m] + v[m];
+    }
+    block_reduce_n<M>(acc);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int m = 0; m < M; ++m) s_all[m] = acc[m];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < M; ++m) out[m] = s_all[m];
+}
+
 int launch_reduce(gip_ctx *c, int op, int n, const double *x, const double *y, double *dres);
 
 }  // namespace gip

@@ -69,6 +69,8 @@ enum {
     S_TOL2 = 6,     // (tol ||b||)^2, negative when no tolerance
     S_TRUE_RR = 7,  // FP64 true residual of the last correction (MIXED)
     S_ZAZ = 8,      // 1.A1 over the solved rows (MIXED coarse step)
+    S_ALPHA0 = 10,  // alpha of even iterations, as the update kernel formed it
+    S_ALPHA1 = 11,  // alpha of odd iterations
     S_COUNT = 16
 };
 // State word: ST_RUN, ST_BREAK, or ST_CONV; the direction kernel of
@@ -83,6 +85,17 @@ enum { ST_RUN = 0, ST_CONV = 1, ST_BREAK = 2, ST_MASK = 3 };
 enum { P_DN = 0, P_RS = 1, P_RR = 2, P_BB = 3, P_RS2 = 4, P_RR2 = 5, P_SUMR = 6, P_DN2 = 7, P_SA = 8, P_SA2 = 9,
        P_COUNT = 10 };
 
+// The nonzero entries of A 1 in FP32, when there are at most kAzList of
+// them (count > kAzList: too many, the kernels read A 1 row by row).  A
+// pinned Neumann Laplacian has zero row sums but at its pinned row, so its
+// A 1 has one nonzero and the long update kernel need not stream A 1.
+constexpr int kAzList = 8;
+struct AzList {
+    int count;
+    int idx[kAzList];
+    float val[kAzList];
+};
+
 struct PcgDev {
     int n, G;      // solved rows, workgroups of the sweeping kernels (<= kMaxRedBlocks)
     int nchunk;    // 256-row chunks: workgroups of the product kernels
@@ -96,6 +109,7 @@ struct PcgDev {
     float *r32, *s32, *Ap32, *xc32;
     const float *dinv32;
     const float *az32;  // A 1 in FP32 (MIXED deflation)
+    const struct AzList *azl;  // its nonzeros, when few (MIXED deflation)
     int deflate;        // MIXED: deflate the constant vector in the inner iteration
     const double *az;  // A 1 on the solved rows (MIXED)
     double *sc;        // S_COUNT scalars
@@ -129,6 +143,34 @@ __device__ inline double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = v + __shfl_down(v, off, 64);
     return v;
+}
+
+// v of lane i - (CTRL - 0x110) within the lane's row of 16 (0 past the
+// row's start): a DPP row shift, moved as two 32-bit halves
+template <int CTRL>
+__device__ inline double dpp_row_shr(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ inline double read_lane(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over the wavefront with DPP row shifts (an inclusive scan of each row
+// of 16 lanes) and four lane reads, the same value in every lane; a fixed
+// order, but not wave_sum's (the FP32 product of the mixed iteration only)
+__device__ inline double wave_sum_dpp(double v) {
+    v = v + dpp_row_shr<0x111>(v);
+    v = v + dpp_row_shr<0x112>(v);
+    v = v + dpp_row_shr<0x114>(v);
+    v = v + dpp_row_shr<0x118>(v);
+    return ((read_lane(v, 15) + read_lane(v, 31)) + read_lane(v, 47)) + read_lane(v, 63);
 }
 constexpr int kWaves = kBlock / 64;  // wavefronts per workgroup
 __device__ inline double rho_slot(const PcgDev &d, int k) { return d.sc[S_RHO0 + (k & 1)]; }
@@ -218,9 +260,12 @@ __global__ __launch_bounds__(kBlock) void k_setup_fin(PcgDev d, double tol) {
 
 // ---- FP64 iteration --------------------------------------------------------
 
-// alpha of iteration k from its p.Ap partials (the value K2 of k used)
-__device__ inline double alpha_of(const PcgDev &d, int k) {
-    return rho_slot(d, k) / sum_partials<RED_SUM>(d.P(P_DN), d.Gdn);
+// alpha of iteration k, stored by workgroup 0 of K2 of k (every workgroup
+// of K2 forms the same value from the same partials), so that a later
+// kernel reads one scalar instead of re-reducing the p.Ap partials
+__device__ inline double alpha_slot(const PcgDev &d, int k) { return d.sc[S_ALPHA0 + (k & 1)]; }
+__device__ inline void store_alpha(const PcgDev &d, int k, double alpha) {
+    if (leader()) d.sc[S_ALPHA0 + (k & 1)] = alpha;
 }
 
 // x += alpha_k p over the solved rows
@@ -271,7 +316,7 @@ __global__ __launch_bounds__(kBlock) void k_iter_dir(PcgDev d, double *__restric
     const int st = *d.state;
     if (st != ST_RUN && st != conv_stamp(k)) return;
     const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);  // :107
-    const double alpha = alpha_of(d, k - 1);
+    const double alpha = alpha_slot(d, k - 1);
     const double tol2 = d.sc[S_TOL2];
     if (tol2 >= 0.0) {
         const double rr = sum_partials<RED_SUM>(d.P(P_RR), d.G);
@@ -290,11 +335,10 @@ __global__ __launch_bounds__(kBlock) void k_iter_dir(PcgDev d, double *__restric
     }
 }
 
-// (dn: iteration k's p.Ap partials)
-__global__ __launch_bounds__(kBlock) void k_x_flush(PcgDev d, const double *__restrict__ p, int k,
-                                                    const double *__restrict__ dn) {
+// x += alpha_k p of the last iteration k
+__global__ __launch_bounds__(kBlock) void k_x_flush(PcgDev d, const double *__restrict__ p, int k) {
     if (!running(d)) return;
-    x_update(d, p, rho_slot(d, k) / sum_partials<RED_SUM>(dn, d.Gdn));
+    x_update(d, p, alpha_slot(d, k));
 }
 
 // K1: Ap = A p (empty rows keep Ap), one partial of p.Ap per 256-row chunk;
@@ -388,18 +432,18 @@ __global__ __launch_bounds__(kBlock) void k_iter_dirprod(EllView A, PcgDev d, co
     }
     const double tol2 = early_load(d.sc + S_TOL2);
     const double rho_prev = early_load(d.sc + S_RHO0 + ((k - 1) & 1));
-    PaddedSums<3> ps;
-    ps.load({d.P(P_RS), dn_part(d, k - 1), d.P(P_RR)}, {d.G, d.Gdn, d.G});
+    const double alpha = early_load(d.sc + S_ALPHA0 + ((k - 1) & 1));  // alpha_{k-1}, as K2 of k-1 formed it (:93)
+    PaddedSums<2> ps;
+    ps.load({d.P(P_RS), d.P(P_RR)}, {d.G, d.G});
     // stencil layout: the operator's loads (none depends on beta) go out
     // while the prologue's are in flight
     constexpr int SW = W < 0 ? -W : 1;
     StencilPre<SW, double> q;
     if (W < 0 && own) stencil_row_pre<SW, double>(A, A.val, row, q);
-    double sums[3];
+    double sums[2];
     ps.reduce(sums);
     if (st != ST_RUN && st != conv_stamp(k)) return;
-    const double alpha = rho_prev / sums[1];  // alpha_{k-1}, bitwise as K2 of k-1 formed it (:93)
-    if (tol2 >= 0.0 && sums[2] <= tol2) {     // every workgroup reaches the same decision
+    if (tol2 >= 0.0 && sums[1] <= tol2) {  // every workgroup reaches the same decision
         if (own) d.x[row] = xo + alpha * po;
         if (leader()) *d.state = conv_stamp(k);
         return;
@@ -444,11 +488,16 @@ __global__ __launch_bounds__(kBlock) void k_fold_dn(PcgDev d) {
             for (int j = 0; j < kWaves; ++j) sa = sa + ws[j];
         }
     }
-    v = block_reduce<RED_SUM>(v);
-    if (threadIdx.x == 0) d.P(P_DN)[blockIdx.x] = v;
     if (d.deflate) {
-        sa = block_reduce<RED_SUM>(sa);
-        if (threadIdx.x == 0) d.P(P_SA)[blockIdx.x] = sa;
+        double red[2] = {v, sa};
+        block_reduce_n<2>(red);
+        if (threadIdx.x == 0) {
+            d.P(P_DN)[blockIdx.x] = red[0];
+            d.P(P_SA)[blockIdx.x] = red[1];
+        }
+    } else {
+        v = block_reduce<RED_SUM>(v);
+        if (threadIdx.x == 0) d.P(P_DN)[blockIdx.x] = v;
     }
 }
 
@@ -468,6 +517,7 @@ __global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, int k) {
             if (leader()) *d.state = ST_BREAK;
             return;
         }
+        store_alpha(d, k, alpha);
         const double malpha = -1.0 * alpha;
         FOR_ROWS(d.n, row) {
             // every operand first, so the row costs one memory round trip
@@ -505,6 +555,7 @@ __global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, int k) {
             if (leader()) *d.state = ST_BREAK;
             return;
         }
+        store_alpha(d, k, alpha);
         const double malpha = -1.0 * alpha;
         while (t.valid()) {
             const int row = t.row;
@@ -557,14 +608,7 @@ __global__ __launch_bounds__(kBlock) void k_m_dir(PcgDev d, float *__restrict__ 
     const float *__restrict__ s = d.s32;
     if (!running(d)) return;
     const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);
-    float a32 = 0.0f;
-    if (pending) {  // alpha_{k-1}, as K2 of k-1 formed it
-        const double zaz = d.sc[S_ZAZ];
-        const bool dfl = deflating(d, zaz);
-        const double dn = sum_partials<RED_SUM>(d.P(P_DN), d.Gdn);
-        const double sap = dfl ? sum_partials<RED_SUM>(d.P(P_SA), d.Gdn) : 0.0;
-        a32 = (float)(rho_slot(d, k - 1) / m_denom(dn, sap, zaz, dfl));
-    }
+    const float a32 = pending ? (float)alpha_slot(d, k - 1) : 0.0f;  // alpha_{k-1}, as K2 of k-1 formed it
     const float beta = (float)(rho / rho_slot(d, k - 1));
     if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
     FOR_ROWS(d.n, row) {
@@ -601,18 +645,18 @@ __global__ __launch_bounds__(kBlock) void k_m_dirprod(EllView A, PcgDev d, const
         po = pold[row];
     }
     const double rho_prev = early_load(d.sc + S_RHO0 + ((k - 1) & 1));
+    const double alpha = early_load(d.sc + S_ALPHA0 + ((k - 1) & 1));  // alpha_{k-1}, as K2 of k-1 formed it
     const double zaz = early_load(d.sc + S_ZAZ);
-    PaddedSums<3> ps;
-    ps.load({d.P(P_RS), dn_part(d, k - 1), sa_part(d, k - 1)}, {d.G, d.Gdn, d.Gdn});
+    PaddedSums<1> ps;
+    ps.load({d.P(P_RS)}, {d.G});
     constexpr int SW = W < 0 ? -W : 1;
     StencilPre<SW, float> q;
     if (W < 0 && own) stencil_row_pre<SW, float>(A, A.val32, row, q);
-    double sums[3];
+    double sums[1];
     ps.reduce(sums);
     if (st != ST_RUN) return;
     const bool dfl = deflating(d, zaz);
-    // alpha_{k-1}, as K2 of k-1 formed it
-    const float a32 = pending ? (float)(rho_prev / m_denom(sums[1], sums[2], zaz, dfl)) : 0.0f;
+    const float a32 = pending ? (float)alpha : 0.0f;
     const float beta = (float)(sums[0] / rho_prev);
     if (leader()) d.sc[S_RHO0 + (k & 1)] = sums[0];
     double acc = 0.0, sa = 0.0;
@@ -659,11 +703,11 @@ __device__ inline void m_spmv(const EllView &A, const PcgDev &d, const float *__
             acc = (double)pr * (double)sum;
             sa = (double)sum;
         }
-        acc = wave_sum(acc);
+        acc = wave_sum_dpp(acc);
         const int wv = c * kWaves + (threadIdx.x >> 6);
         if ((threadIdx.x & 63) == 0) d.chunk[wv] = acc;
         if (d.deflate) {
-            sa = wave_sum(sa);
+            sa = wave_sum_dpp(sa);
             if ((threadIdx.x & 63) == 0) d.chunk_s[wv] = sa;
         }
         return;
@@ -710,22 +754,56 @@ __global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, int k) {
     double rs = 0.0, rr = 0.0;
     if constexpr (!SMALL) {
         if (!running(d)) return;
+        // every prologue load issued before the first is used: 1.A1, the
+        // list of A 1's nonzeros, and the p.Ap and 1.Ap partials (summed
+        // together; alpha goes to its scalar slot, so later kernels need
+        // not reproduce this sum)
         const double zaz = d.sc[S_ZAZ];
+        const int naz_all = d.azl->count;
+        int li[kAzList];
+        float lv[kAzList];
+#pragma unroll
+        for (int j = 0; j < kAzList; ++j) {
+            li[j] = d.azl->idx[j];
+            lv[j] = d.azl->val[j];
+        }
+        double sums[2];
+        sum_partials_n<2>({d.P(P_DN), d.P(P_SA)}, d.Gdn, sums);
         const bool dfl = deflating(d, zaz);
-        const double sap = dfl ? sum_partials<RED_SUM>(d.P(P_SA), d.Gdn) : 0.0;
-        const double dn = m_denom(sum_partials<RED_SUM>(d.P(P_DN), d.Gdn), sap, zaz, dfl);
+        const double sap = dfl ? sums[1] : 0.0;
+        const double dn = m_denom(sums[0], sap, zaz, dfl);
         const double alpha = rho_slot(d, k) / dn;
         if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) {
             if (leader()) *d.state = ST_BREAK;
             return;
         }
+        store_alpha(d, k, alpha);
         const float a32 = (float)alpha, c32 = dfl ? (float)(sap / zaz) : 0.0f;
         const float *__restrict__ az = d.az32;
+        // A 1 from its list of nonzeros when it is short (the same values
+        // as the dense array, without streaming it), else row by row
+        const int naz = dfl ? naz_all : 0;
+        const bool listed = naz <= kAzList;
+#pragma unroll
+        for (int j = 0; j < kAzList; ++j) li[j] = j < naz ? li[j] : -1;
         FOR_ROWS(d.n, row) {
             // every operand first, so the row costs one memory round trip
             const float ro = r[row], apr = Ap[row];
             const float di = precond ? dinv[row] : 0.0f;
-            const float azr = dfl ? az[row] : 0.0f;
+            float azr = 0.0f;
+            if (dfl && !listed) {
+                azr = az[row];
+            } else {
+                // rows compared with the list only in a chunk that holds a
+                // listed row (a workgroup-uniform test per chunk)
+                const int base = __builtin_amdgcn_readfirstlane(row - (int)threadIdx.x);
+                bool hit = false;
+#pragma unroll
+                for (int j = 0; j < kAzList; ++j) hit = hit || (unsigned)(li[j] - base) < (unsigned)kBlock;
+                if (hit)
+#pragma unroll
+                    for (int j = 0; j < kAzList; ++j) azr = row == li[j] ? lv[j] : azr;
+            }
             const float wr = dfl ? apr - c32 * azr : apr;  // (P A p)_row
             const float ri = ro - a32 * wr;
             r[row] = ri;
@@ -758,6 +836,7 @@ __global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, int k) {
             if (leader()) *d.state = ST_BREAK;
             return;
         }
+        store_alpha(d, k, alpha);
         const float a32 = (float)alpha, c32 = dfl ? (float)(sums[1] / zaz) : 0.0f;
         while (t.valid()) {
             const int row = t.row;
@@ -793,13 +872,8 @@ __global__ __launch_bounds__(kBlock) void k_m_status(PcgDev d) {
 }
 
 // x += (xc + alpha_k p) (the last iteration's pending update first), xc = 0
-// (dn, sa: iteration k's p.Ap and 1.Ap partials)
-__global__ __launch_bounds__(kBlock) void k_m_fold(PcgDev d, const float *__restrict__ p, int k,
-                                                   const double *__restrict__ dn, const double *__restrict__ sa) {
-    const double zaz = d.sc[S_ZAZ];
-    const bool dfl = deflating(d, zaz);
-    const double sap = dfl ? sum_partials<RED_SUM>(sa, d.Gdn) : 0.0;
-    const float a32 = (float)(rho_slot(d, k) / m_denom(sum_partials<RED_SUM>(dn, d.Gdn), sap, zaz, dfl));
+__global__ __launch_bounds__(kBlock) void k_m_fold(PcgDev d, const float *__restrict__ p, int k) {
+    const float a32 = (float)alpha_slot(d, k);
     FOR_ROWS(d.n, row) {
         const float xcv = d.xc32[row] + a32 * p[row];
         d.x[row] = d.x[row] + (double)xcv;
@@ -893,10 +967,30 @@ __global__ __launch_bounds__(kBlock) void k_to32(int n, const double *__restrict
 }
 
 // z = 1 on rows [0, n), 0 on [n, n_all); az = 0 on [0, n)
-__global__ __launch_bounds__(kBlock) void k_ones(int n, int n_all, double *__restrict__ z, double *__restrict__ az) {
+__global__ __launch_bounds__(kBlock) void k_ones(int n, int n_all, double *__restrict__ z, double *__restrict__ az,
+                                                 AzList *azl) {
     int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n_all) z[i] = i < n ? 1.0 : 0.0;
     if (i < n) az[i] = 0.0;
+    if (i == 0) azl->count = 0;
+}
+
+// az32 = (float) az, and the list of its nonzeros (entries past kAzList
+// are only counted; the order of the entries does not matter, each row
+// matches at most one)
+__global__ __launch_bounds__(kBlock) void k_az32(int n, const double *__restrict__ az, float *__restrict__ az32,
+                                                 AzList *azl) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float v = (float)az[i];
+    az32[i] = v;
+    if (v != 0.0f) {
+        const int slot = atomicAdd(&azl->count, 1);
+        if (slot < kAzList) {
+            azl->idx[slot] = i;
+            azl->val[slot] = v;
+        }
+    }
 }
 
 }  // namespace gip
@@ -913,6 +1007,7 @@ struct gip_pcg {
     float *p32b;  // its FP32 counterpart
     double *az;   // A 1 (MIXED coarse step)
     float *az32;  // its FP32 copy (MIXED deflation)
+    AzList *azl;  // the FP32 copy's nonzeros, when few
     double *sc, *part, *chunk;
     int nchunk, gdn;
     bool fold;  // nchunk > kMaxRedBlocks: the chunk partials need the fold kernel;
@@ -957,6 +1052,7 @@ static PcgDev make_dev(gip_pcg *S, const double *b, double *x, int precond) {
     d.chunk = S->fold ? S->chunk : S->part + P_DN * kMaxRedBlocks;
     d.chunk_s = S->fold ? S->chunk + (size_t)S->nchunk * kWaves : nullptr;
     d.az32 = S->az32;
+    d.azl = S->azl;
     d.deflate = 0;
     d.x = x;
     d.r = S->r;
@@ -1047,7 +1143,7 @@ int gip_pcg_destroy(gip_pcg *S) {
     hipSetDevice(S->ctx->device);
     hipStreamSynchronize(S->ctx->stream);
     void *ptrs[] = {S->r,    S->p,      S->s,  S->Ap, S->dinv, S->r32,   S->p32,  S->s32,  S->Ap32,
-                    S->xc32, S->dinv32, S->az, S->sc, S->part, S->chunk, S->state, S->p2, S->p32b, S->az32};
+                    S->xc32, S->dinv32, S->az, S->sc, S->part, S->chunk, S->state, S->p2, S->p32b, S->az32, S->azl};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (S->h_state) hipHostFree(S->h_state);
@@ -1069,6 +1165,8 @@ static int ensure_mixed(gip_pcg *S) {
     for (float **b : bufs) GIP_HIP_TRY(hipMalloc(b, n * 4));
     GIP_HIP_TRY(hipMalloc(&S->az, n * 8));
     GIP_HIP_TRY(hipMalloc(&S->az32, n * 4));
+    GIP_HIP_TRY(hipMalloc(&S->azl, sizeof(AzList)));
+    GIP_HIP_TRY(hipMemsetAsync(S->azl, 0, sizeof(AzList), S->ctx->stream));
     // the FP32 direction is an SpMV input: ghost columns included, and zero
     // there (the correction leaves the ghost values of x unchanged)
     GIP_HIP_TRY(hipMalloc(&S->p32, na * 4));
@@ -1091,12 +1189,12 @@ static int coarse_setup(gip_pcg *S) {
     gip_ctx *c = S->ctx;
     if (S->n_all == 0) return GIP_OK;
     double *z = S->p;
-    k_ones<<<(S->n_all + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(S->n, S->n_all, z, S->az);
+    k_ones<<<(S->n_all + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(S->n, S->n_all, z, S->az, S->azl);
     GIP_LAUNCH_CHECK("coarse vector");
     int rc = spmv_launch(c, S->A, z, S->az, 1.0, 0.0, 0, nullptr);
     if (!rc) rc = launch_reduce(c, RED_SUM, S->n, S->az, z, S->sc + S_ZAZ);
     if (!rc && S->n) {
-        k_to32<<<(S->n + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(S->n, S->az, S->az32);
+        k_az32<<<(S->n + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(S->n, S->az, S->az32, S->azl);
         GIP_LAUNCH_CHECK("coarse vector fp32");
     }
     return rc;
@@ -1130,16 +1228,10 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
     const int pollk = o->poll > 0 ? o->poll : 16;
     const int use_coarse = o->coarse >= 0 ? 1 : 0;
     d.deflate = mixed && use_coarse;
-    // direction and p.Ap partials of iteration k: one buffer and one partial
-    // array with the fold; alternating by parity in the fused iteration
+    // direction of iteration k: one buffer with the fold, alternating by
+    // parity in the fused iteration
     auto pdir = [&](int k) { return (!S->fold && (k & 1)) ? S->p2 : S->p; };
     auto pdir32 = [&](int k) { return (!S->fold && (k & 1)) ? S->p32b : S->p32; };
-    auto pdn = [&](int k) {
-        return (const double *)(S->part + (size_t)((!S->fold && (k & 1)) ? P_DN2 : P_DN) * kMaxRedBlocks);
-    };
-    auto psa = [&](int k) {
-        return (const double *)(S->part + (size_t)((!S->fold && (k & 1)) ? P_SA2 : P_SA) * kMaxRedBlocks);
-    };
     // time_products: events around one product launch (k >= 1) per host
     // poll, read after the poll's synchronisation
     const bool timing = o->time_products != 0;
@@ -1208,7 +1300,7 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
                 sample_read();
             }
         }
-        if (k > 0) k_x_flush<<<S->grid, kBlock, 0, st>>>(d, pdir(k - 1), k - 1, pdn(k - 1));
+        if (k > 0) k_x_flush<<<S->grid, kBlock, 0, st>>>(d, pdir(k - 1), k - 1);
         k_fin<<<1, kBlock, 0, st>>>(d);
         GIP_LAUNCH_CHECK("pcg final test");
     } else {
@@ -1252,7 +1344,7 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
             const double rr32 = S->h_sc[S_RR], tol2 = S->h_sc[S_TOL2];
             const bool near_target = tol2 >= 0.0 && rr32 <= tol2;
             if (rr32 <= inner2 * last_rr || near_target || k >= o->maxit) {
-                k_m_fold<<<S->grid, kBlock, 0, st>>>(d, pdir32(k - 1), k - 1, pdn(k - 1), psa(k - 1));
+                k_m_fold<<<S->grid, kBlock, 0, st>>>(d, pdir32(k - 1), k - 1);
                 pending = 0;
                 DISPATCH_W(A, k_m_resid, v, d);
                 k_m_coarse<<<S->grid, kBlock, 0, st>>>(d, use_coarse);
