@@ -344,7 +344,13 @@ def test_pcg_mixed_precision_reaches_fp64_residual(dev, cav46):
     assert true_rel <= 1e-10 * 1.01, true_rel
     want, st, it, rel = pyoracle.pcg(pyoracle.Csr.of(L), b, x0, n, 5000, tol=1e-12)
     assert np.linalg.norm(x - want) <= 1e-7 * np.linalg.norm(want)
-    print(f"[mixed] iters {r.iters} corrections {r.corrections} true {true_rel:.3e} fp64 iters {it}")
+    r64 = solver.solve(dev.to_device(b), dx, dev.to_device(x0), maxit=5000, tol=1e-10)
+    print(f"[mixed] iters {r.iters} corrections {r.corrections} true {true_rel:.3e} "
+          f"fp64 iters {r64.iters} (oracle to 1e-12: {it})")
+    # the deflated inner iteration (the small operators' dense A1 read)
+    # needs fewer iterations than FP64 PCG to the same residual; without the
+    # deflation it needed more (608 against 484 on the 46^3 benchmark case)
+    assert r64.status == 0 and r.iters <= 0.85 * r64.iters, (r.iters, r64.iters)
 
 
 @pytest.mark.parametrize("coarse", [True, False])
