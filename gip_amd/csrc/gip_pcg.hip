@@ -768,7 +768,10 @@ __global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, int k) {
             lv[j] = d.azl->val[j];
         }
         double sums[2];
-        sum_partials_n<2>({d.P(P_DN), d.P(P_SA)}, d.Gdn, sums);
+        if (d.Gdn <= kBlock)  // one round of the uniform loop (up to 256^3)
+            sum_partials_n<2>({d.P(P_DN), d.P(P_SA)}, d.Gdn, sums);
+        else  // all slots in one round trip
+            sum_padded<2>({d.P(P_DN), d.P(P_SA)}, {d.Gdn, d.Gdn}, sums);
         const bool dfl = deflating(d, zaz);
         const double sap = dfl ? sums[1] : 0.0;
         const double dn = m_denom(sums[0], sap, zaz, dfl);
