@@ -511,7 +511,12 @@ __global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, int k) {
     double rs = 0.0, rr = 0.0;
     if constexpr (!SMALL) {
         if (!running(d)) return;
-        const double dn = sum_partials<RED_SUM>(d.P(P_DN), d.Gdn);
+        double dsum[1];
+        if (d.Gdn <= kBlock)  // one round of the uniform loop (up to 256^3)
+            sum_partials_n<1>({d.P(P_DN)}, d.Gdn, dsum);
+        else
+            dsum[0] = sum_partials<RED_SUM>(d.P(P_DN), d.Gdn);
+        const double dn = dsum[0];
         const double alpha = rho_slot(d, k) / dn;  // :93
         if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) {
             if (leader()) *d.state = ST_BREAK;
