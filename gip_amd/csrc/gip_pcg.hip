@@ -18,10 +18,11 @@
 // workgroups compute bit-identical scalars.  The kernel boundary orders the
 // stores before the loads; no atomics, no device-scope fences (a fence per
 // workgroup costs an L2 write-back on every XCD).  Scalars that must survive
-// a kernel (rho of the previous iteration, the state word) are written by
-// workgroup 0 only: rho into the slot of its iteration's parity, which no
-// workgroup of the same kernel reads; the state word as described at
-// running().
+// a kernel (rho and alpha of the last two iterations, the state word) are
+// written by workgroup 0 only: rho and alpha into the slots of their
+// iteration's parity, which no workgroup of the same kernel reads (a later
+// kernel that needs alpha reads its slot instead of re-reducing the p.Ap
+// partials); the state word as described at running().
 // The direction update has its own streaming kernel so that the product
 // gathers one vector (p) per neighbour; forming p inside the product would
 // gather s and the old p for every neighbour.  Every kernel reads the
