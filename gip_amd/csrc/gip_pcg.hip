@@ -418,9 +418,12 @@ struct DirOperand {
 // buffer, Ap = A p_k with p_k formed at every column read, and the p_k.Ap
 // partials of iteration k (read by K2 of k while this kernel's own
 // partials of k-1 stay intact: two partial arrays by parity).
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_iter_dirprod(EllView A, PcgDev d, const double *__restrict__ pold,
-                                                         double *__restrict__ pnew, int k) {
+// TOL: a solve with a tolerance (the r.r partials and the convergence
+// test); without one (the reference's fixed iteration count) the prologue
+// sums the rho partials only.
+template <int W, bool TOL>
+__device__ inline void iter_dirprod(const EllView &A, const PcgDev &d, const double *__restrict__ pold,
+                                    double *__restrict__ pnew, int k) {
     const int st = early_load(d.state);
     const int c = xcd_block(blockIdx.x, gridDim.x);
     const int row = c * kBlock + threadIdx.x;
@@ -431,20 +434,24 @@ __global__ __launch_bounds__(kBlock) void k_iter_dirprod(EllView A, PcgDev d, co
         si = d.s[row];
         po = pold[row];
     }
-    const double tol2 = early_load(d.sc + S_TOL2);
+    const double tol2 = TOL ? early_load(d.sc + S_TOL2) : -1.0;
     const double rho_prev = early_load(d.sc + S_RHO0 + ((k - 1) & 1));
     const double alpha = early_load(d.sc + S_ALPHA0 + ((k - 1) & 1));  // alpha_{k-1}, as K2 of k-1 formed it (:93)
-    PaddedSums<2> ps;
-    ps.load({d.P(P_RS), d.P(P_RR)}, {d.G, d.G});
+    constexpr int NS = TOL ? 2 : 1;
+    PaddedSums<NS> ps;
+    if constexpr (TOL)
+        ps.load({d.P(P_RS), d.P(P_RR)}, {d.G, d.G});
+    else
+        ps.load({d.P(P_RS)}, {d.G});
     // stencil layout: the operator's loads (none depends on beta) go out
     // while the prologue's are in flight
     constexpr int SW = W < 0 ? -W : 1;
     StencilPre<SW, double> q;
     if (W < 0 && own) stencil_row_pre<SW, double>(A, A.val, row, q);
-    double sums[2];
+    double sums[NS];
     ps.reduce(sums);
     if (st != ST_RUN && st != conv_stamp(k)) return;
-    if (tol2 >= 0.0 && sums[1] <= tol2) {  // every workgroup reaches the same decision
+    if (TOL && tol2 >= 0.0 && sums[NS - 1] <= tol2) {  // every workgroup reaches the same decision
         if (own) d.x[row] = xo + alpha * po;
         if (leader()) *d.state = conv_stamp(k);
         return;
@@ -470,6 +477,18 @@ __global__ __launch_bounds__(kBlock) void k_iter_dirprod(EllView A, PcgDev d, co
     double acc = own ? pr * sum : 0.0;
     acc = block_reduce<RED_SUM>(acc);
     if (threadIdx.x == 0) dn_part(d, k)[c] = acc;
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_iter_dirprod(EllView A, PcgDev d, const double *__restrict__ pold,
+                                                         double *__restrict__ pnew, int k) {
+    iter_dirprod<W, true>(A, d, pold, pnew, k);
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_iter_dirprod_fixed(EllView A, PcgDev d, const double *__restrict__ pold,
+                                                               double *__restrict__ pnew, int k) {
+    iter_dirprod<W, false>(A, d, pold, pnew, k);
 }
 
 // the per-wavefront partials of the long products, folded in index order:
@@ -1295,8 +1314,10 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
                     k_iter_update<false><<<S->grid, kBlock, 0, st>>>(d, k);
                 } else {
                     const bool smp = sample_begin(k);
-                    if (k > 0)
+                    if (k > 0 && o->tol > 0.0)
                         DISPATCH_WG(A, S->nchunk, k_iter_dirprod, v, d, pdir(k - 1), pdir(k), k);
+                    else if (k > 0)  // fixed iteration count: no r.r test
+                        DISPATCH_WG(A, S->nchunk, k_iter_dirprod_fixed, v, d, pdir(k - 1), pdir(k), k);
                     else
                         DISPATCH_WG(A, S->nchunk, k_iter_spmv_small, v, d, S->p);
                     sample_end(smp);
