@@ -80,16 +80,21 @@ class Adjoint:
             pass
 
 
-def fd_check(adj: Adjoint, lid, u0, v0, w0, ndir=16, eps=1e-6, seed=0):
+def fd_check(adj: Adjoint, lid, u0, v0, w0, ndir=16, eps=1e-6, seed=0, space="both"):
     """Central finite differences of J along random directions against the
-    adjoint gradient; returns (directional derivatives adjoint, fd, rel errors)."""
+    adjoint gradient; returns (J, directional derivatives adjoint, fd, rel
+    errors).  `space` picks the directions: "lid" (the lid profile only),
+    "velocity" (the initial u, v, w only) or "both" (all together), so that
+    each gradient can be checked on its own."""
+    if space not in ("lid", "velocity", "both"):
+        raise ValueError(space)
     J0 = adj.forward(lid, u0, v0, w0)
     gl, gu, gv, gw = adj.backward()
     rng = np.random.default_rng(seed)
     ad, fd = [], []
     for _ in range(ndir):
-        dl = rng.standard_normal(lid.size)
-        du, dv, dw = (rng.standard_normal(u0.size) for _ in range(3))
+        dl = rng.standard_normal(lid.size) if space != "velocity" else np.zeros(lid.size)
+        du, dv, dw = (rng.standard_normal(u0.size) if space != "lid" else np.zeros(u0.size) for _ in range(3))
         jp = adj.forward(lid + eps * dl, u0 + eps * du, v0 + eps * dv, w0 + eps * dw)
         jm = adj.forward(lid - eps * dl, u0 - eps * du, v0 - eps * dv, w0 - eps * dw)
         fd.append((jp - jm) / (2 * eps))

@@ -28,6 +28,7 @@ DATA = ROOT / "data"
 REFERENCE = pathlib.Path(os.environ.get("GIP_REFERENCE", "/root/reference"))
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+ROCM_LIB = pathlib.Path(os.environ.get("ROCM_PATH", "/opt/rocm")) / "lib"
 # no multiply-add contraction anywhere: products and sums round separately,
 # as in the x86-64 CPU specification; packed FP32 instructions disabled.
 HIP_FLAGS = [
@@ -81,9 +82,11 @@ def build_dns():
     BIN.mkdir(exist_ok=True)
     srcs = [SPEC / s for s in SPEC_SOURCES] + [SPEC / "gip_dns_capi.cpp"]
     flags = ["-O2", "-fPIC", "-std=c++17", "-fopenmp", *spec_includes()]
-    _run(["g++", *flags, "-shared", *srcs, f"-L{PKG}", "-lgip_hip", "-lgip_cavity",
+    flags += ["-isystem", str(ROCM_LIB.parent / "include")]  # roctx ranges (gip_markers.h)
+    roctx = [f"-L{ROCM_LIB}", "-lrocprofiler-sdk-roctx", f"-Wl,-rpath,{ROCM_LIB}"]
+    _run(["g++", *flags, "-shared", *srcs, f"-L{PKG}", "-lgip_hip", "-lgip_cavity", *roctx,
           "-Wl,-rpath,$ORIGIN", "-o", PKG / "libgip_dns.so"])
-    _run(["g++", *flags, SPEC / "DrivenCavity.cpp", f"-L{PKG}", "-lgip_dns", "-lgip_hip", "-lgip_cavity",
+    _run(["g++", *flags, SPEC / "DrivenCavity.cpp", f"-L{PKG}", "-lgip_dns", "-lgip_hip", "-lgip_cavity", *roctx,
           "-Wl,-rpath,$ORIGIN/..", "-o", BIN / "DrivenCavityHIP"])
 
 
@@ -91,8 +94,10 @@ def build_oracle():
     out = ORACLE / "_build"
     out.mkdir(exist_ok=True)
     # -O2 without -march: no FMA contraction on x86-64, same roundings as the reference build
-    _run(["gcc", "-O2", "-std=c99", "-fPIC", "-shared", ORACLE / "gip_oracle.c", "-lm", "-o", out / "liboracle.so"])
-    _run(["gcc", "-O2", "-std=c99", ORACLE / "orc_cavity.c", ORACLE / "gip_oracle.c", "-lm", "-o", out / "orc_cavity"])
+    _run(["gcc", "-O2", "-std=c99", "-fopenmp", "-fPIC", "-shared", ORACLE / "gip_oracle.c", "-lm", "-o",
+          out / "liboracle.so"])
+    _run(["gcc", "-O2", "-std=c99", "-fopenmp", ORACLE / "orc_cavity.c", ORACLE / "gip_oracle.c", "-lm", "-o",
+          out / "orc_cavity"])
 
 
 def build_ref():

@@ -1,6 +1,7 @@
 // Device-side reduction helpers shared by the vector and PCG kernels.
-// Wave64 shuffle tree, then an LDS tree across the workgroup's wavefronts,
-// then the last workgroup to finish combines the per-workgroup partials.
+// Wave64 shuffle tree, then an LDS tree across the workgroup's wavefronts;
+// per-workgroup partials are stored with plain stores and combined, in a
+// fixed order, by a later kernel (no atomics, no device-scope fences).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -78,61 +79,6 @@ __device__ inline void block_reduce_n(double (&v)[M]) {
 #pragma unroll
             for (int k = 1; k < kBlock / 64; ++k) v[m] = v[m] + sh[m][k];
         }
-}
-
-// Publish this workgroup's value (thread 0's v) and, in the workgroup that
-// finishes last, combine all partials in index order into *out (thread 0).
-// Returns true in every thread of that last workgroup.  The counter is
-// reset to zero by the last workgroup, ready for the next launch.
-template <int OP>
-__device__ inline bool last_block_reduce(double v, double *partials, unsigned int *counter, double *out) {
-    __shared__ unsigned int s_ticket;
-    if (threadIdx.x == 0) {
-        partials[blockIdx.x] = v;
-        __threadfence();
-        s_ticket = atomicAdd(counter, 1u);
-    }
-    __syncthreads();
-    if (s_ticket != gridDim.x - 1) return false;
-    __threadfence();
-    double acc = RedOp<OP>::identity();
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock)
-        acc = RedOp<OP>::combine(acc, __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    acc = block_reduce<OP>(acc);
-    if (threadIdx.x == 0) {
-        *out = acc;
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return true;
-}
-
-// Two sums published with one ticket: in the last workgroup thread 0 gets
-// both totals (out0, out1).  partials holds 2 * gridDim.x values.
-__device__ inline bool last_block_reduce2(double v0, double v1, double *partials, unsigned int *counter, double *out0,
-                                          double *out1) {
-    __shared__ unsigned int s_ticket2;
-    if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = v0;
-        partials[2 * blockIdx.x + 1] = v1;
-        __threadfence();
-        s_ticket2 = atomicAdd(counter, 1u);
-    }
-    __syncthreads();
-    if (s_ticket2 != gridDim.x - 1) return false;
-    __threadfence();
-    double a0 = 0.0, a1 = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock) {
-        a0 = a0 + __hip_atomic_load(&partials[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        a1 = a1 + __hip_atomic_load(&partials[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    a0 = block_reduce<RED_SUM>(a0);
-    a1 = block_reduce<RED_SUM>(a1);
-    if (threadIdx.x == 0) {
-        *out0 = a0;
-        *out1 = a1;
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return true;
 }
 
 // M sums of partials at once, for partial arrays of kMaxRedBlocks doubles

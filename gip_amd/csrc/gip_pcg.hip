@@ -121,19 +121,24 @@ struct PcgDev {
     __device__ double *P(int k) const { return part + k * kMaxRedBlocks; }
 };
 
-// A kernel reads the state word once, with a plain load: words written by
-// earlier kernels are ordered by the kernel boundary.  A workgroup may or
-// may not see the word workgroup 0 of its own kernel writes, and each
-// writer makes both outcomes give the same result: the direction kernel
-// stamps a convergence with its iteration, and a workgroup that reads its
-// own kernel's stamp takes the convergence path every workgroup computes;
-// a breakdown leaves the iterates undefined, so skipping work is harmless.
-__device__ inline bool running(const PcgDev &d) { return *d.state == ST_RUN; }
+// The state word is written by workgroup 0 of a kernel while the other
+// workgroups of that kernel may be reading it, so every access is an atomic
+// one (relaxed, device scope): a reader sees either the word an earlier
+// kernel left (ordered by the kernel boundary) or the one workgroup 0 of
+// its own kernel is writing.  Each writer makes both outcomes give the same
+// result: the direction kernel stamps a convergence with its iteration, and
+// a workgroup that reads its own kernel's stamp takes the convergence path
+// every workgroup computes; a breakdown leaves the iterates undefined, so
+// skipping work is harmless.  An atomic load is an ordered memory
+// reference: the compiler issues it where it is written, together with the
+// kernel's other early loads.
+__device__ inline int state_load(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline void state_store(int *p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline bool running(const PcgDev &d) { return state_load(d.state) == ST_RUN; }
 // A word read at the top of a kernel and used only after its prologue: an
 // atomic load (workgroup scope, relaxed: a plain load instruction) is an
 // ordered memory reference, so the compiler issues it where it is written,
 // together with the other loads, instead of sinking it to its first use.
-__device__ inline int early_load(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
 __device__ inline double early_load(const double *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
@@ -254,7 +259,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_fin(PcgDev d, double tol) {
         d.sc[S_BB] = bb;
         d.sc[S_RR] = rr;
         d.sc[S_TOL2] = tol2;
-        *d.state = (tol > 0.0 && rr <= tol2) ? ST_CONV : ST_RUN;
+        state_store(d.state, (tol > 0.0 && rr <= tol2) ? ST_CONV : ST_RUN);
         *d.iters = 0;
     }
 }
@@ -314,7 +319,7 @@ struct SelfOperand {
 __global__ __launch_bounds__(kBlock) void k_iter_dir(PcgDev d, double *__restrict__ p, int k) {
     double *__restrict__ x = d.x;
     const double *__restrict__ s = d.s;
-    const int st = *d.state;
+    const int st = state_load(d.state);
     if (st != ST_RUN && st != conv_stamp(k)) return;
     const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);  // :107
     const double alpha = alpha_slot(d, k - 1);
@@ -323,7 +328,7 @@ __global__ __launch_bounds__(kBlock) void k_iter_dir(PcgDev d, double *__restric
         const double rr = sum_partials<RED_SUM>(d.P(P_RR), d.G);
         if (rr <= tol2) {  // every workgroup reaches the same decision
             x_update(d, p, alpha);
-            if (leader()) *d.state = conv_stamp(k);
+            if (leader()) state_store(d.state, conv_stamp(k));
             return;
         }
     }
@@ -370,7 +375,7 @@ __device__ inline void iter_spmv(const EllView &A, const PcgDev &d, const double
         if ((threadIdx.x & 63) == 0) d.chunk[c * kWaves + (threadIdx.x >> 6)] = acc;
         return;
     }
-    const int st = early_load(d.state);
+    const int st = state_load(d.state);
     const int c = xcd_block(blockIdx.x, gridDim.x);
     const int row = c * kBlock + threadIdx.x;
     double pr = 0.0, sum = 0.0;
@@ -424,7 +429,7 @@ struct DirOperand {
 template <int W, bool TOL>
 __device__ inline void iter_dirprod(const EllView &A, const PcgDev &d, const double *__restrict__ pold,
                                     double *__restrict__ pnew, int k) {
-    const int st = early_load(d.state);
+    const int st = state_load(d.state);
     const int c = xcd_block(blockIdx.x, gridDim.x);
     const int row = c * kBlock + threadIdx.x;
     const bool own = row < d.n;
@@ -453,7 +458,7 @@ __device__ inline void iter_dirprod(const EllView &A, const PcgDev &d, const dou
     if (st != ST_RUN && st != conv_stamp(k)) return;
     if (TOL && tol2 >= 0.0 && sums[NS - 1] <= tol2) {  // every workgroup reaches the same decision
         if (own) d.x[row] = xo + alpha * po;
-        if (leader()) *d.state = conv_stamp(k);
+        if (leader()) state_store(d.state, conv_stamp(k));
         return;
     }
     const double beta = sums[0] / rho_prev;  // :107-109
@@ -539,7 +544,7 @@ __global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, int k) {
         const double dn = dsum[0];
         const double alpha = rho_slot(d, k) / dn;  // :93
         if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) {
-            if (leader()) *d.state = ST_BREAK;
+            if (leader()) state_store(d.state, ST_BREAK);
             return;
         }
         store_alpha(d, k, alpha);
@@ -564,7 +569,7 @@ __global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, int k) {
     } else {
         ThreadRows t(d.n);
         double ro = 0.0, apr = 0.0, di = 0.0;
-        const int st = early_load(d.state);
+        const int st = state_load(d.state);
         if (t.valid()) {
             ro = r[t.row];
             apr = Ap[t.row];
@@ -577,7 +582,7 @@ __global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, int k) {
         if (st != ST_RUN) return;
         const double alpha = rho / dn;
         if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) {
-            if (leader()) *d.state = ST_BREAK;
+            if (leader()) state_store(d.state, ST_BREAK);
             return;
         }
         store_alpha(d, k, alpha);
@@ -619,7 +624,7 @@ __global__ __launch_bounds__(kBlock) void k_fin(PcgDev d) {
     if (threadIdx.x == 0) {
         d.sc[S_RR] = rr;
         const double tol2 = d.sc[S_TOL2];
-        if (*d.state == ST_RUN && tol2 >= 0.0 && rr <= tol2) *d.state = ST_CONV;
+        if (state_load(d.state) == ST_RUN && tol2 >= 0.0 && rr <= tol2) state_store(d.state, ST_CONV);
     }
 }
 
@@ -659,7 +664,7 @@ struct DirOperand32 {
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_m_dirprod(EllView A, PcgDev d, const float *__restrict__ pold,
                                                       float *__restrict__ pnew, int k, int pending) {
-    const int st = early_load(d.state);
+    const int st = state_load(d.state);
     const int c = xcd_block(blockIdx.x, gridDim.x);
     const int row = c * kBlock + threadIdx.x;
     const bool own = row < d.n;
@@ -737,7 +742,7 @@ __device__ inline void m_spmv(const EllView &A, const PcgDev &d, const float *__
         }
         return;
     }
-    const int st = early_load(d.state);
+    const int st = state_load(d.state);
     const int c = xcd_block(blockIdx.x, gridDim.x);
     const int row = c * kBlock + threadIdx.x;
     float pr = 0.0f, sum = 0.0f;
@@ -802,7 +807,7 @@ __global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, int k) {
         const double dn = m_denom(sums[0], sap, zaz, dfl);
         const double alpha = rho_slot(d, k) / dn;
         if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) {
-            if (leader()) *d.state = ST_BREAK;
+            if (leader()) state_store(d.state, ST_BREAK);
             return;
         }
         store_alpha(d, k, alpha);
@@ -844,7 +849,7 @@ __global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, int k) {
         ThreadRows t(d.n);
         float ro = 0.0f, apr = 0.0f, di = 0.0f, azr = 0.0f;
         const float *__restrict__ az = d.az32;
-        const int st = early_load(d.state);
+        const int st = state_load(d.state);
         const bool want_az = d.deflate != 0;
         if (t.valid()) {
             ro = r[t.row];
@@ -861,7 +866,7 @@ __global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, int k) {
         const double dn = m_denom(sums[0], sums[1], zaz, dfl);
         const double alpha = rho / dn;
         if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) {
-            if (leader()) *d.state = ST_BREAK;
+            if (leader()) state_store(d.state, ST_BREAK);
             return;
         }
         store_alpha(d, k, alpha);
@@ -977,7 +982,7 @@ __global__ __launch_bounds__(kBlock) void k_m_corr_fin(PcgDev d, int k, int firs
         d.sc[S_TRUE_RR] = rrt;
         d.sc[S_RR] = rrt;
         const double tol2 = d.sc[S_TOL2];
-        if (*d.state == ST_RUN && tol2 >= 0.0 && rrt <= tol2) *d.state = ST_CONV;
+        if (state_load(d.state) == ST_RUN && tol2 >= 0.0 && rrt <= tol2) state_store(d.state, ST_CONV);
     }
 }
 

@@ -714,6 +714,27 @@ int gip_dcsrmv_vals(gip_ctx *c, const gip_mat *A, const double *x, gip_mat *B) {
     return GIP_OK;
 }
 
+// B's values (CSR order) <- alpha A x + beta B's values, rows of A = nnz of
+// B, empty rows of A leave their value (GIP_MatrixMulticore.cpp:125-143):
+// the product's alpha/beta epilogue reads and writes B's stored values
+// through the CSR -> storage map.
+int gip_dcsrmv_vals_ab(gip_ctx *c, const gip_mat *A, const double *x, gip_mat *B, double alpha, double beta) {
+    if (!c || !A || !B || (A->nrows && !x)) return set_error(GIP_EINVAL, "gip_dcsrmv_vals_ab: bad argument");
+    if (A->nrows != B->nnz) return set_error(GIP_EINVAL, "gip_dcsrmv_vals_ab: rows of A must equal nnz of B");
+    int rc;
+    if (B->format == GIP_FMT_ELL || B->format == GIP_FMT_STENCIL)
+        rc = spmv_launch(c, A, x, B->d_ell_val, alpha, beta, 1, B->d_csr2ell);
+    else if (B->format == GIP_FMT_DIAG)
+        rc = spmv_launch(c, A, x, B->d_diag, alpha, beta, 1, nullptr);
+    else
+        rc = spmv_launch(c, A, x, B->d_val, alpha, beta, 1, nullptr);
+    if (rc) return rc;
+    if (B->d_ell_val32) k_to_f32<<<nblocks(B->ell_stored), kBlock, 0, c->stream>>>(B->ell_stored, B->d_ell_val, B->d_ell_val32);
+    if (B->d_val32) k_to_f32<<<nblocks(B->nnz), kBlock, 0, c->stream>>>(B->nnz, B->d_val, B->d_val32);
+    GIP_LAUNCH_CHECK("gip_dcsrmv_vals_ab");
+    return GIP_OK;
+}
+
 int gip_scsrmv(gip_ctx *c, const gip_mat *A, const float *x, float *y) {
     if (!c || !A || (A->nrows && (!x || !y))) return set_error(GIP_EINVAL, "gip_scsrmv: bad argument");
     return spmv32_launch(c, A, x, y);

@@ -66,6 +66,7 @@ SIGNATURES = {
     "gip_dcsrmv": ([c_void_p, c_void_p, c_void_p, c_void_p], c_int),
     "gip_dcsrmv_ab": ([c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double], c_int),
     "gip_dcsrmv_vals": ([c_void_p, c_void_p, c_void_p, c_void_p], c_int),
+    "gip_dcsrmv_vals_ab": ([c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double], c_int),
     "gip_dcsrmv_t": ([c_void_p, c_void_p, c_void_p, c_void_p, c_int32], c_int),
     "gip_scsrmv": ([c_void_p, c_void_p, c_void_p, c_void_p], c_int),
     "gip_dfill": ([c_void_p, c_int32, c_void_p, c_double], c_int),
@@ -262,6 +263,11 @@ class Matrix:
         if x.n < self.cols_needed or self.nrows != B.nnz:
             raise ValueError("spmv_vals: input too short or rows(A) != nnz(B)")
         check(lib().gip_dcsrmv_vals(self.dev.ctx, self.ptr, x.ptr, B.ptr))
+
+    def spmv_vals_ab(self, x, B: "Matrix", alpha, beta):
+        if x.n < self.cols_needed or self.nrows != B.nnz:
+            raise ValueError("spmv_vals_ab: input too short or rows(A) != nnz(B)")
+        check(lib().gip_dcsrmv_vals_ab(self.dev.ctx, self.ptr, x.ptr, B.ptr, alpha, beta))
 
     def spmv_t(self, x, y, ncols_out):
         if x.n < self.nrows or y.n < ncols_out or ncols_out < self.cols_needed:

@@ -3,8 +3,13 @@
 // templated integrator.
 //   DrivenCavityHIP <name> [--steps N] [--gamma G] [--pcg-maxit K] [--pcg-tol T]
 //                          [--pcg-mode reference|fused|mixed] [--full] [--root DIR] [--device D]
+//                          [--timing]
 // Reads InputFiles/... (one partition, <name>_1p-0) below DIR (default: the
 // working directory) and prints one line per step in the reference's format.
+// --timing times every step with the HIP timer specification (a channel
+// started before and stopped after the step, GIP_Timer.h:6-26) and prints
+// its last, summed and mean times; each phase of a step is also a profiler
+// range (roctx) named after it.
 #include <cstdlib>
 #include <cstring>
 #include <exception>
@@ -12,6 +17,7 @@
 
 #include "GIP_DNS.h"
 #include "HIP.h"
+#include "gip_markers.h"
 
 typedef GIP_MatrixHIP Matrix;
 typedef GIP_VectorHIP Vector;
@@ -21,7 +27,7 @@ typedef GIP_ArchHIP Architecture;
 int main(int argc, char **argv) {
     if (argc < 2) {
         std::cout << "DrivenCavityHIP <Mesh_name> [--steps N] [--gamma G] [--pcg-maxit K] [--pcg-tol T] "
-                     "[--pcg-mode reference|fused|mixed] [--full] [--root DIR] [--device D]"
+                     "[--pcg-mode reference|fused|mixed] [--full] [--root DIR] [--device D] [--timing]"
                   << std::endl;
         return 0;
     }
@@ -30,6 +36,7 @@ int main(int argc, char **argv) {
     p.setDouble("gamma", 1e-3);
     p.setInt("print", 1);
     std::string root;
+    bool timing = false;
     for (int a = 2; a < argc; ++a) {
         std::string k = argv[a];
         auto next = [&]() -> const char * {
@@ -47,6 +54,7 @@ int main(int argc, char **argv) {
         else if (k == "--full") p.setInt("print", 2);
         else if (k == "--root") root = next();
         else if (k == "--device") p.setInt("device", std::atoi(next()));
+        else if (k == "--timing") timing = true;
         else {
             std::cerr << "unknown option " << k << std::endl;
             return 1;
@@ -57,7 +65,21 @@ int main(int argc, char **argv) {
         driven.setRoot(root);
         driven.configure(p);
         driven.setUp();
-        driven.integrate();
+        if (!timing) {
+            driven.integrate();
+            return 0;
+        }
+        const int steps = p.getInt("steps");
+        driven.setPhaseHook(gip_phase_ranges);
+        GIP_TimerHIP timer(&driven.node());
+        for (int k = 0; k < steps; ++k) {
+            GIP_StepInfo info;
+            timer.startChannel();
+            driven.step(&info);
+            timer.stopAllChannels();
+        }
+        std::printf("timer steps %d last_s %.9e sum_s %.9e avg_s %.9e\n", steps, timer.getTimeG(), timer.getSumG(),
+                    timer.getTimeAvg());
     } catch (const std::exception &e) {  // unreadable or unsupported input files
         std::cerr << e.what() << std::endl;
         return 2;

@@ -11,7 +11,7 @@
 #include <map>
 #include <vector>
 
-#include "GIP_Arch.h"
+#include "../../AbstractInterfaces/GIP_Arch.h"
 #include "gip_hip.h"
 
 class GIP_ArchHIP : public GIP_Arch {

@@ -94,21 +94,15 @@ void GIP_MatrixHIP::spmv(GIP_Vector *x, GIP_Matrix *B) {
 }
 
 void GIP_MatrixHIP::spmv(GIP_Vector *x, GIP_Matrix *B, double alpha, double beta) {
-    // B.values = alpha A x + beta B.values, through B's values in CSR order
+    // B.values = alpha A x + beta B.values on the device, through B's
+    // values in CSR order (GIP_MatrixMulticore.cpp:125-143)
     GIP_MatrixHIP *Bh = dynamic_cast<GIP_MatrixHIP *>(B);
     if (!Bh) {
         std::fprintf(stderr, "gip_hip: spmv into the values of a non-HIP operator\n");
         std::abort();
     }
-    GIP_ArchHIP *node = GIP_ArchHIP::of(myArch);
-    size_t bytes = sizeof(double) * (size_t)(Bh->nnz > 0 ? Bh->nnz : 1);
-    double *tmp = static_cast<double *>(node->acquire(bytes));
-    Bh->downloadValues();
-    gip_check(gip_memcpy_h2d(ctx(), tmp, Bh->csrValA, sizeof(double) * (size_t)Bh->nnz), "values upload");
-    gip_check(gip_dcsrmv_ab(ctx(), dmat, x->getDevicePtr(), tmp, alpha, beta), "spmv_ab into values");
-    gip_check(gip_memcpy_d2h(ctx(), Bh->csrValA, tmp, sizeof(double) * (size_t)Bh->nnz), "values download");
-    node->release(tmp, bytes);
-    gip_check(gip_mat_set_values(Bh->dmat, Bh->csrValA), "values set");
+    checkShapes(x, Bh->nnz);
+    gip_check(gip_dcsrmv_vals_ab(ctx(), dmat, x->getDevicePtr(), Bh->dmat, alpha, beta), "spmv_vals_ab");
 }
 
 void GIP_MatrixHIP::downloadValues() {

@@ -8,7 +8,7 @@
 #include <string>
 
 #include "GIP_ArchHIP.h"
-#include "GIP_Matrix.h"
+#include "../../AbstractInterfaces/GIP_Matrix.h"
 #include "GIP_VectorHIP.h"
 #include "gip_cavity.h"
 

@@ -4,7 +4,7 @@
 #ifndef _gip_nlinearhip_
 #define _gip_nlinearhip_
 
-#include "GIP_NonLinear.h"
+#include "../../AbstractInterfaces/GIP_NonLinear.h"
 #include "GIP_VectorHIP.h"
 
 class GIP_NonLinearHIP : public GIP_NonLinear {

@@ -6,7 +6,7 @@
 #define _gip_timerhip_
 
 #include "GIP_ArchHIP.h"
-#include "GIP_Timer.h"
+#include "../../AbstractInterfaces/GIP_Timer.h"
 
 class GIP_TimerHIP : public GIP_Timer {
    public:

@@ -10,7 +10,7 @@
 #include <string>
 
 #include "GIP_ArchHIP.h"
-#include "GIP_Vector.h"
+#include "../../AbstractInterfaces/GIP_Vector.h"
 
 class GIP_VectorHIP : public GIP_Vector {
    public:

@@ -9,6 +9,7 @@
 #include "GIP_DNS.h"
 #include "HIP.h"
 #include "gip_dns.h"
+#include "gip_markers.h"
 
 typedef GIP_DNS<GIP_MatrixHIP, GIP_VectorHIP, GIP_ArchHIP, GIP_NonLinearHIP> HipDNS;
 
@@ -22,12 +23,14 @@ struct gip_dns {
             if (e) gip_event_destroy(e);
     }
     // events recorded on the node's stream at each phase boundary: no
-    // synchronisation inside the step; read after it
+    // synchronisation inside the step; read after it (each phase is also a
+    // profiler range)
     void attach_timer() {
         gip_ctx *c = sim.node().ctx();
         for (gip_event *&e : ev)
             if (gip_event_create(c, &e) != GIP_OK) throw std::runtime_error("event create failed");
         sim.setPhaseHook([this, c](int ph) {
+            gip_phase_ranges(ph);
             gip_event_record(c, ev[ph]);
             if (ph == 5) timed = true;
         });

@@ -127,11 +127,14 @@ int gip_mat_diag_inv(const gip_mat *A, double *dinv);
  * gip_dcsrmv_ab:   y_i = alpha*(Ax)_i + beta*y_i, non-empty rows   GIP_cudaDCsrSpMV 10-arg (:24), :85-103
  * gip_dcsrmv_vals: B's values (CSR order) <- Ax, non-empty rows   GIP_MatrixMulticore.cpp:105-123
  *                  (nrows(A) must equal nnz(B))
+ * gip_dcsrmv_vals_ab: B's values <- alpha*(Ax)_i + beta*B's values, non-empty rows
+ *                  (nrows(A) must equal nnz(B))                    GIP_MatrixMulticore.cpp:125-143
  * gip_dcsrmv_t:    y = A^T x over all columns < ncols_out, y zeroed first (adjoint operators)
  * gip_scsrmv:      FP32 y = A32 x32 (needs GIP_MAT_FP32)                                               */
 int gip_dcsrmv(gip_ctx *ctx, const gip_mat *A, const double *x, double *y);
 int gip_dcsrmv_ab(gip_ctx *ctx, const gip_mat *A, const double *x, double *y, double alpha, double beta);
 int gip_dcsrmv_vals(gip_ctx *ctx, const gip_mat *A, const double *x, gip_mat *B);
+int gip_dcsrmv_vals_ab(gip_ctx *ctx, const gip_mat *A, const double *x, gip_mat *B, double alpha, double beta);
 int gip_dcsrmv_t(gip_ctx *ctx, const gip_mat *A, const double *x, double *y, int32_t ncols_out);
 int gip_scsrmv(gip_ctx *ctx, const gip_mat *A, const float *x, float *y);
 

@@ -44,30 +44,11 @@ g++ "$OUT/o0/DrivenCavity.o" $O0 -o "$OUT/DrivenCavityCPU" $LINK
 mpic++ -include cstring -O2 -c "$HERE/ref_harness.cpp" -fopenmp -I"$REF" -I"$REPO/include" -o "$OUT/o2/ref_harness.o"
 g++ "$OUT/o2/ref_harness.o" $O2 -o "$OUT/gip_ref_harness" $LINK -L"$REPO/gip_amd" -lgip_cavity -Wl,-rpath,'$ORIGIN/../../gip_amd'
 # Drop-in check: the reference's unmodified templates and abstract/basic
-# interfaces instantiated with this repository's HIP specification, whose
-# sources are compiled here against the reference's abstract class headers.
-SPEC="$REPO/gip_amd/spec"
+# interfaces instantiated with this repository's HIP specification, built by
+# the recipe INTEGRATION.md documents (run verbatim in a temporary copy of
+# the reference tree).
 if [ -f "$REPO/gip_amd/libgip_hip.so" ]; then
-    mkdir -p "$OUT/dropin"
-    DI=""
-    for s in src/BasicInterfaces/GIP_Topo.cpp src/BasicInterfaces/GIP_Parameters.cpp src/AbstractInterfaces/GIP_Vector.cpp \
-             src/AbstractInterfaces/GIP_Matrix.cpp src/AbstractInterfaces/GIP_Timer.cpp src/AbstractInterfaces/GIP_Arch.cpp \
-             src/AbstractInterfaces/GIP_NonLinear.cpp; do
-        b=$(basename "$s" .cpp)
-        mpic++ -include cstring -O2 -c "$REF/$s" -o "$OUT/dropin/$b.o"
-        DI="$DI $OUT/dropin/$b.o"
-    done
-    INC="-I$REF/src/AbstractInterfaces -I$REF/src/BasicInterfaces -I$SPEC/Specifications/HIP -I$REPO/include -I$REF"
-    for s in GIP_ArchHIP GIP_VectorHIP GIP_MatrixHIP GIP_NonLinearHIP GIP_TimerHIP; do
-        mpic++ -include cstring -O2 -std=c++17 $INC -c "$SPEC/Specifications/HIP/$s.cpp" -o "$OUT/dropin/$s.o"
-        DI="$DI $OUT/dropin/$s.o"
-    done
-    mpic++ -include cstring -O2 -std=c++17 $INC -c "$HERE/ref_dropin.cpp" -o "$OUT/dropin/ref_dropin.o"
-    # the HIP runtime needs the system libstdc++: libmpi is linked by path so
-    # that conda's older runtime stays out of the search path
-    g++ "$OUT/dropin/ref_dropin.o" $DI -o "$OUT/DrivenCavityHIP_reftemplates" -fopenmp /opt/conda/lib/libmpi.so.12 \
-        -L"$REPO/gip_amd" -lgip_hip -lgip_cavity -Wl,-rpath,/usr/lib/x86_64-linux-gnu:/opt/conda/lib:'$ORIGIN/../../gip_amd'
-    echo "build_ref: built $OUT/DrivenCavityHIP_reftemplates (reference templates + HIP specification)"
+    python3 "$HERE/dropin_build.py" "$REF" "$OUT/DrivenCavityHIP_reftemplates"
 fi
-rm -rf "$OUT/o0" "$OUT/o2" "$OUT/dropin"
+rm -rf "$OUT/o0" "$OUT/o2"
 echo "build_ref: built $OUT/DrivenCavityCPU $OUT/gip_ref_harness"

@@ -5,9 +5,42 @@
 #include "gip_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* Row loops (products, axpy) run on OpenMP threads above this many rows:
+ * each row's value is computed by one thread with the serial expression, so
+ * results do not depend on the thread count.  Reductions (dot, norm, CFL)
+ * stay serial, in index order, as in the reference. */
+#define ORC_PAR_MIN 65536
+
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
+/* Blocked dot products (off by default): partial sums of ORC_DOT_BLOCK
+ * consecutive terms, each in index order, computed in parallel and added in
+ * block order.  The result does not depend on the thread count; it differs
+ * from the serial sum by rounding only (a different association). */
+#define ORC_DOT_BLOCK 4096
+static int g_blocked_dot = 0;
+void orc_set_blocked_dot(int on) { g_blocked_dot = on != 0; }
+
+static double dot_blocked(const double *y, const double *x, int n) {
+    const int nb = (n + ORC_DOT_BLOCK - 1) / ORC_DOT_BLOCK;
+    double *part = (double *)malloc(sizeof(double) * (size_t)(nb > 0 ? nb : 1));
+#pragma omp parallel for schedule(static)
+    for (int b = 0; b < nb; b++) {
+        const int i1 = (b + 1) * ORC_DOT_BLOCK < n ? (b + 1) * ORC_DOT_BLOCK : n;
+        double t = 0.0;
+        for (int i = b * ORC_DOT_BLOCK; i < i1; i++) t += y[i] * x[i];
+        part[b] = t;
+    }
+    double t = 0.0;
+    for (int b = 0; b < nb; b++) t += part[b];
+    free(part);
+    return t;
+}
 
 /* ---- files --------------------------------------------------------------- */
 
@@ -68,6 +101,7 @@ static int read_topo(const char *path, int sizes[9]) {
 
 /* GIP_MatrixMulticore.cpp:64-83 */
 void orc_spmv(const orc_csr *A, const double *x, double *y) {
+#pragma omp parallel for schedule(static) if (A->nrows > ORC_PAR_MIN)
     for (int i = 0; i < A->nrows; i++) {
         double sum = 0;
         int jb = A->rowptr[i], je = A->rowptr[i + 1];
@@ -78,6 +112,7 @@ void orc_spmv(const orc_csr *A, const double *x, double *y) {
 
 /* GIP_MatrixMulticore.cpp:85-103 */
 void orc_spmv_ab(const orc_csr *A, const double *x, double *y, double alpha, double beta) {
+#pragma omp parallel for schedule(static) if (A->nrows > ORC_PAR_MIN)
     for (int i = 0; i < A->nrows; i++) {
         double sum = 0;
         int jb = A->rowptr[i], je = A->rowptr[i + 1];
@@ -88,6 +123,11 @@ void orc_spmv_ab(const orc_csr *A, const double *x, double *y, double alpha, dou
 
 /* GIP_MatrixMulticore.cpp:105-123 */
 void orc_spmv_vals(const orc_csr *A, const double *x, double *bval) { orc_spmv(A, x, bval); }
+
+/* GIP_MatrixMulticore.cpp:125-143: B's values = alpha*(Ax) + beta*B's values, non-empty rows */
+void orc_spmv_vals_ab(const orc_csr *A, const double *x, double *bval, double alpha, double beta) {
+    orc_spmv_ab(A, x, bval, alpha, beta);
+}
 
 /* GIP_MatrixMulticore.cpp:177-210 */
 int orc_diag_setup(const orc_csr *A, orc_csr *D) {
@@ -114,14 +154,17 @@ int orc_diag_setup(const orc_csr *A, orc_csr *D) {
 /* ---- Multicore vector ---------------------------------------------------- */
 
 void orc_axpy(double *y, const double *x, double alpha, int n) {
+#pragma omp parallel for schedule(static) if (n > ORC_PAR_MIN)
     for (int i = 0; i < n; i++) y[i] += alpha * x[i];
 }
 
 void orc_axpby(double *y, const double *x, double alpha, double beta, int n) {
+#pragma omp parallel for schedule(static) if (n > ORC_PAR_MIN)
     for (int i = 0; i < n; i++) y[i] = alpha * x[i] + beta * y[i];
 }
 
 double orc_dot(const double *y, const double *x, int n) {
+    if (g_blocked_dot && n > ORC_PAR_MIN) return dot_blocked(y, x, n);
     double t = 0.0;
     for (int i = 0; i < n; i++) t += y[i] * x[i];
     return t;
@@ -257,11 +300,65 @@ int orc_dns_open(const char *root, const char *name, orc_dns **out) {
     return 0;
 }
 
+/* The operator set in the order of orc_dns_open_mem's ops argument. */
+static orc_csr *const *mem_ops(orc_dns *s, orc_csr *slots[15]) {
+    orc_csr *t[15] = {&s->D,  &s->Ec, &s->L,   &s->Lb,  &s->Mx,  &s->My, &s->Mz, &s->Gx,
+                      &s->Gy, &s->Gz, &s->SFx, &s->SFy, &s->SFz, &s->SO, &s->Mres};
+    memcpy(slots, t, sizeof t);
+    return slots;
+}
+
+/* GIP_DNS.h:115-306 with the operators and fields taken from memory: the
+ * operators are borrowed (they must outlive the state), C gets its own
+ * value array over D's structure (its values are rewritten every step,
+ * GIP_DNS.h:312), the fields are copied. */
+int orc_dns_open_mem(int inner, int all, int faces, const orc_csr *ops, const double *u, const double *v,
+                     const double *w, const double *dxs, orc_dns **out) {
+    *out = NULL;
+    if (inner <= 0 || all < inner || faces < 0 || !ops || !u || !v || !w || !dxs) return -1;
+    orc_dns *s = (orc_dns *)calloc(1, sizeof(orc_dns));
+    s->inner = inner;
+    s->all = all;
+    s->faces = faces;
+    s->borrowed = 1;
+    orc_csr *slots[15];
+    mem_ops(s, slots);
+    for (int i = 0; i < 15; ++i) *slots[i] = ops[i];
+    s->C = s->D;
+    s->C.val = (double *)malloc(sizeof(double) * (size_t)(s->D.nnz > 0 ? s->D.nnz : 1));
+    memcpy(s->C.val, s->D.val, sizeof(double) * (size_t)s->D.nnz);
+    double **cellv[] = {&s->ax, &s->ay, &s->az, &s->ax0, &s->ay0, &s->az0, &s->rhs, &s->p, &s->p0,
+                        &s->gpx, &s->gpy, &s->gpz, &s->temp};
+    for (size_t i = 0; i < sizeof cellv / sizeof cellv[0]; ++i) *cellv[i] = (double *)calloc((size_t)s->all, sizeof(double));
+    s->mf = (double *)calloc((size_t)s->faces > 0 ? (size_t)s->faces : 1, sizeof(double));
+    const double *src[] = {u, v, w, dxs};
+    double **fv[] = {&s->u, &s->v, &s->w, &s->dxs};
+    for (int f = 0; f < 4; ++f) {
+        *fv[f] = (double *)malloc(sizeof(double) * (size_t)s->all);
+        memcpy(*fv[f], src[f], sizeof(double) * (size_t)s->all);
+    }
+    orc_diag_setup(&s->L, &s->Dinv);
+    s->rho = 1.0;
+    s->gamma = 1.0 / 1.0e3;
+    s->dt = 1e-3;
+    s->time = 0.0;
+    s->pcg_maxit = 100;
+    s->pcg_tol = 0.0;
+    s->precond = 1;
+    *out = s;
+    return 0;
+}
+
 void orc_dns_close(orc_dns *s) {
     if (!s) return;
-    orc_csr *ops[] = {&s->C, &s->Ec, &s->D, &s->L, &s->Lb, &s->Mx, &s->My, &s->Mz, &s->Gx,
-                      &s->Gy, &s->Gz, &s->SFx, &s->SFy, &s->SFz, &s->SO, &s->Mres, &s->Dinv};
-    for (size_t i = 0; i < sizeof ops / sizeof ops[0]; ++i) orc_csr_free(ops[i]);
+    if (s->borrowed) {
+        free(s->C.val);
+        orc_csr_free(&s->Dinv);
+    } else {
+        orc_csr *ops[] = {&s->C, &s->Ec, &s->D, &s->L, &s->Lb, &s->Mx, &s->My, &s->Mz, &s->Gx,
+                          &s->Gy, &s->Gz, &s->SFx, &s->SFy, &s->SFz, &s->SO, &s->Mres, &s->Dinv};
+        for (size_t i = 0; i < sizeof ops / sizeof ops[0]; ++i) orc_csr_free(ops[i]);
+    }
     double *vs[] = {s->u, s->v, s->w, s->ax, s->ay, s->az, s->ax0, s->ay0, s->az0, s->rhs, s->p,
                     s->p0, s->gpx, s->gpy, s->gpz, s->mf, s->dxs, s->temp};
     for (size_t i = 0; i < sizeof vs / sizeof vs[0]; ++i) free(vs[i]);

@@ -46,9 +46,21 @@ void orc_spmv(const orc_csr *A, const double *x, double *y);
 void orc_spmv_ab(const orc_csr *A, const double *x, double *y, double alpha, double beta);
 /* GIP_MatrixMulticore.cpp:105-123: result written into another matrix's value array. */
 void orc_spmv_vals(const orc_csr *A, const double *x, double *bval);
+/* GIP_MatrixMulticore.cpp:125-143: B's values = alpha*(Ax)_i + beta*B's values, non-empty rows. */
+void orc_spmv_vals_ab(const orc_csr *A, const double *x, double *bval, double alpha, double beta);
 /* GIP_MatrixMulticore.cpp:177-210: Dinv = diag(1/a_ii) as an nrows-nnz CSR.
  * Returns -1 when a row has no diagonal entry (the reference leaves it uninitialised). */
 int orc_diag_setup(const orc_csr *A, orc_csr *Dinv);
+
+/* Threads for the row loops (products, axpy, axpby) above 65,536 rows.
+ * Each row is computed by one thread with the serial expression and the
+ * reductions stay serial, so results are bitwise the same at any count. */
+void orc_set_threads(int n);
+/* Blocked dot products (default off): sums of 4096-term blocks, each in
+ * index order, added in block order; independent of the thread count,
+ * equal to the serial sum up to rounding.  Off, orc_dot is the reference's
+ * serial sum (GIP_VectorMulticore.cpp:93-128). */
+void orc_set_blocked_dot(int on);
 
 /* GIP_VectorMulticore.cpp:44-67 */
 void orc_axpy(double *y, const double *x, double alpha, int n);
@@ -80,10 +92,16 @@ typedef struct orc_dns {
     int last_iters;
     double last_relres;
     int started;
+    int borrowed; /* operators owned by the caller (orc_dns_open_mem) */
 } orc_dns;
 
 /* Reads <root>/InputFiles/.../<name>_1p-0.* (GIP_DNS.h:115-306, one partition). */
 int orc_dns_open(const char *root, const char *name, orc_dns **out);
+/* The same state from operators and fields in memory (a generated case):
+ * ops holds D, Ec, L, Lb, Mx, My, Mz, Gx, Gy, Gz, SFx, SFy, SFz, SO, Mres,
+ * borrowed (they must outlive the state); u, v, w, dxs (all values) are copied. */
+int orc_dns_open_mem(int inner, int all, int faces, const orc_csr *ops, const double *u, const double *v,
+                     const double *w, const double *dxs, orc_dns **out);
 void orc_dns_close(orc_dns *s);
 /* Parameters the reference hardcodes (GIP_DNS.h:132-137, :180-185). */
 void orc_dns_params(orc_dns *s, double gamma, double rho, int pcg_maxit, double pcg_tol, int precond);

@@ -39,6 +39,8 @@ def lib():
         L = ctypes.CDLL(str(path))
         L.orc_spmv.argtypes = [P(OrcCsr), d_p, d_p]
         L.orc_spmv_ab.argtypes = [P(OrcCsr), d_p, d_p, ctypes.c_double, ctypes.c_double]
+        L.orc_spmv_vals_ab.argtypes = [P(OrcCsr), d_p, d_p, ctypes.c_double, ctypes.c_double]
+        L.orc_spmv_vals_ab.restype = None
         L.orc_diag_setup.argtypes = [P(OrcCsr), P(OrcCsr)]
         L.orc_csr_free.argtypes = [P(OrcCsr)]
         L.orc_csr_free.restype = None
@@ -51,6 +53,12 @@ def lib():
         L.orc_pcg.argtypes = [P(OrcCsr), P(OrcCsr), ctypes.c_int, ctypes.c_int, d_p, d_p, d_p, ctypes.c_int,
                               ctypes.c_double, ctypes.c_int, i_p, d_p]
         L.orc_dns_open.argtypes = [ctypes.c_char_p, ctypes.c_char_p, P(ctypes.c_void_p)]
+        L.orc_dns_open_mem.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(OrcCsr), d_p, d_p, d_p, d_p,
+                                       P(ctypes.c_void_p)]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_threads.restype = None
+        L.orc_set_blocked_dot.argtypes = [ctypes.c_int]
+        L.orc_set_blocked_dot.restype = None
         L.orc_dns_close.argtypes = [ctypes.c_void_p]
         L.orc_dns_close.restype = None
         L.orc_dns_params.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int,
@@ -91,6 +99,11 @@ def spmv_ab(A: Csr, x, y, alpha, beta):
     lib().orc_spmv_ab(ctypes.byref(A.s), _dp(x), _dp(y), alpha, beta)
 
 
+def spmv_vals_ab(A: Csr, x, bval, alpha, beta):
+    """GIP_MatrixMulticore.cpp:125-143 (bval, another operator's values, modified in place)."""
+    lib().orc_spmv_vals_ab(ctypes.byref(A.s), _dp(x), _dp(bval), alpha, beta)
+
+
 def dot(a, b):
     return lib().orc_dot(_dp(a), _dp(b), a.size)
 
@@ -116,14 +129,40 @@ def pcg(A: Csr, b, x0, n_inner, maxit, tol=0.0, precond=True):
     return x, st, it.value, rr.value
 
 
+def set_threads(n):
+    """Threads of the oracle's row loops (products, axpy); results are
+    bitwise the same at any count (reductions stay serial)."""
+    lib().orc_set_threads(int(n))
+
+
+def set_blocked_dot(on):
+    """Dot products as sums of 4096-term blocks added in block order (not
+    the reference's serial order: equal up to rounding); off by default."""
+    lib().orc_set_blocked_dot(1 if on else 0)
+
+
+MEM_OPS = ("D", "Ec", "L", "Lb", "Mx", "My", "Mz", "Gx", "Gy", "Gz", "SFx", "SFy", "SFz", "SO", "Mres")
+
+
 class Dns:
-    """GIP_DNS on one partition, read from <root>/InputFiles (the C restatement)."""
+    """GIP_DNS on one partition (the C restatement), read from
+    <root>/InputFiles, or from a generated case in memory (`cavity`: an
+    object with n_inner, n_all, n_faces, op(name) -> CSR and field(name);
+    its operators are borrowed and must outlive this object)."""
 
     FIELDS = ("u", "v", "w", "p", "mf")
 
-    def __init__(self, root, name, gamma=1e-3, rho=1.0, pcg_maxit=100, pcg_tol=0.0):
+    def __init__(self, root=None, name=None, gamma=1e-3, rho=1.0, pcg_maxit=100, pcg_tol=0.0, cavity=None):
         self.h = ctypes.c_void_p()
-        if lib().orc_dns_open(str(root).encode(), name.encode(), ctypes.byref(self.h)) != 0:
+        if cavity is not None:
+            self._keep = [cavity] + [Csr.of(cavity.op(o)) for o in MEM_OPS]
+            ops = (OrcCsr * len(MEM_OPS))(*[c.s for c in self._keep[1:]])
+            f = {k: np.ascontiguousarray(cavity.field(k), dtype=np.float64) for k in ("u", "v", "w", "dxs")}
+            rc = lib().orc_dns_open_mem(cavity.n_inner, cavity.n_all, cavity.n_faces, ops, _dp(f["u"]), _dp(f["v"]),
+                                        _dp(f["w"]), _dp(f["dxs"]), ctypes.byref(self.h))
+            if rc != 0:
+                raise RuntimeError("oracle cannot take the case from memory")
+        elif lib().orc_dns_open(str(root).encode(), name.encode(), ctypes.byref(self.h)) != 0:
             raise RuntimeError(f"oracle cannot open case {name} under {root}")
         lib().orc_dns_params(self.h, gamma, rho, pcg_maxit, pcg_tol, 1)
 

@@ -1,0 +1,271 @@
+"""The workload configurations at their own sizes on the device, against the
+oracle on identical operators (the 46^3 case, Re=100, tolerance 1e-8, is
+covered in test_gpu_dns.py at its own size).
+
+  * 128^3 on the tanh-clustered mesh (beta = 1.5), Re = 1000: the Laplacian
+    product bitwise, 50 fixed FP64 PCG iterations and three Navier-Stokes
+    steps with the reference's fixed 100 iterations, all to 1e-10; one step
+    with the pressure solved to 1e-10 (FP64: iteration count within one;
+    mixed precision: error per field against the FP64 oracle).
+  * 256^3 uniform: FP64 and mixed-precision PCG to a 1e-10 FP64 relative
+    true residual (checked with scipy's FP64 product) on a seeded zero-mean
+    N(0,1) right-hand side; two Navier-Stokes steps at Re = 1000 with the
+    fixed 100 iterations against the oracle to 1e-10.
+  * 64 cavities with edges drawn from {32..96}, Re log-uniform in
+    [100, 3200], mixed-precision PCG to 1e-8: every instance of the batch
+    against the single-cavity path, four of them against the FP64 oracle.
+  * 64^3, Re = 400, 200 steps with a checkpoint every 20: adjoint gradients
+    against central differences (eps = 1e-6) along 16 directions, 8 in the
+    lid profile only and 8 in the initial velocity only.
+
+The oracle runs its row loops on the host's cores (bitwise the serial
+results; its reductions stay serial).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from gip_amd import adjoint, batch, cavity, dns, hip
+from oracle import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-10
+
+
+@pytest.fixture(scope="module", autouse=True)
+def oracle_threads():
+    pyoracle.set_threads(min(16, os.cpu_count() or 1))
+    yield
+    pyoracle.set_threads(1)
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = hip.Device(0)
+    yield d
+    d.close()
+
+
+def _rel(a, b):
+    return abs(a - b) / abs(b) if b else abs(a)
+
+
+def _field_rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / nb if nb else np.linalg.norm(a)
+
+
+def _rhs(cav, seed):
+    b = np.zeros(cav.n_all)
+    b[:cav.n_inner] = cavity.rhs_normal(cav.n_inner, seed)
+    return b
+
+
+def _true_rel(L, x, b, n, nall):
+    res = b[:n] - (L.to_scipy(nall) @ x)[:n]
+    return np.linalg.norm(res) / np.linalg.norm(b[:n])
+
+
+def _steps_vs_oracle(sim, orc, steps, tag):
+    for k in range(steps):
+        g, w = sim.step(), orc.step()
+        print(f"[{tag} step {k + 1}] gpu time {g['time']:.12e} dt {g['dt']:.12e} massr {g['massr']:.9e} "
+              f"pcg {g['pcg_iters']} | oracle massr {w['massr']:.9e} pcg {w['pcg_iters']}", flush=True)
+        assert g["pcg_iters"] == w["pcg_iters"]
+        for key in ("time", "dt", "massr"):
+            assert _rel(g[key], w[key]) <= REL, (k, key, g[key], w[key])
+    for f in ("u", "v", "w", "p", "mf"):
+        err = _field_rel(sim.field(f), orc.field(f))
+        print(f"[{tag} field {f}] rel L2 {err:.3e}", flush=True)
+        assert err <= REL, (f, err)
+
+
+# ---- 128^3, tanh-clustered mesh, Re = 1000 ------------------------------------
+
+@pytest.fixture(scope="module")
+def clu128():
+    return cavity.generate(128, beta=1.5, gamma=1e-3)
+
+
+def test_clustered128_laplacian_product_bitwise(dev, clu128):
+    L = clu128.op("L")
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(clu128.n_all)
+    y0 = rng.standard_normal(clu128.n_all)
+    A = hip.Matrix.from_csr(dev, L)
+    print(f"[128^3 clustered L] layout {A.info()}", flush=True)
+    dx, dy = dev.to_device(x), dev.to_device(y0)
+    A.spmv(dx, dy)
+    want = y0.copy()
+    pyoracle.spmv(pyoracle.Csr.of(L), x, want)
+    assert np.array_equal(dy.download(), want)
+    dy.upload(y0)
+    A.spmv_ab(dx, dy, -0.5, 2.0)
+    want = y0.copy()
+    pyoracle.spmv_ab(pyoracle.Csr.of(L), x, want, -0.5, 2.0)
+    assert np.array_equal(dy.download(), want)
+
+
+def test_clustered128_pcg_fixed_iterations_and_mixed(dev, clu128):
+    L = clu128.op("L")
+    n, nall = clu128.n_inner, clu128.n_all
+    b = _rhs(clu128, 5)
+    x0 = np.zeros(nall)
+    A = hip.Matrix.from_csr(dev, L, fp32=True)
+    solver = hip.PCG(A, nall)
+    dx = dev.array(nall)
+    r = solver.solve(dev.to_device(b), dx, dev.to_device(x0), maxit=50)
+    want, st, it, rel = pyoracle.pcg(pyoracle.Csr.of(L), b, x0, n, 50)
+    err = _field_rel(dx.download(), want)
+    print(f"[128^3 clustered fp64 50 it] relres {r.relres:.6e} oracle {rel:.6e} iterate rel {err:.3e}", flush=True)
+    assert r.iters == it == 50 and r.status == 0 and err <= REL
+    r = solver.solve(dev.to_device(b), dx, dev.to_device(x0), maxit=20000, tol=1e-10, precision=hip.PCG_MIXED,
+                     poll=32)
+    tr = _true_rel(L, dx.download(), b, n, nall)
+    print(f"[128^3 clustered mixed] iters {r.iters} corrections {r.corrections} true {tr:.3e}", flush=True)
+    assert r.status == 0 and tr <= 1.01e-10
+
+
+def test_clustered128_ns_fixed_iterations_vs_oracle(dev, clu128):
+    sim = dns.Simulation(cavity=clu128, gamma=1e-3, pcg_mode="fused")
+    orc = pyoracle.Dns(cavity=clu128, gamma=1e-3)
+    _steps_vs_oracle(sim, orc, 3, "128^3 clustered")
+    sim.close()
+    orc.close()
+
+
+def test_clustered128_ns_tolerance_step_fp64_and_mixed_vs_oracle(dev, clu128):
+    """One step with the pressure solved to 1e-10, as in the configuration:
+    the FP64 device solve stops within one iteration of the oracle's; the
+    mixed-precision run's error per field against the FP64 oracle is at the
+    solver tolerance, not at FP32 round-off."""
+    orc = pyoracle.Dns(cavity=clu128, gamma=1e-3, pcg_maxit=20000, pcg_tol=1e-10)
+    w = orc.step()
+    want = {f: orc.field(f) for f in ("u", "v", "w", "p")}
+    orc.close()
+    for mode in ("fused", "mixed"):
+        sim = dns.Simulation(cavity=clu128, gamma=1e-3, pcg_mode=mode, pcg_tol=1e-10, pcg_maxit=20000)
+        g = sim.step()
+        print(f"[128^3 clustered tol 1e-10 {mode}] pcg {g['pcg_iters']} (oracle {w['pcg_iters']}) "
+              f"massr {g['massr']:.3e} (oracle {w['massr']:.3e})", flush=True)
+        assert g["pcg_status"] == 0
+        for key in ("time", "dt"):  # the first dt comes from the initial field
+            assert _rel(g[key], w[key]) <= REL
+        if mode == "fused":
+            assert abs(g["pcg_iters"] - w["pcg_iters"]) <= 1
+        for f in ("u", "v", "w", "p"):
+            err = _field_rel(sim.field(f), want[f])
+            print(f"[128^3 clustered tol 1e-10 {mode} field {f}] rel L2 vs FP64 oracle {err:.3e}", flush=True)
+            assert err <= 1e-7, (mode, f, err)
+        sim.close()
+
+
+# ---- 256^3 uniform, Re = 1000 ---------------------------------------------------
+
+@pytest.fixture(scope="module")
+def cav256():
+    return cavity.generate(256, gamma=1e-3)
+
+
+@pytest.mark.timeout(600)
+def test_256_pcg_fp64_and_mixed_to_1e10_true_residual(dev, cav256):
+    L = cav256.op("L")
+    n, nall = cav256.n_inner, cav256.n_all
+    b = _rhs(cav256, 2024)
+    A = hip.Matrix.from_csr(dev, L, fp32=True)
+    assert A.info()["format"] == hip.FMT_STENCIL
+    solver = hip.PCG(A, nall)
+    dx = dev.array(nall)
+    z = dev.to_device(np.zeros(nall))
+    db = dev.to_device(b)
+    r64 = solver.solve(db, dx, z, maxit=20000, tol=1e-10, poll=64)
+    tr64 = _true_rel(L, dx.download(), b, n, nall)
+    r32 = solver.solve(db, dx, z, maxit=20000, tol=1e-10, precision=hip.PCG_MIXED, poll=32)
+    x = dx.download()
+    tr32 = _true_rel(L, x, b, n, nall)
+    print(f"[256^3 pcg] fp64 iters {r64.iters} recursive {r64.relres:.3e} true {tr64:.3e} | mixed iters "
+          f"{r32.iters} corrections {r32.corrections} true {tr32:.3e}", flush=True)
+    assert r64.status == 0 and r64.relres <= 1e-10
+    # the FP64 recursion stops on its own residual, which tracks the true one closely
+    assert tr64 <= 1.05e-10
+    # the mixed solver decides convergence on the FP64 true residual
+    assert r32.status == 0 and tr32 <= 1.01e-10
+    assert np.array_equal(x[n:], np.zeros(nall - n))
+
+
+@pytest.mark.timeout(900)
+def test_256_ns_fixed_iterations_vs_oracle(dev, cav256):
+    sim = dns.Simulation(cavity=cav256, gamma=1e-3, pcg_mode="fused")
+    orc = pyoracle.Dns(cavity=cav256, gamma=1e-3)
+    _steps_vs_oracle(sim, orc, 2, "256^3")
+    sim.close()
+    orc.close()
+
+
+# ---- 64 uneven cavities, batched ----------------------------------------------
+
+def _batch_case(seed=0, count=64):
+    rng = np.random.default_rng(seed)
+    sizes = [int(v) for v in rng.integers(32, 97, count)]
+    re = np.exp(rng.uniform(np.log(100.0), np.log(3200.0), count))
+    return sizes, re
+
+
+@pytest.mark.timeout(900)
+def test_batch64_uneven_vs_single_cavity_and_oracle(dev):
+    sizes, re = _batch_case()
+    cavs = [cavity.generate(n, gamma=1.0 / r) for n, r in zip(sizes, re)]
+    steps = 3
+    bt = batch.Batch(dev, cavs, 1.0 / re, pcg_tol=1e-8, pcg_maxit=20000, precision=hip.PCG_MIXED, poll=16,
+                     use_graph=True, autotune=True)
+    for _ in range(steps):
+        bt.step()
+    got = [{f: bt.field(i, f) for f in ("u", "v", "w", "p")} for i in range(len(cavs))]
+    info = [bt.info(i) for i in range(len(cavs))]
+    bt.close()
+    worst = 0.0
+    for i, (c, r) in enumerate(zip(cavs, re)):
+        assert info[i]["pcg_status"] == 0
+        sim = dns.Simulation(cavity=c, gamma=1.0 / r, pcg_mode="mixed", pcg_tol=1e-8, pcg_maxit=20000)
+        for _ in range(steps):
+            s = sim.step()
+        assert _rel(info[i]["time"], s["time"]) <= 1e-8, (i, info[i]["time"], s["time"])
+        for f in ("u", "v", "w", "p"):
+            err = _field_rel(got[i][f], sim.field(f))
+            worst = max(worst, err)
+            assert err <= 1e-6, (i, sizes[i], f, err)
+        sim.close()
+    print(f"[batch64] sizes {min(sizes)}..{max(sizes)}, {sum(n ** 3 for n in sizes)} cells; worst field "
+          f"difference against the single-cavity path {worst:.2e}", flush=True)
+    # against the FP64 oracle: the smallest, the largest and two more instances
+    order = np.argsort(sizes)
+    for i in sorted({int(order[0]), int(order[-1]), 1, 2}):
+        orc = pyoracle.Dns(cavity=cavs[i], gamma=1.0 / re[i], pcg_maxit=20000, pcg_tol=1e-8)
+        for _ in range(steps):
+            w = orc.step()
+        assert _rel(info[i]["time"], w["time"]) <= 1e-8
+        for f in ("u", "v", "w", "p"):
+            err = _field_rel(got[i][f], orc.field(f))
+            print(f"[batch64 inst {i} n={sizes[i]} Re={re[i]:.0f} {f}] rel L2 vs FP64 oracle {err:.3e}", flush=True)
+            assert err <= 1e-6, (i, f, err)
+        orc.close()
+
+
+# ---- adjoint, 64^3, Re = 400, 200 steps ----------------------------------------
+
+@pytest.mark.timeout(900)
+def test_adjoint64_200_steps_lid_and_velocity_directions(dev):
+    n, steps, every = 64, 200, 20
+    rng = np.random.default_rng(3)
+    lid = 1.0 + 0.05 * rng.standard_normal(n * n)
+    cav = cavity.generate(n, gamma=1.0 / 400.0, lid_profile=lid)
+    a = adjoint.Adjoint(dev, cav, steps=steps, checkpoint_every=every, gamma=1.0 / 400.0, pcg_tol=1e-13)
+    z = np.zeros(cav.n_inner)
+    a.forward(lid, z, z, z)  # freezes dt
+    for space, seed in (("lid", 1), ("velocity", 2)):
+        J, ad, fd, rel = adjoint.fd_check(a, lid, z, z, z, ndir=8, eps=1e-6, seed=seed, space=space)
+        print(f"[adjoint 64^3 {space}] J {J:.9e} max rel {rel.max():.3e} median {np.median(rel):.3e}", flush=True)
+        assert np.all(rel <= 1e-6), (space, rel)
+    a.close()

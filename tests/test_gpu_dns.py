@@ -222,6 +222,33 @@ def test_drivencavityhip_executable(cav46_dir):
             assert _rel(g[key], w[key]) <= REL
 
 
+def test_timer_specification_times_each_step(cav46_dir):
+    """DrivenCavityHIP --timing wraps every step in a channel of the HIP
+    timer specification (GIP_Timer.h:6-26; events on the stream as in
+    GIP_TimerCuda.cpp:3-32): the printed last, summed and mean times are
+    positive, consistent with each other, bounded by the process's wall
+    time, and the steps themselves still match the oracle."""
+    t0 = time.perf_counter()
+    out = subprocess.run([str(ROOT / "gip_amd/bin/DrivenCavityHIP"), "cav46", "--root", str(cav46_dir), "--steps",
+                          "5", "--gamma", "0.01", "--full", "--timing"], capture_output=True, text=True, check=True,
+                         timeout=120)
+    wall = time.perf_counter() - t0
+    lines = [ln.split() for ln in out.stdout.splitlines() if ln.startswith("timer")]
+    assert len(lines) == 1, out.stdout
+    t = lines[0]
+    steps, last, total, avg = int(t[2]), float(t[4]), float(t[6]), float(t[8])
+    print(f"[timer] steps {steps} last {last:.3e} s sum {total:.3e} s avg {avg:.3e} s wall {wall:.2f} s", flush=True)
+    assert steps == 5
+    assert 0.0 < last <= total < wall
+    assert abs(avg - total / steps) <= 1e-6 * avg
+    got = pyoracle.parse_lines(out.stdout)
+    want = pyoracle.run_cli(cav46_dir, "cav46", 5, 1e-2)
+    assert len(got) == 5
+    for g, w in zip(got, want):
+        for key in ("time", "dt", "massr"):
+            assert _rel(g[key], w[key]) <= REL
+
+
 def test_dns_argument_errors(tmp_path, cav46_dir):
     """The integrator's C ABI reports a missing case, an unknown field and a
     field of the wrong length as errors instead of failing later."""

@@ -214,6 +214,25 @@ def test_spmv_into_values_rebuilds_convection(dev, cav46):
     assert np.array_equal(C.values(), want)
 
 
+@pytest.mark.parametrize("fmt", [None, "ELL", "CSR"])
+def test_spmv_into_values_alpha_beta(dev, cav46, fmt):
+    """alpha Ec.mf + beta C's values, written into C's values on the device
+    (GIP_MatrixMulticore.cpp:125-143): bitwise the oracle's, in every layout
+    of C; Ec's empty rows leave their value."""
+    Ec, D = cav46.op("Ec"), cav46.op("D")
+    assert (np.diff(Ec.rowptr) == 0).any()
+    mf = np.random.default_rng(4).standard_normal(cav46.n_faces)
+    E = hip.Matrix.from_csr(dev, Ec)
+    kw = {} if fmt is None else {"fmt": getattr(hip, "FMT_" + fmt)}
+    C = hip.Matrix.from_csr(dev, D, **kw)
+    E.spmv_vals_ab(dev.to_device(mf), C, 0.75, -1.5)
+    want = D.val.copy()
+    pyoracle.spmv_vals_ab(pyoracle.Csr.of(Ec), mf, want, 0.75, -1.5)
+    assert np.array_equal(C.values(), want)
+    with pytest.raises(ValueError):
+        C.spmv_vals_ab(dev.to_device(mf), C, 1.0, 0.0)  # rows(C) != nnz(C)
+
+
 def test_transposed_spmv(dev, cav46):
     for name in ("Mx", "SO", "Ec"):
         A = cav46.op(name)

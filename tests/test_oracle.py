@@ -94,6 +94,12 @@ def test_oracle_spmv_matches_scipy_and_keeps_empty_rows():
     pyoracle.spmv_ab(C, x, y2, -0.5, 3.0)
     np.testing.assert_allclose(y2[nonempty], -0.5 * ref[nonempty] + 6.0, rtol=1e-13, atol=1e-13)
     assert np.all(y2[~nonempty] == 2.0)
+    # the fourth variant writes into another operator's value array
+    bv = np.linspace(-1.0, 1.0, 300)
+    bw = bv.copy()
+    pyoracle.spmv_vals_ab(C, x, bw, 1.5, -0.25)
+    np.testing.assert_allclose(bw[nonempty], 1.5 * ref[nonempty] - 0.25 * bv[nonempty], rtol=1e-13, atol=1e-13)
+    assert np.array_equal(bw[~nonempty], bv[~nonempty])
 
 
 def test_oracle_pcg_solves_spd_system_and_keeps_residual_invariant():
@@ -162,3 +168,42 @@ def test_reference_binary_live_if_present(c100k_root):
     ref = pyoracle.parse_lines(out)
     got = pyoracle.run_cli(c100k_root, "C100K", 3, 1e-3)
     assert [(r["time"], r["dt"], r["massr"]) for r in ref] == [(g["time"], g["dt"], g["massr"]) for g in got]
+
+
+def test_oracle_threads_and_memory_case_are_bitwise_the_serial_file_case(tmp_path):
+    """The oracle's threaded row loops (products, axpy) and its in-memory
+    case give bitwise the serial, file-read results: each row is computed
+    by one thread with the serial expression, reductions stay serial."""
+    from gip_amd import cavity
+    cav = cavity.generate(48, beta=1.5, gamma=1e-3)  # 110,592 rows: above the threading threshold
+    cav.write(tmp_path, "t48")
+    runs = []
+    for threads, mem in ((1, False), (4, False), (4, True)):
+        pyoracle.set_threads(threads)
+        d = pyoracle.Dns(cavity=cav, gamma=1e-3) if mem else pyoracle.Dns(tmp_path, "t48", gamma=1e-3)
+        rows = [d.step() for _ in range(2)]
+        runs.append((rows, [d.field(f) for f in ("u", "v", "w", "p", "mf")]))
+        d.close()
+    pyoracle.set_threads(1)
+    for rows, fields in runs[1:]:
+        assert rows == runs[0][0]
+        for a, b in zip(fields, runs[0][1]):
+            assert np.array_equal(a, b)
+
+
+def test_oracle_blocked_dot_is_the_serial_sum_up_to_rounding():
+    rng = np.random.default_rng(5)
+    x, y = rng.standard_normal(300_001), rng.standard_normal(300_001)
+    serial = pyoracle.dot(x, y)
+    pyoracle.set_blocked_dot(True)
+    try:
+        blocked = pyoracle.dot(x, y)
+        pyoracle.set_threads(3)
+        again = pyoracle.dot(x, y)
+    finally:
+        pyoracle.set_blocked_dot(False)
+        pyoracle.set_threads(1)
+    exact = float(np.sum(x * y, dtype=np.longdouble))
+    assert blocked == again  # independent of the thread count
+    assert abs(blocked - exact) <= 1e-12 * np.abs(x * y).sum()
+    assert abs(serial - exact) <= 1e-12 * np.abs(x * y).sum()
