@@ -42,6 +42,7 @@ def _lib():
         lib.gip_cavity_write.argtypes = [POINTER(_Cavity), ctypes.c_char_p, ctypes.c_char_p]
         lib.gip_cavity_read.argtypes = [ctypes.c_char_p, ctypes.c_char_p, POINTER(POINTER(_Cavity))]
         lib.gip_cavity_merge.argtypes = [ctypes.c_char_p, ctypes.c_char_p, c_int32, POINTER(POINTER(_Cavity))]
+        lib.gip_cavity_split.argtypes = [POINTER(_Cavity), c_int32, ctypes.c_char_p, ctypes.c_char_p]
         lib.gip_cavity_free.argtypes = [POINTER(_Cavity)]
         lib.gip_cavity_free.restype = None
         lib.gip_cavity_last_error.restype = ctypes.c_char_p
@@ -114,6 +115,10 @@ class Cavity:
 
     def write(self, root, name):
         _check(_lib().gip_cavity_write(self._p, str(root).encode(), name.encode()))
+
+    def write_partitions(self, root, name, nparts):
+        """Write as an nparts-partition case (<name>_<nparts>p-<rank> files)."""
+        _check(_lib().gip_cavity_split(self._p, nparts, str(root).encode(), name.encode()))
 
 
 def generate(n, ny=None, nz=None, *, beta=0.0, gamma=1e-2, lid_u=1.0, lid_profile=None, pin=True, ops=OP_ALL):

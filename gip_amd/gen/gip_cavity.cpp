@@ -482,6 +482,25 @@ int write_topo(const std::string &path, const int sizes[9]) {
     return GIP_CAV_OK;
 }
 
+// A topology file with halo lists (format of GIP_Topo.cpp:12-49).
+int write_topo_lists(const std::string &path, const int sizes[9], const std::vector<int32_t> &send_indx,
+                     const std::vector<int32_t> &send_row, const std::vector<int32_t> &recv_indx,
+                     const std::vector<int32_t> &recv_row) {
+    FILE *fp = std::fopen(path.c_str(), "w");
+    if (!fp) return fail(GIP_CAV_EIO, "cannot write " + path);
+    for (int i = 0; i < 9; ++i) std::fprintf(fp, " %d", sizes[i]);
+    auto put = [&](const std::vector<int32_t> &v) {
+        std::fprintf(fp, " \n");
+        for (int32_t x : v) std::fprintf(fp, " %d", x);
+    };
+    put(send_indx);
+    put(send_row);
+    put(recv_indx);
+    put(recv_row);
+    std::fprintf(fp, " \n");
+    return std::fclose(fp) == 0 ? GIP_CAV_OK : fail(GIP_CAV_EIO, "write failed " + path);
+}
+
 }  // namespace
 
 extern "C" {
@@ -859,6 +878,186 @@ int gip_cavity_merge(const char *root, const char *name, int32_t np, gip_cavity 
         return rc;
     }
     *out = c;
+    return GIP_CAV_OK;
+}
+
+int gip_cavity_split(const gip_cavity *c, int32_t np, const char *root, const char *name) {
+    if (!c || !name || !name[0] || np < 1) return fail(GIP_CAV_EINVAL, "cavity, name and a positive partition count required");
+    const gip_csr *cellops[] = {&c->D, &c->L, &c->Mx, &c->My, &c->Mz, &c->Gx, &c->Gy, &c->Gz};
+    const gip_csr *faceops[] = {&c->SFx, &c->SFy, &c->SFz, &c->SO};
+    const gip_csr *all[] = {&c->D, &c->Ec, &c->L, &c->Lb, &c->Mx, &c->My, &c->Mz, &c->Gx,
+                            &c->Gy, &c->Gz, &c->SFx, &c->SFy, &c->SFz, &c->SO, &c->Mres};
+    for (const gip_csr *A : all)
+        if (!A->rowptr) return fail(GIP_CAV_EINVAL, "gip_cavity_split: every operator must be built");
+    const int64_t N = c->n_inner, NA = c->n_all, F = c->n_faces;
+    if (np > N) return fail(GIP_CAV_EINVAL, "gip_cavity_split: more partitions than cells");
+    if (c->Ec.nrows != c->D.nnz) return fail(GIP_CAV_ETOPO, "gip_cavity_split: Ec rows != nnz(D)");
+    // owners: inner cells in contiguous index blocks (slabs of a hex grid),
+    // each boundary ghost with the cell it is adjacent to (its Lb entry)
+    std::vector<int32_t> owner((size_t)NA, -1);
+    for (int r = 0; r < np; ++r)  // rank r: cells [r N / np, (r+1) N / np)
+        for (int64_t i = (int64_t)r * N / np; i < (int64_t)(r + 1) * N / np; ++i) owner[(size_t)i] = r;
+    for (int64_t g = N; g < NA; ++g) {
+        const int32_t b = c->Lb.rowptr[g], e = c->Lb.rowptr[g + 1];
+        if (e - b != 1 || c->Lb.col[b] < 0 || c->Lb.col[b] >= N)
+            return fail(GIP_CAV_ETOPO, "gip_cavity_split: a ghost row of Lb does not name one adjacent cell");
+        owner[(size_t)g] = owner[(size_t)c->Lb.col[b]];
+    }
+    // faces of each rank: the faces of its cells (Mres); interface faces on both sides
+    std::vector<std::vector<int32_t>> faces((size_t)np);
+    for (int64_t i = 0; i < N; ++i)
+        for (int32_t t = c->Mres.rowptr[i]; t < c->Mres.rowptr[i + 1]; ++t) {
+            const int32_t f = c->Mres.col[t];
+            if (f < 0 || f >= F) return fail(GIP_CAV_ETOPO, "gip_cavity_split: Mres column out of range");
+            faces[(size_t)owner[(size_t)i]].push_back(f);
+        }
+    for (auto &v : faces) {
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+    }
+    // cells each rank reads but does not own (its halo), and the cells that
+    // border another rank (reading or read by it): the interface cells
+    std::vector<std::vector<int32_t>> halo((size_t)np);
+    std::vector<char> iface((size_t)N, 0);
+    for (int r = 0; r < np; ++r) {
+        std::vector<int32_t> &h = halo[(size_t)r];
+        auto touch = [&](int32_t row_cell, int32_t col) -> int {
+            if (col < 0 || col >= NA) return fail(GIP_CAV_ETOPO, "gip_cavity_split: cell column out of range");
+            if (owner[(size_t)col] == r) return GIP_CAV_OK;
+            if (col >= N) return fail(GIP_CAV_ETOPO, "gip_cavity_split: a partition would read another's ghost");
+            h.push_back(col);
+            iface[(size_t)col] = 1;
+            if (row_cell >= 0) iface[(size_t)row_cell] = 1;
+            return GIP_CAV_OK;
+        };
+        const int64_t lo = (int64_t)r * N / np, hi = (int64_t)(r + 1) * N / np;
+        for (const gip_csr *A : cellops)
+            for (int64_t i = lo; i < hi; ++i)
+                for (int32_t t = A->rowptr[i]; t < A->rowptr[i + 1]; ++t)
+                    if (int rc = touch((int32_t)i, A->col[t])) return rc;
+        for (const gip_csr *A : faceops)
+            for (int32_t f : faces[(size_t)r]) {
+                int32_t mine = -1;  // the face's cell in this rank marks the interface
+                for (int32_t t = A->rowptr[f]; t < A->rowptr[f + 1]; ++t)
+                    if (A->col[t] >= 0 && A->col[t] < N && owner[(size_t)A->col[t]] == r) mine = A->col[t];
+                for (int32_t t = A->rowptr[f]; t < A->rowptr[f + 1]; ++t)
+                    if (int rc = touch(mine, A->col[t])) return rc;
+            }
+        std::sort(h.begin(), h.end(), [&](int32_t a, int32_t b) {
+            return owner[(size_t)a] != owner[(size_t)b] ? owner[(size_t)a] < owner[(size_t)b] : a < b;
+        });
+        h.erase(std::unique(h.begin(), h.end()), h.end());
+    }
+    CasePaths p = make_paths(root, name, np);
+    const char *groups[] = {"CD", "L", "M", "G", "S"};
+    for (const char *g : groups)
+        if (int rc = mkdirs(p.dir("Operators", g))) return rc;
+    if (int rc = mkdirs(p.dir("Scalarfields", ""))) return rc;
+    if (int rc = mkdirs(p.dir("Topos", ""))) return rc;
+    // local numbering of every rank: [interiors | interfaces | ghosts | halo by source rank]
+    std::vector<std::vector<int32_t>> cells((size_t)np);  // local -> global
+    std::vector<int32_t> ninter((size_t)np), nifc((size_t)np), nghost((size_t)np);
+    for (int r = 0; r < np; ++r) {
+        const int64_t lo = (int64_t)r * N / np, hi = (int64_t)(r + 1) * N / np;
+        std::vector<int32_t> &v = cells[(size_t)r];
+        for (int64_t i = lo; i < hi; ++i)
+            if (!iface[(size_t)i]) v.push_back((int32_t)i);
+        ninter[(size_t)r] = (int32_t)v.size();
+        for (int64_t i = lo; i < hi; ++i)
+            if (iface[(size_t)i]) v.push_back((int32_t)i);
+        nifc[(size_t)r] = (int32_t)(v.size() - ninter[(size_t)r]);
+        for (int64_t g = N; g < NA; ++g)
+            if (owner[(size_t)g] == r) v.push_back((int32_t)g);
+        nghost[(size_t)r] = (int32_t)(v.size() - ninter[(size_t)r] - nifc[(size_t)r]);
+        v.insert(v.end(), halo[(size_t)r].begin(), halo[(size_t)r].end());
+    }
+    std::vector<int32_t> loc((size_t)NA), floc((size_t)F);
+    for (int r = 0; r < np; ++r) {
+        const std::vector<int32_t> &cl = cells[(size_t)r];
+        const std::vector<int32_t> &fl = faces[(size_t)r];
+        const int32_t inner = ninter[(size_t)r] + nifc[(size_t)r], owned = inner + nghost[(size_t)r];
+        const int32_t alln = (int32_t)cl.size(), nf = (int32_t)fl.size();
+        std::fill(loc.begin(), loc.end(), -1);
+        std::fill(floc.begin(), floc.end(), -1);
+        for (int32_t l = 0; l < alln; ++l) loc[(size_t)cl[(size_t)l]] = l;
+        for (int32_t l = 0; l < nf; ++l) floc[(size_t)fl[(size_t)l]] = l;
+        // the rows of an operator in this rank, in its order, columns mapped
+        // (stored order kept, so every row sums its terms as the whole case does)
+        auto emit = [&](const char *group, const char *op, const gip_csr &A, const std::vector<int32_t> &rows,
+                        bool face_cols, int32_t ncols) -> int {
+            std::vector<int32_t> len(rows.size());
+            for (size_t e = 0; e < rows.size(); ++e) len[e] = A.rowptr[rows[e] + 1] - A.rowptr[rows[e]];
+            gip_csr B;
+            csr_zero(&B);
+            int rc = csr_alloc(&B, (int32_t)rows.size(), ncols, len);
+            for (size_t e = 0; e < rows.size() && !rc; ++e) {
+                int32_t q = B.rowptr[e];
+                for (int32_t t = A.rowptr[rows[e]]; t < A.rowptr[rows[e] + 1]; ++t, ++q) {
+                    const int32_t m = face_cols ? floc[(size_t)A.col[t]] : loc[(size_t)A.col[t]];
+                    if (m < 0) {
+                        rc = fail(GIP_CAV_ETOPO, std::string("gip_cavity_split: ") + op + " reads outside its partition");
+                        break;
+                    }
+                    B.col[q] = m;
+                    B.val[q] = A.val[t];
+                }
+            }
+            if (!rc) rc = gip_csr_write(p.op(group, op, r).c_str(), &B);
+            gip_csr_release(&B);
+            return rc;
+        };
+        std::vector<int32_t> inner_rows(cl.begin(), cl.begin() + inner), owned_rows(cl.begin(), cl.begin() + owned);
+        // Ec: one row per entry of this rank's D rows, in their order
+        std::vector<int32_t> ec_rows;
+        for (int32_t i : inner_rows)
+            for (int32_t t = c->D.rowptr[i]; t < c->D.rowptr[i + 1]; ++t) ec_rows.push_back(t);
+        int rc = emit("CD", "D", c->D, inner_rows, false, owned);
+        if (!rc) rc = emit("CD", "Ec", c->Ec, ec_rows, true, nf);
+        if (!rc) rc = emit("L", "L", c->L, inner_rows, false, owned);
+        if (!rc) rc = emit("L", "Lb", c->Lb, owned_rows, false, owned);
+        if (!rc) rc = emit("M", "Mx", c->Mx, inner_rows, false, inner);
+        if (!rc) rc = emit("M", "My", c->My, inner_rows, false, inner);
+        if (!rc) rc = emit("M", "Mz", c->Mz, inner_rows, false, inner);
+        if (!rc) rc = emit("M", "Mres", c->Mres, inner_rows, true, nf);
+        if (!rc) rc = emit("G", "Gx", c->Gx, owned_rows, false, owned);
+        if (!rc) rc = emit("G", "Gy", c->Gy, owned_rows, false, owned);
+        if (!rc) rc = emit("G", "Gz", c->Gz, owned_rows, false, owned);
+        if (!rc) rc = emit("S", "SFx", c->SFx, fl, false, owned);
+        if (!rc) rc = emit("S", "SFy", c->SFy, fl, false, owned);
+        if (!rc) rc = emit("S", "SFz", c->SFz, fl, false, owned);
+        if (!rc) rc = emit("S", "SO", c->SO, fl, false, owned);
+        if (rc) return rc;
+        const char *fields[] = {"u", "v", "w", "dxs"};
+        const double *fv[] = {c->u, c->v, c->w, c->dxs};
+        std::vector<double> buf((size_t)alln);
+        for (int f = 0; f < 4 && !rc; ++f) {
+            for (int32_t l = 0; l < alln; ++l) buf[(size_t)l] = fv[f][cl[(size_t)l]];
+            rc = gip_vec_write(p.vec(fields[f], r).c_str(), buf.data(), alln);
+        }
+        if (rc) return rc;
+        // halo lists: this rank's slots grouped by source rank; every other
+        // rank's send block to this rank lists the same cells in the same order
+        std::vector<int32_t> send_indx, send_row(1, 0), recv_indx, recv_row(1, 0);
+        for (int q = 0; q < np; ++q) {
+            for (int32_t l = owned; l < alln; ++l)
+                if (owner[(size_t)cl[(size_t)l]] == q) recv_indx.push_back(l);
+            recv_row.push_back((int32_t)recv_indx.size());
+            for (int32_t g : halo[(size_t)q])
+                if (owner[(size_t)g] == r) send_indx.push_back(loc[(size_t)g]);
+            send_row.push_back((int32_t)send_indx.size());
+        }
+        for (int32_t l : send_indx)
+            if (l < ninter[(size_t)r] || l >= inner)
+                return fail(GIP_CAV_ETOPO, "gip_cavity_split: a sent cell is not an interface cell");
+        const int sizes[9] = {(int)send_indx.size(), (int)recv_indx.size(), np + 1, owned, alln, ninter[(size_t)r],
+                              nifc[(size_t)r], nghost[(size_t)r], alln - owned};
+        rc = write_topo_lists(p.topo("topoCols", r), sizes, send_indx, send_row, recv_indx, recv_row);
+        // the faces topology: only its `all` (the face count) is read (GIP_DNS.h:271)
+        const int fsizes[9] = {0, 0, np + 1, nf, nf, ninter[(size_t)r], nifc[(size_t)r], nghost[(size_t)r], alln - owned};
+        const std::vector<int32_t> none, zeros((size_t)np + 1, 0);
+        if (!rc) rc = write_topo_lists(p.topo("topoFaces", r), fsizes, none, zeros, none, zeros);
+        if (rc) return rc;
+    }
     return GIP_CAV_OK;
 }
 

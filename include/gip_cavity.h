@@ -104,6 +104,19 @@ int gip_cavity_read(const char *root, const char *name, gip_cavity **out);
  * face spaces stay per rank (concatenated). */
 int gip_cavity_merge(const char *root, const char *name, int32_t nparts, gip_cavity **out);
 
+/* Write a one-partition case as an <nparts>-partition case
+ * (<name>_<nparts>p-<rank> files), the decomposition the reference's
+ * driver reads with one MPI rank per partition (GIP_DNS.h:149,
+ * DrivenCavity.cpp:30-33): inner cells in contiguous index blocks (z slabs
+ * of a hex grid), each boundary ghost with its adjacent cell, a rank's
+ * faces those of its cells (faces between two ranks on both sides), local
+ * order [interiors | interfaces | ghosts | halo grouped by source rank],
+ * halo lists paired as GIP_Topo.cpp:124-181 exchanges them.  Operator rows
+ * keep their stored column order, so every row sums its terms as in the
+ * undivided case.  gip_cavity_merge reads the result back (with the cells
+ * renumbered rank by rank). */
+int gip_cavity_split(const gip_cavity *cav, int32_t nparts, const char *root, const char *name);
+
 /* Seeded zero-mean N(0,1) field over n values (splitmix64 + Box-Muller,
  * then the mean is removed): the Poisson right-hand side of the PCG
  * workloads.  Deterministic for a given (n, seed). */
