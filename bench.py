@@ -28,9 +28,16 @@ Other suites (one JSON line each, not the default):
     --suite clustered 128^3 tanh-clustered mesh (beta 1.5), Re=1000, 100 steps,
                       PCG to 1e-10: steps/s FP64 vs mixed, the L SpMV rate and
                       the FP64/mixed PCG on a zero-mean N(0,1) right-hand side
+    --suite cpu       the CPU-baseline plan: the reference integrator at 1 and 4
+                      ranks, as shipped (-O0) and -O3, on the generated 46^3
+                      cavity and C100K, beside the GPU with 1 and 4 partitions
     --probe spmv      a few launches of the FP64 SpMV, eight iterations of each
                       PCG, and a plain copy of known size, for rocprofv3 --pmc
                       traffic runs
+    --probe stream    the practical HBM ceiling: copy, read, write and the PCG
+                      direction sweep's mix (3 reads, 2 writes) over 2 GiB
+                      arrays, in 8- and 16-byte, plain and nontemporal
+                      accesses, one access per thread or grid-strided
 """
 from __future__ import annotations
 
@@ -136,25 +143,57 @@ def spmv32_bytes(nnz, rows, cols):
     return 8 * nnz + 4 * (rows + 1) + 4 * rows + 4 * cols
 
 
-def measured_traffic(n, layout, which="fp64"):
-    """HBM bytes per launch of the FP64 SpMV at n^3 in this layout ("fp32_solve":
-    the FP32 product inside the mixed PCG), from the committed rocprofv3
-    --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known
-    size)."""
-    f = ROOT / "profiles" / f"spmv_{layout}_{which}_{n}_traffic.txt"
+def measured_traffic(n, kernel):
+    """HBM bytes per launch of a kernel of the n^3 PCG workload, from the
+    committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by a
+    copy kernel of known size): its `traffic_bytes_per_launch[kernel]` line."""
+    f = ROOT / "profiles" / f"pcg_kernels_{n}_traffic.txt"
     if not f.exists():
         return None
+    key = f"traffic_bytes_per_launch[{kernel}]:"
     for line in f.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key):
             return int(float(line.split(":", 1)[1]))
     return None
 
 
+def stream_ceiling():
+    """The best measured streaming rates (GB/s) from the committed
+    `--probe stream` summary, or None."""
+    f = ROOT / "profiles" / "stream_ceiling.txt"
+    if not f.exists():
+        return None
+    out = {}
+    for line in f.read_text().splitlines():
+        if line.startswith("best_gbs["):
+            k, v = line[len("best_gbs["):].split("]:", 1)
+            out[k] = float(v)
+    return out or None
+
+
+def host_cpu():
+    """The host's CPU model and the cores this process may use."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"model": model, "cpu_count": os.cpu_count(), "usable_cores": usable}
+
+
 def cpu_baseline(n, iters, seed, fp64_iters):
-    """The reference's own GIP_LinearSolver + Multicore spec (oracle/_ref, one
-    rank, one core) on the same Laplacian and RHS, for a bounded number of
-    fixed iterations; extrapolated to the FP64 iteration count."""
-    exe = ROOT / "oracle" / "_ref" / "gip_ref_harness"
+    """The reference's own GIP_LinearSolver + Multicore spec (oracle/_ref, the
+    reference sources built -O3 -march=x86-64-v3, one rank, one core) on the
+    same Laplacian and RHS, for a bounded number of fixed iterations, then
+    extrapolated (labelled so) to the FP64 solve's iteration count."""
+    exe = ROOT / "oracle" / "_ref" / "gip_ref_harness_O3"
     if not exe.exists():
         return None
     out = subprocess.run([str(exe), "pcg", str(n), str(iters), str(seed)], capture_output=True, text=True,
@@ -164,11 +203,12 @@ def cpu_baseline(n, iters, seed, fp64_iters):
         return None
     r = json.loads(out.stdout.strip().splitlines()[-1])
     return {"value": r["ms_per_iter"] * fp64_iters, "unit": "ms", "cores": 1, "kind": "reference",
-            "ms_per_iter": r["ms_per_iter"],
-            "sample": f"reference GIP_LinearSolver::solve (Multicore spec, 1 rank, -O2) on the {n}^3 Laplacian, "
-                      f"{iters} fixed Jacobi-PCG iterations timed ({r['solve_ms']:.0f} ms), extrapolated to the "
-                      f"{fp64_iters} iterations the FP64 solve needs to 1e-10",
-            "host": os.uname().nodename, "cpu_count": os.cpu_count()}
+            "ms_per_iter": r["ms_per_iter"], "extrapolated": True, "measured_iters": iters,
+            "measured_ms": r["solve_ms"],
+            "sample": f"reference GIP_LinearSolver::solve (Multicore spec, 1 rank, -O3 -march=x86-64-v3) on the "
+                      f"{n}^3 Laplacian: {iters} fixed Jacobi-PCG iterations timed ({r['solve_ms']:.0f} ms); value "
+                      f"EXTRAPOLATED to the {fp64_iters} iterations the FP64 solve needs to 1e-10",
+            "host": host_cpu()}
 
 
 def main():
@@ -183,13 +223,24 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=60)
     ap.add_argument("--skip-ns", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--suite", choices=("pcg", "batch", "adjoint", "clustered", "reference"), default="pcg")
-    ap.add_argument("--probe", choices=("spmv",), default=None)
+    ap.add_argument("--suite", choices=("pcg", "batch", "adjoint", "clustered", "reference", "cpu"), default="pcg")
+    ap.add_argument("--probe", choices=("spmv", "stream"), default=None)
+    ap.add_argument("--summarize", choices=("traffic", "stream"), default=None,
+                    help="write a profiles/ summary from rocprofv3 --pmc outputs or a --probe stream JSON line")
+    ap.add_argument("--fetch", default=None, help="--summarize traffic: the FETCH_SIZE run's output directory")
+    ap.add_argument("--write", default=None, help="--summarize traffic: the WRITE_SIZE run's output directory")
+    ap.add_argument("--json", default=None, help="--summarize stream: the probe's JSON output")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs one partition on one device "
               f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
         sys.exit(2)
+    if args.summarize == "traffic":
+        return summary_traffic(args)
+    if args.summarize == "stream":
+        return summary_stream(args)
+    if args.probe == "stream":
+        return probe_stream(args)
     if args.probe:
         return probe_spmv(args)
     if args.suite == "batch":
@@ -200,6 +251,8 @@ def main():
         return clustered_suite(args)
     if args.suite == "reference":
         return reference_suite(args)
+    if args.suite == "cpu":
+        return cpu_suite(args)
 
     from gip_amd import cavity, dns, hip
 
@@ -256,6 +309,11 @@ def main():
     log(f"fp64 pcg {fp64['ms']:.1f} ms ({fp64['iters']} it, true {fp64['true_relres']:.2e}); "
         f"mixed {mixed['ms']:.1f} ms ({mixed['iters']} it, true {mixed['true_relres']:.2e}); "
         f"speedup {fp64['ms'] / mixed['ms']:.2f}x")
+    # bytes one product moves in the operator's own layout (stencil: slot
+    # values, slot offsets, per-slice tail records and tail entries, the
+    # input once, the output once): no column index per entry, so fewer
+    # than the CSR count
+    lay64, lay32 = A.product_bytes(), A.product_bytes(fp32=True)
     A.destroy()
     dev.sync()
 
@@ -280,6 +338,29 @@ def main():
 
     cpu = None if args.skip_cpu else cpu_baseline(n, args.cpu_iters, args.seed, fp64["iters"])
 
+    # The dominant kernel of the path: the FP64 product k_iter_spmv inside
+    # the FP64 PCG (its median launch, timed by events inside the solves),
+    # achieved = the CSR algorithmic bytes 12 nnz + 4 (rows+1) + 8 rows +
+    # 8 cols per launch; the layout's own bytes beside it.
+    us64, us32 = fp64["product_us"], mixed["product_us"]
+    ceiling = stream_ceiling()
+    ach64 = alg / (us64 * 1e-6) / 1e9
+    roofline = {
+        "bound": "hbm", "achieved": round(ach64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(ach64 / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n, "k_iter_spmv"),
+        "algorithmic_bytes": alg, "us_per_launch": round(us64, 2),
+        "layout_bytes": lay64, "frac_layout": round(lay64 / (us64 * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+        "kernel": f"k_iter_spmv (FP64 stencil-layout product of the FP64 PCG, timed inside the solves), {n}^3"}
+    # The dominant kernel of the timed mixed solve, its FP32 product: achieved
+    # = the layout's bytes (the FP32 CSR count includes column indices this
+    # layout never reads, and would put the rate above the peak).
+    ach32 = lay32 / (us32 * 1e-6) / 1e9
+    roofline_solve = {
+        "bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n, "k_m_spmv"),
+        "algorithmic_bytes": lay32, "us_per_launch": round(us32, 2), "csr_bytes": alg32,
+        "kernel": f"k_m_spmv (FP32 stencil-layout product of the mixed PCG, timed inside the solves), {n}^3"}
+
     out = {
         "metric": METRIC,
         "value": round(mixed["ms"], 3),
@@ -299,28 +380,143 @@ def main():
         "pcg_fp64_ms": round(fp64["ms"], 3),
         "pcg_mixed_ms": round(mixed["ms"], 3),
         "mixed_speedup": round(fp64["ms"] / mixed["ms"], 3),
+        # the speed-up splits into the cheaper FP32 iteration and the fewer
+        # iterations of the deflated inner solve
+        "mixed_speedup_per_iteration": round(fp64["ms_per_iter"] / mixed["ms_per_iter"], 3),
+        "mixed_iteration_ratio": round(fp64["iters"] / max(1, mixed["iters"]), 3),
         "pcg_fp64": fp64,
         "pcg_mixed": mixed,
         "spmv_fp64": {"ms": spmv_ms, "gbs": spmv_gbs, "algorithmic_bytes": alg},
         "spmv_fp32": {"ms": spmv32_ms, "gbs": alg32 / (spmv32_ms * 1e-3) / 1e9, "algorithmic_bytes": alg32},
         "ns_steps": ns,
-        "roofline": {"bound": "hbm", "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n, hip.FORMAT_NAMES.get(info["format"], "?")),
-                     "algorithmic_bytes": alg,
-                     "kernel": {4: "k_stencil (FP64 stencil-layout SpMV)",
-                                2: "k_ell (FP64 sliced-ELL SpMV)"}.get(info["format"], "FP64 SpMV")
-                               + f", {n}^3 Laplacian"},
-        # the dominant kernel of the timed mixed solve: its FP32 product,
-        # timed in the solve (events around one launch per host poll)
-        "roofline_solve": {
-            "bound": "hbm", "achieved": round(alg32 / (mixed["product_us"] * 1e-6) / 1e9, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(alg32 / (mixed["product_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(n, hip.FORMAT_NAMES.get(info["format"], "?"), "fp32_solve"),
-            "algorithmic_bytes": alg32, "us_per_launch": round(mixed["product_us"], 2),
-            "kernel": f"k_m_spmv (FP32 product of the mixed PCG, timed inside the solve), {n}^3 Laplacian"},
+        "roofline": roofline,
+        "roofline_solve": roofline_solve,
+        "stream_ceiling_gbs": ceiling,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def summarize_pmc(fetch_dir, write_dir, kernels):
+    """Per-dispatch HBM traffic of each named kernel from two rocprofv3 --pmc
+    runs of `--probe spmv` (FETCH_SIZE, then WRITE_SIZE; counter CSVs under
+    the two directories), scaled by what the same counters read for k_copy
+    (2^27 doubles read and written per launch).  Returns
+    {kernel: {"fetch_kib", "write_kib", "bytes"}} (medians over dispatches)
+    and the two scale factors."""
+    import csv
+
+    def load(d, counter):
+        rows = {}
+        for f in pathlib.Path(d).rglob("*counter_collection.csv"):
+            with open(f, newline="") as fh:
+                for r in csv.DictReader(fh):
+                    if r.get("Counter_Name") != counter:
+                        continue
+                    rows.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        return rows
+
+    fet, wri = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
+
+    def med(d, key):
+        hits = [v for k, vs in d.items() if key in k for v in vs]
+        return float(np.median(hits)) if hits else None
+
+    copy_bytes = 8 * (1 << 27)
+    f_scale = copy_bytes / 1024 / med(fet, "k_copy")
+    w_scale = copy_bytes / 1024 / med(wri, "k_copy")
+    out = {}
+    for k in kernels:
+        fk, wk = med(fet, k), med(wri, k)
+        if fk is None or wk is None:
+            continue
+        out[k] = {"fetch_kib": fk, "write_kib": wk, "bytes": 1024 * (fk * f_scale + wk * w_scale)}
+    return out, f_scale, w_scale
+
+
+PCG_KERNELS = ("k_iter_dir", "k_iter_spmv", "k_fold_dn", "k_iter_update", "k_m_dir", "k_m_spmv", "k_m_update",
+               "k_stencil<0")
+
+
+def summary_traffic(args):
+    """profiles/pcg_kernels_<n>_traffic.txt from the two --pmc runs of
+    `--probe spmv --n <n>`."""
+    n = args.n
+    res, fs, ws = summarize_pmc(args.fetch, args.write, PCG_KERNELS)
+    N = n ** 3
+    lines = [
+        f"# HBM traffic per launch of the {n}^3 PCG kernels (rocprofv3 PMC: TCC FETCH_SIZE and WRITE_SIZE in KiB per",
+        "# dispatch, median over the dispatches of `bench.py --probe spmv`: 5 standalone FP64 products, 8",
+        "# iterations of the FP64 PCG, 8 of the mixed PCG, 5 copies of 2^27 doubles)",
+        "#",
+        "# commands (run from the repository root on one MI355X):",
+        f"#   rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -o run -- python3 -u bench.py "
+        f"--probe spmv --n {n}",
+        f"#   rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write -o run -- python3 -u bench.py "
+        f"--probe spmv --n {n}",
+        "#   python3 bench.py --summarize traffic --fetch gpurun_out/pmc_fetch --write gpurun_out/pmc_write "
+        f"--n {n}",
+        "#",
+        "# calibration: k_copy of 2^27 doubles (1 GiB read + 1 GiB written per launch, known):",
+        f"#   FETCH_SIZE scale {fs:.4f}, WRITE_SIZE scale {ws:.4f} (bytes = 1024 KiB x scale)",
+        "#",
+        "# kernel            fetch KiB     write KiB      bytes/launch   bytes/row",
+    ]
+    for k in PCG_KERNELS:
+        if k not in res:
+            continue
+        r = res[k]
+        lines.append(f"#   {k:14s} {r['fetch_kib']:12.1f} {r['write_kib']:12.1f} {r['bytes']:16.0f} "
+                     f"{r['bytes'] / N:9.2f}")
+    for k in PCG_KERNELS:
+        if k in res:
+            lines.append(f"traffic_bytes_per_launch[{k}]: {int(res[k]['bytes'])}")
+    print("\n".join(lines))
+
+
+def summary_stream(args):
+    """profiles/stream_ceiling.txt from a `--probe stream` JSON line."""
+    d = json.loads(pathlib.Path(args.json).read_text().strip().splitlines()[-1])
+    lines = [
+        "# Practical HBM streaming ceiling of this MI355X: the best of each access mix over 2 GiB arrays",
+        "# (8 GiB of HBM per test, far beyond the 256 MB Infinity Cache), HIP events over 10 launches.",
+        "#",
+        "# commands (run from the repository root on one MI355X):",
+        "#   python3 bench.py --probe stream > gpurun_out/stream.json",
+        "#   python3 bench.py --summarize stream --json gpurun_out/stream.json",
+        "#",
+        "# mix    bytes/access  nontemporal  grid   ms/launch   GB/s",
+    ]
+    for r in d["all"]:
+        lines.append(f"#   {r['kind']:6s} {r['bytes_per_access']:4d} {str(r['nontemporal']):>12s} {r['grid']:6d} "
+                     f"{r['ms']:10.4f} {r['gbs']:8.1f}")
+    for k, r in d["best"].items():
+        lines.append(f"best_gbs[{k}]: {r['gbs']}")
+    print("\n".join(lines))
+
+
+def probe_stream(args):
+    """GB/s of each streaming access mix over 2^28-double (2 GiB) arrays;
+    bytes per launch: copy 16 n, read 8 n, write 8 n, mix 40 n."""
+    from gip_amd import hip
+    dev = hip.Device(0)
+    n = 1 << 28
+    bufs = (dev.array(n), dev.array(n), dev.array(n), dev.array(1 << 22))
+    for a in bufs[:3]:
+        hip.fill(a, 1.0)
+    per = {"copy": 16, "read": 8, "write": 8, "mix": 40}
+    rows = []
+    for kind in ("copy", "read", "write", "mix"):
+        for vec in (1, 2):
+            for nt in (False, True):
+                for grid in (0, 1024, 2048, 4096, 8192, 16384):
+                    ms = hip.stream_probe(dev, kind, n, vec=vec, nontemporal=nt, grid=grid, reps=10, bufs=bufs)
+                    gbs = per[kind] * n / (ms * 1e-3) / 1e9
+                    rows.append({"kind": kind, "bytes_per_access": 8 * vec, "nontemporal": nt, "grid": grid,
+                                 "ms": round(ms, 4), "gbs": round(gbs, 1)})
+                    log(f"{kind:5s} {8 * vec:2d}B nt={int(nt)} grid={grid:5d}: {ms:.4f} ms {gbs:7.1f} GB/s")
+    best = {k: max((r for r in rows if r["kind"] == k), key=lambda r: r["gbs"]) for k in per}
+    print(json.dumps({"probe": "stream", "n_doubles": n, "best": best, "all": rows}), flush=True)
 
 
 def probe_spmv(args):
@@ -540,6 +736,74 @@ def reference_suite(args):
         "cases": rows, "cpu": {"cores": "1 (and 4 for the four-rank C100K row)", "kind": "reference",
                                "host": os.uname().nodename, "cpu_count": os.cpu_count()},
         "data": "C100K (the reference's shipped case, merged to one partition) and generated cavities"}), flush=True)
+
+
+def cpu_suite(args):
+    """The CPU-baseline plan: the reference's GIP_DNS::integrate over its
+    Multicore specification (oracle/_ref/gip_ref_harness_O0: the as-shipped
+    flags, no -O; _O3: -O3 -march=x86-64-v3), MPI_Wtime around integrate(),
+    at 1 rank (1 core) and 4 ranks (4 cores, the reference's own
+    decomposition), on the generated 46^3 cavity (Re=100; its 4-partition
+    files from gip_cavity_split) and the shipped C100K (Re=1000; 4 partitions
+    as shipped, 1 partition merged), 10 steps of fixed 100 PCG iterations;
+    beside them the GPU: one partition (device PCG) and the same 4
+    partitions on one device."""
+    import tempfile
+    from gip_amd import cavity, dns
+    exes = {b: ROOT / "oracle" / "_ref" / f"gip_ref_harness_{b}" for b in ("O0", "O3")}
+    mpirun = shutil.which("mpirun", path="/opt/conda/bin") or shutil.which("mpirun")
+    if not all(e.exists() for e in exes.values()) or not mpirun:
+        log("reference harness builds or MPI absent (built where the reference sources are present)")
+        return
+    tmp = pathlib.Path(tempfile.mkdtemp(prefix="gip_cpu_"))
+    cav = cavity.generate(46, gamma=1e-2)
+    cav.write(tmp, "cav46")
+    cav.write_partitions(tmp, "cav46", 4)
+    cases = [("cavity 46^3, Re=100", tmp, "cav46", 1e-2)]
+    if (ROOT / "data" / "InputFiles" / "Topos" / "C100K" / "C100K_4p").exists():
+        cases.append(("C100K, Re=1000", ROOT / "data", "C100K", 1e-3))
+    steps, rows = 10, []
+    for label, root, name, gamma in cases:
+        res = {"case": label, "steps": steps}
+        for build, exe in exes.items():
+            for np_ in (1, 4):
+                cmd = [str(exe), "dns", name, str(steps), str(gamma), "100"]
+                if np_ > 1:
+                    cmd = [mpirun, "-np", str(np_)] + cmd
+                out = subprocess.run(cmd, cwd=root, capture_output=True, text=True, timeout=900,
+                                     env=dict(os.environ, OMP_NUM_THREADS="1"))
+                ms = None
+                for line in out.stderr.splitlines():
+                    if line.startswith("integrate_ms"):
+                        ms = float(line.split()[1]) / steps
+                res[f"cpu_{build}_np{np_}_ms_per_step"] = ms
+        sim = dns.Simulation(root=root, name=name, gamma=gamma, pcg_mode="fused")
+        sim.step()
+        sim.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps - 1):
+            sim.step()
+        sim.sync()
+        res["gpu_1partition_device_pcg_ms_per_step"] = 1e3 * (time.perf_counter() - t0) / (steps - 1)
+        sim.close()
+        parts = dns.PartitionedSimulation(root=root, name=name, nparts=4, gamma=gamma)
+        parts.step()
+        t0 = time.perf_counter()
+        for _ in range(steps - 1):
+            parts.step()
+        res["gpu_4partitions_ms_per_step"] = 1e3 * (time.perf_counter() - t0) / (steps - 1)
+        parts.close()
+        log(f"{label}: " + ", ".join(f"{k} {v:.2f}" for k, v in res.items() if isinstance(v, float)))
+        rows.append(res)
+    print(json.dumps({
+        "metric": "integrator ms per step: the reference CPU path at 1 and 4 ranks, -O0 and -O3, beside the GPU",
+        "value": rows[0]["gpu_1partition_device_pcg_ms_per_step"], "unit": "ms/step", "n_gpus": 1,
+        "higher_is_better": False, "cases": rows,
+        "cpu": {"kind": "reference", "cores": "1 per rank (1 and 4 ranks)", "host": host_cpu(),
+                "builds": {"O0": "as shipped (compile_cpu.sh: no -O)", "O3": "-O3 -march=x86-64-v3"},
+                "timer": "MPI_Wtime around integrate()"},
+        "data": "generated 46^3 cavity (1 and 4 partitions) and the shipped C100K (4 partitions; 1 merged)"}),
+        flush=True)
 
 
 def adjoint_suite(args):

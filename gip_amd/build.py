@@ -53,7 +53,8 @@ def build_cavity():
           "-Wl,-rpath,$ORIGIN/..", "-o", BIN / "gip_cavity_tool"])
 
 
-HIP_SOURCES = ["gip_runtime.hip", "gip_blas1.hip", "gip_spmv.hip", "gip_pcg.hip", "gip_batch.hip", "gip_adjoint.hip"]
+HIP_SOURCES = ["gip_runtime.hip", "gip_blas1.hip", "gip_spmv.hip", "gip_pcg.hip", "gip_batch.hip", "gip_adjoint.hip",
+               "gip_stream.hip"]
 
 
 def build_hip():
@@ -66,7 +67,7 @@ SPEC_SOURCES = [
     "AbstractInterfaces/GIP_Abstract.cpp",
     "Specifications/HIP/GIP_ArchHIP.cpp", "Specifications/HIP/GIP_VectorHIP.cpp",
     "Specifications/HIP/GIP_MatrixHIP.cpp", "Specifications/HIP/GIP_NonLinearHIP.cpp",
-    "Specifications/HIP/GIP_TimerHIP.cpp",
+    "Specifications/HIP/GIP_TimerHIP.cpp", "Specifications/HIP/GIP_PartitionHIP.cpp",
 ]
 
 

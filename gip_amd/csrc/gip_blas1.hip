@@ -24,6 +24,15 @@ __global__ __launch_bounds__(kBlock) void k_axpby(int n, double a, const double 
     if (i < n) y[i] = a * x[i] + b * y[i];
 }
 
+// y[dst ? dst[i] : i] = x[src ? src[i] : i]: the halo pack and unpack
+// (send list -> send block, senders' blocks -> halo slots)
+__global__ __launch_bounds__(kBlock) void k_copy_indexed(int n, const int32_t *__restrict__ src,
+                                                        const double *__restrict__ x, const int32_t *__restrict__ dst,
+                                                        double *__restrict__ y) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) y[dst ? dst[i] : i] = x[src ? src[i] : i];
+}
+
 __global__ __launch_bounds__(kBlock) void k_copy(int n, const double *__restrict__ x, double *__restrict__ y) {
     int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) y[i] = x[i];
@@ -141,6 +150,14 @@ int gip_dcopy(gip_ctx *c, int32_t n, const double *x, double *y) {
     if (!n) return GIP_OK;
     k_copy<<<blocks_for(n), kBlock, 0, c->stream>>>(n, x, y);
     GIP_LAUNCH_CHECK("gip_dcopy");
+    return GIP_OK;
+}
+
+int gip_dcopy_indexed(gip_ctx *c, int32_t n, const int32_t *src, const double *x, const int32_t *dst, double *y) {
+    if (!c || n < 0 || (n && (!x || !y))) return set_error(GIP_EINVAL, "gip_dcopy_indexed: bad argument");
+    if (!n) return GIP_OK;
+    k_copy_indexed<<<blocks_for(n), kBlock, 0, c->stream>>>(n, src, x, dst, y);
+    GIP_LAUNCH_CHECK("gip_dcopy_indexed");
     return GIP_OK;
 }
 

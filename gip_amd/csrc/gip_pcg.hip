@@ -72,6 +72,7 @@ enum {
     S_ZAZ = 8,      // 1.A1 over the solved rows (MIXED coarse step)
     S_ALPHA0 = 10,  // alpha of even iterations, as the update kernel formed it
     S_ALPHA1 = 11,  // alpha of odd iterations
+    S_BETA = 12,    // beta of the next direction update (long FP64 operators)
     S_COUNT = 16
 };
 // State word: ST_RUN, ST_BREAK, or ST_CONV; the direction kernel of
@@ -616,6 +617,147 @@ __global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, int k) {
         d.P(P_RR)[blockIdx.x] = red[1];
     }
     if (leader()) *d.iters = k + 1;
+}
+
+// ---- FP64 iteration on long operators (more than 2^19 rows) -----------------
+//
+// One row per thread over the whole operator, workgroups in dispatch order
+// (consecutive 256-row chunks on consecutive XCDs, the whole chip streaming
+// one narrow window of every array): the direction and update sweeps run
+// at the pace of a plain stream of their access mix.  So many workgroups
+// cannot each re-reduce the previous kernel's partials, so the scalars come
+// from one-workgroup kernels instead:
+//   K0  x += alpha_{k-1} p, p = 1.0 s + beta p        (beta, alpha from slots)
+//   K1  Ap = A p, per-wavefront p.Ap partials          (iter_spmv)
+//   fold + F1: alpha = rho_k / (p.Ap)                  (breakdown test)
+//   K2  r += (-alpha) Ap, s = Dinv r, per-wavefront r.s and r.r partials
+//   fold + F2: rho_{k+1}, r.r, the convergence test, beta
+// F2 stamps a convergence with the next iteration's index; that
+// iteration's K0 (or the final flush) then applies the last x update.
+
+__device__ inline double ntload(const double *p) { return __builtin_nontemporal_load(p); }
+__device__ inline void ntstore(double *p, double v) { __builtin_nontemporal_store(v, p); }
+
+// K0 of iteration k >= 1
+__global__ __launch_bounds__(kBlock) void k_iter_dir_rows(PcgDev d, double *__restrict__ p, int k) {
+    const int st = state_load(d.state);
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    if (row >= d.n) return;
+    const double alpha = alpha_slot(d, k - 1);
+    double *__restrict__ x = d.x;
+    if (st == conv_stamp(k)) {  // converged after iteration k-1: its x update only
+        x[row] = x[row] + alpha * p[row];  // :95
+        return;
+    }
+    if (st != ST_RUN) return;
+    const double beta = d.sc[S_BETA];
+    // nontemporal accesses: each array is streamed once per sweep (a plain
+    // stream of this mix runs 8% faster with them on this device)
+    const double xo = ntload(x + row), si = ntload(d.s + row), po = ntload(p + row);
+    ntstore(x + row, xo + alpha * po);       // :95 of iteration k-1
+    ntstore(p + row, 1.0 * si + beta * po);  // :113
+}
+
+// the per-wavefront partials of a and b, folded in index order into the
+// Gdn partials of arrays ia and ib (b may be null)
+__global__ __launch_bounds__(kBlock) void k_fold_two(PcgDev d, const double *__restrict__ a,
+                                                    const double *__restrict__ b, int ia, int ib) {
+    if (!running(d)) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double va = 0.0, vb = 0.0;
+    if (i < d.nchunk) {
+#pragma unroll
+        for (int j = 0; j < kWaves; ++j) va = va + a[(size_t)i * kWaves + j];
+        if (b)
+#pragma unroll
+            for (int j = 0; j < kWaves; ++j) vb = vb + b[(size_t)i * kWaves + j];
+    }
+    double red[2] = {va, vb};
+    block_reduce_n<2>(red);
+    if (threadIdx.x == 0) {
+        d.P(ia)[blockIdx.x] = red[0];
+        if (b) d.P(ib)[blockIdx.x] = red[1];
+    }
+}
+
+// F1 (one workgroup): alpha_k = rho_k / p.Ap, or a breakdown
+__global__ __launch_bounds__(kBlock) void k_alpha_fin(PcgDev d, int k) {
+    if (!running(d)) return;
+    const double dn = sum_partials<RED_SUM>(d.P(P_DN), d.Gdn);
+    const double alpha = rho_slot(d, k) / dn;  // :93
+    if (threadIdx.x == 0) {
+        if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha))
+            state_store(d.state, ST_BREAK);
+        else
+            d.sc[S_ALPHA0 + (k & 1)] = alpha;
+    }
+}
+
+// K2 of iteration k
+__global__ __launch_bounds__(kBlock) void k_iter_update_rows(PcgDev d, int k) {
+    if (!running(d)) return;
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    const double malpha = -1.0 * alpha_slot(d, k);
+    double rs = 0.0, rr = 0.0;
+    if (row < d.n) {
+        const double ro = ntload(d.r + row), apr = ntload(d.Ap + row);
+        const double di = d.precond ? ntload(d.dinv + row) : 0.0;
+        const double ri = ro + malpha * apr;  // :97
+        ntstore(d.r + row, ri);
+        double si;
+        if (d.precond) {  // :99-105
+            si = 0.0;
+            si = si + ri * di;
+        } else {
+            si = ri;
+        }
+        ntstore(d.s + row, si);
+        rs = ri * si;
+        rr = ri * ri;
+    }
+    // one partial per wavefront: no barrier
+    rs = wave_sum(rs);
+    rr = wave_sum(rr);
+    const int wv = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if ((threadIdx.x & 63) == 0) {
+        d.chunk[wv] = rs;
+        d.chunk_s[wv] = rr;
+    }
+    if (leader()) *d.iters = k + 1;
+}
+
+// F2 (one workgroup) after K2 of iteration k - 1: rho_k, r.r, the
+// convergence test (stamped with k), beta_k
+__global__ __launch_bounds__(kBlock) void k_beta_fin(PcgDev d, int k) {
+    if (!running(d)) return;
+    double sums[2];
+    sum_partials_n<2>({d.P(P_RS), d.P(P_RR)}, d.Gdn, sums);
+    if (threadIdx.x != 0) return;
+    const double rho = sums[0], rr = sums[1], tol2 = d.sc[S_TOL2];
+    d.sc[S_RR] = rr;
+    if (tol2 >= 0.0 && rr <= tol2) {
+        state_store(d.state, conv_stamp(k));
+        return;
+    }
+    d.sc[S_BETA] = rho / rho_slot(d, k - 1);  // :107-111
+    d.sc[S_RHO0 + (k & 1)] = rho;
+}
+
+// x += alpha_k p of the last iteration k, unless a breakdown stopped the solve
+__global__ __launch_bounds__(kBlock) void k_x_flush_rows(PcgDev d, const double *__restrict__ p, int k) {
+    const int st = state_load(d.state);
+    if (st != ST_RUN && st != conv_stamp(k + 1)) return;
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    if (row < d.n) d.x[row] = d.x[row] + alpha_slot(d, k) * p[row];
+}
+
+// one workgroup after the last iteration: the final convergence test on the
+// r.r F2 left
+__global__ __launch_bounds__(kBlock) void k_fin_rows(PcgDev d) {
+    if (threadIdx.x != 0) return;
+    const double rr = d.sc[S_RR], tol2 = d.sc[S_TOL2];
+    const int st = state_load(d.state);
+    if ((st == ST_RUN || (st & ST_MASK) == ST_CONV) && tol2 >= 0.0 && rr <= tol2) state_store(d.state, ST_CONV);
 }
 
 // one workgroup after the last iteration: r.r and the final convergence test
@@ -1241,9 +1383,10 @@ static int poll(gip_pcg *S, bool scalars) {
     return GIP_OK;
 }
 
-int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, const gip_pcg_opts *o, gip_pcg_stats *stats) {
-    if (!S || !b || !x || !x0 || !o || o->maxit < 0) return set_error(GIP_EINVAL, "gip_pcg_solve: bad argument");
-    if (o->precond && S->A->diag_missing) return set_error(GIP_EINVAL, "gip_pcg_solve: Jacobi needs every diagonal entry");
+// The solve on a solver's workspace (allocated by gip_pcg_create);
+// arguments already checked.
+static int pcg_solve_impl(gip_pcg *S, const double *b, double *x, const double *x0, const gip_pcg_opts *o,
+                          gip_pcg_stats *stats) {
     gip_ctx *c = S->ctx;
     hipStream_t st = c->stream;
     const gip_mat *A = S->A;
@@ -1311,12 +1454,16 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
             int chunk = std::min(fixed ? 4096 : pollk, o->maxit - k);
             for (int j = 0; j < chunk; ++j, ++k) {
                 if (S->fold) {
-                    if (k > 0) k_iter_dir<<<S->grid, kBlock, 0, st>>>(d, S->p, k);
+                    // (partials: p.Ap in chunk, then r.s in chunk and r.r in chunk_s)
+                    if (k > 0) k_iter_dir_rows<<<S->nchunk, kBlock, 0, st>>>(d, S->p, k);
                     const bool smp = sample_begin(k);
                     DISPATCH_WG(A, S->nchunk, k_iter_spmv, v, d, S->p);
                     sample_end(smp);
-                    k_fold_dn<<<S->gdn, kBlock, 0, st>>>(d);
-                    k_iter_update<false><<<S->grid, kBlock, 0, st>>>(d, k);
+                    k_fold_two<<<S->gdn, kBlock, 0, st>>>(d, d.chunk, nullptr, P_DN, P_DN);
+                    k_alpha_fin<<<1, kBlock, 0, st>>>(d, k);
+                    k_iter_update_rows<<<S->nchunk, kBlock, 0, st>>>(d, k);
+                    k_fold_two<<<S->gdn, kBlock, 0, st>>>(d, d.chunk, d.chunk_s, P_RS, P_RR);
+                    k_beta_fin<<<1, kBlock, 0, st>>>(d, k + 1);
                 } else {
                     const bool smp = sample_begin(k);
                     if (k > 0 && o->tol > 0.0)
@@ -1335,8 +1482,13 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
                 sample_read();
             }
         }
-        if (k > 0) k_x_flush<<<S->grid, kBlock, 0, st>>>(d, pdir(k - 1), k - 1);
-        k_fin<<<1, kBlock, 0, st>>>(d);
+        if (S->fold) {
+            if (k > 0) k_x_flush_rows<<<S->nchunk, kBlock, 0, st>>>(d, S->p, k - 1);
+            k_fin_rows<<<1, kBlock, 0, st>>>(d);
+        } else {
+            if (k > 0) k_x_flush<<<S->grid, kBlock, 0, st>>>(d, pdir(k - 1), k - 1);
+            k_fin<<<1, kBlock, 0, st>>>(d);
+        }
         GIP_LAUNCH_CHECK("pcg final test");
     } else {
         // FP64 residual of x0 (with the coarse step), FP32 copies, first direction
@@ -1413,6 +1565,12 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
         }
     }
     return status == GIP_EBREAKDOWN ? set_error(GIP_EBREAKDOWN, "PCG breakdown (p.Ap == 0 or non-finite step)") : GIP_OK;
+}
+
+int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, const gip_pcg_opts *o, gip_pcg_stats *stats) {
+    if (!S || !b || !x || !x0 || !o || o->maxit < 0) return set_error(GIP_EINVAL, "gip_pcg_solve: bad argument");
+    if (o->precond && S->A->diag_missing) return set_error(GIP_EINVAL, "gip_pcg_solve: Jacobi needs every diagonal entry");
+    return pcg_solve_impl(S, b, x, x0, o, stats);
 }
 
 }  // extern "C"

@@ -85,6 +85,11 @@ int gip_finalize(gip_ctx *c) {
     return GIP_OK;
 }
 
+int gip_use_device(int device) {
+    GIP_HIP_TRY(hipSetDevice(device));
+    return GIP_OK;
+}
+
 int gip_sync(gip_ctx *c) {
     if (!c) return set_error(GIP_EINVAL, "null context");
     GIP_HIP_TRY(hipStreamSynchronize(c->stream));

@@ -540,6 +540,32 @@ int gip_mat_info(const gip_mat *A, int32_t *nrows, int32_t *nnz, int32_t *format
     return GIP_OK;
 }
 
+int gip_mat_product_bytes(const gip_mat *A, int32_t fp32, int64_t *bytes) {
+    if (!A || !bytes) return set_error(GIP_EINVAL, "gip_mat_product_bytes: bad argument");
+    const int64_t vb = fp32 ? 4 : 8, rows = A->nrows, nnz = A->nnz, cols = A->cols_needed;
+    const int64_t vecs = vb * cols + vb * rows;  // the input's columns once, the output once
+    int64_t b;
+    switch (A->format) {
+        case GIP_FMT_STENCIL: {
+            const int64_t slots = (int64_t)A->nslices * A->st_w;
+            b = slots * 64 * vb + slots * 4                        // slot values, slot offsets
+                + (int64_t)A->nslices * (int64_t)sizeof(StencilTail)  // per-slice tail records
+                + (int64_t)A->st_tail_nnz * (vb + 4) + vecs;        // tail values and columns
+            break;
+        }
+        case GIP_FMT_ELL:
+            b = A->ell_stored * (vb + 4) + 4 * ((int64_t)A->nslices + 1) + vecs;
+            break;
+        case GIP_FMT_DIAG:
+            b = vb * rows + vecs;
+            break;
+        default:
+            b = nnz * (vb + 4) + 4 * (rows + 1) + vecs;
+    }
+    *bytes = b;
+    return GIP_OK;
+}
+
 int gip_mat_autotune(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, const int32_t *rowptr, const int32_t *col,
                      const double *val, int32_t flags, const double *x, double *y, int32_t reps, gip_mat **out,
                      gip_tune_report *rep) {

@@ -35,6 +35,12 @@ def lib():
         L.gip_dns_sync.argtypes = [c_void_p]
         L.gip_dns_close.argtypes = [c_void_p]
         L.gip_dns_last_error.restype = ctypes.c_char_p
+        L.gip_dns_open_parts.argtypes = [ctypes.c_char_p, ctypes.c_char_p, c_int32, POINTER(Config), POINTER(c_void_p)]
+        L.gip_dns_parts_step.argtypes = [c_void_p, POINTER(StepInfo)]
+        L.gip_dns_parts_sizes.argtypes = [c_void_p, c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32),
+                                          POINTER(c_int32)]
+        L.gip_dns_parts_get_field.argtypes = [c_void_p, c_int32, ctypes.c_char_p, c_void_p, c_int32]
+        L.gip_dns_parts_close.argtypes = [c_void_p]
         L._gip_typed = True
     return L
 
@@ -94,6 +100,54 @@ class Simulation:
     def close(self):
         if self.h:
             lib().gip_dns_close(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PartitionedSimulation:
+    """An <nparts>-partition case (<name>_<nparts>p-<rank> files) stepped on
+    one device, one host thread and stream per partition (gip_dns_open_parts)."""
+
+    def __init__(self, *, root, name, nparts, gamma=1e-3, rho=1.0, pcg_maxit=100, pcg_tol=0.0, print_mode=0,
+                 device=0):
+        cfg = Config(gamma, rho, pcg_maxit, pcg_tol, MODES["reference"], 0.1, print_mode, device)
+        self.h = c_void_p()
+        _check(lib().gip_dns_open_parts(str(root).encode(), name.encode(), nparts, byref(cfg), byref(self.h)))
+        self.nparts = nparts
+        self.sizes = []
+        for r in range(nparts):
+            i, o, a, f = c_int32(), c_int32(), c_int32(), c_int32()
+            _check(lib().gip_dns_parts_sizes(self.h, r, byref(i), byref(o), byref(a), byref(f)))
+            self.sizes.append({"inner": i.value, "owned": o.value, "all": a.value, "faces": f.value})
+
+    def step(self):
+        s = StepInfo()
+        _check(lib().gip_dns_parts_step(self.h, byref(s)))
+        return {"time": s.time, "dt": s.dt, "massr": s.massr, "step": s.step, "pcg_iters": s.pcg_iters,
+                "pcg_status": s.pcg_status}
+
+    def field(self, rank, name):
+        n = self.sizes[rank]["faces" if name == "mf" else "all"]
+        out = np.empty(n, np.float64)
+        _check(lib().gip_dns_parts_get_field(self.h, rank, name.encode(), out.ctypes.data, n))
+        return out
+
+    def merged_field(self, name):
+        """A cell field in the order gip_cavity_merge numbers the cells: every
+        partition's inner cells, then every partition's boundary ghosts."""
+        parts = [self.field(r, name) for r in range(self.nparts)]
+        inner = [p[:s["inner"]] for p, s in zip(parts, self.sizes)]
+        ghosts = [p[s["inner"]:s["owned"]] for p, s in zip(parts, self.sizes)]
+        return np.concatenate(inner + ghosts)
+
+    def close(self):
+        if self.h:
+            lib().gip_dns_parts_close(self.h)
             self.h = c_void_p()
 
     def __del__(self):

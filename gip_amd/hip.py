@@ -41,6 +41,7 @@ SIGNATURES = {
     "gip_init": ([c_int, POINTER(c_void_p)], c_int),
     "gip_finalize": ([c_void_p], c_int),
     "gip_sync": ([c_void_p], c_int),
+    "gip_use_device": ([c_int], c_int),
     "gip_stream": ([c_void_p], c_void_p),
     "gip_device_count": ([POINTER(c_int)], c_int),
     "gip_malloc": ([c_void_p, c_size_t, POINTER(c_void_p)], c_int),
@@ -57,6 +58,7 @@ SIGNATURES = {
     "gip_mat_destroy": ([c_void_p], c_int),
     "gip_mat_info": ([c_void_p, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32), POINTER(c_int64)], c_int),
     "gip_mat_cols_needed": ([c_void_p, POINTER(c_int32)], c_int),
+    "gip_mat_product_bytes": ([c_void_p, c_int32, POINTER(c_int64)], c_int),
     "gip_mat_set_lanes": ([c_void_p, c_int32], c_int),
     "gip_mat_autotune": ([c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_int32, c_void_p,
                           c_void_p, c_int32, POINTER(c_void_p), c_void_p], c_int),
@@ -73,6 +75,7 @@ SIGNATURES = {
     "gip_daxpy": ([c_void_p, c_int32, c_double, c_void_p, c_void_p], c_int),
     "gip_daxpby": ([c_void_p, c_int32, c_double, c_void_p, c_double, c_void_p], c_int),
     "gip_dcopy": ([c_void_p, c_int32, c_void_p, c_void_p], c_int),
+    "gip_dcopy_indexed": ([c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
     "gip_ddot": ([c_void_p, c_int32, c_void_p, c_void_p, POINTER(c_double)], c_int),
     "gip_dnrm_inf": ([c_void_p, c_int32, c_void_p, POINTER(c_double)], c_int),
     "gip_cfl": ([c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double,
@@ -82,6 +85,8 @@ SIGNATURES = {
     "gip_pcg_create": ([c_void_p, c_void_p, c_int32, POINTER(c_void_p)], c_int),
     "gip_pcg_solve": ([c_void_p, c_void_p, c_void_p, c_void_p, POINTER(PcgOpts), POINTER(PcgStats)], c_int),
     "gip_pcg_destroy": ([c_void_p], c_int),
+    "gip_stream_probe": ([c_void_p, c_int32, c_int32, c_int32, c_int32, c_size_t, c_void_p, c_void_p, c_void_p,
+                          c_void_p, c_size_t, c_int32, POINTER(c_float)], c_int),
 }
 
 
@@ -226,6 +231,12 @@ class Matrix:
         nr, nz, fm, st = c_int32(), c_int32(), c_int32(), c_int64()
         check(lib().gip_mat_info(self.ptr, byref(nr), byref(nz), byref(fm), byref(st)))
         return {"nrows": nr.value, "nnz": nz.value, "format": fm.value, "stored": st.value}
+
+    def product_bytes(self, fp32=False):
+        """Bytes one product moves in this operator's layout (gip_mat_product_bytes)."""
+        b = c_int64()
+        check(lib().gip_mat_product_bytes(self.ptr, 1 if fp32 else 0, byref(b)))
+        return b.value
 
     def values(self):
         out = np.empty(self.nnz, np.float64)
@@ -391,6 +402,21 @@ class PCG:
             self.destroy()
         except Exception:
             pass
+
+
+STREAM_KINDS = {"copy": 0, "read": 1, "write": 2, "mix": 3}
+
+
+def stream_probe(dev: Device, kind, n, *, vec=1, nontemporal=False, grid=0, reps=10, bufs=None):
+    """Mean time (ms) of one launch of a streaming access mix over n doubles
+    (gip_stream_probe); `bufs` (a, b, c, part) may be passed to reuse arrays."""
+    if bufs is None:
+        bufs = (dev.array(n), dev.array(n), dev.array(n), dev.array(1 << 22))
+    a, b, c, part = bufs
+    ms = c_float()
+    check(lib().gip_stream_probe(dev.ctx, STREAM_KINDS[kind], vec, 1 if nontemporal else 0, grid, n, a.ptr, b.ptr,
+                                 c.ptr, part.ptr, part.n, reps, byref(ms)))
+    return ms.value
 
 
 class Event:

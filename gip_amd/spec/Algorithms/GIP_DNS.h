@@ -15,8 +15,8 @@
 // operators from memory instead of InputFiles; step() runs one step and
 // reports its numbers; an optional hook is called at the six phase
 // boundaries of a step (0 before the predictor ... 5 after the CFL phase),
-// for timing.  Files are read for one partition (<name>_1p/...-0),
-// relative to root().
+// for timing.  Files are read relative to root() for the node's partition
+// (<name>_<np>p/...-<rank>; one partition: <name>_1p/...-0).
 #ifndef _gip_dns_
 #define _gip_dns_
 
@@ -56,6 +56,7 @@ class GIP_DNS {
     Vector &field(const std::string &f);
     int innerSize() { return topoCols.getInnerSize(); }
     int allSize() { return topoCols.getAllSize(); }
+    int ownedSize() { return topoCols.getOwnedSize(); }
     int facesSize() { return topoFaces.getAllSize(); }
     double currentDt() const { return dt; }
     double currentTime() const { return time; }
@@ -103,14 +104,24 @@ void GIP_DNS<Matrix, Vector, Node, NonLinear>::configure(const GIP_Parameters &p
     solverParam.setBool("precond", p.hasBool("pcg_precond") ? p.getBool("pcg_precond") : true);
     solverParam.setString("mode", p.hasString("pcg_mode") ? p.getString("pcg_mode") : std::string("reference"));
     solverParam.setDouble("innerTol", p.hasDouble("pcg_innerTol") ? p.getDouble("pcg_innerTol") : 0.1);
+    // the device-resident solvers work on one partition; several partitions
+    // take the reference sequence (halo refresh and global dot products
+    // through the node's partition group)
+    if (nodo.partCount() > 1 && solverParam.getString("mode") != "reference")
+        throw std::runtime_error("several partitions need the reference PCG sequence (pcg_mode reference)");
+    if (nodo.partRank() != 0) print = 0;  // rank 0 prints, as GIP_DNS.h:403
 }
 
 template <class Matrix, class Vector, class Node, class NonLinear>
 std::string GIP_DNS<Matrix, Vector, Node, NonLinear>::path(const char *kind, const char *group, const char *op,
                                                           const char *ext) {
+    // <name>_<np>p/<op><name>_<np>p-<rank> (GIP_DNS.h:149): np and rank are
+    // those of the node's partition group (1 and 0 without one)
     std::string base = root.empty() ? std::string() : root + "/";
     std::string g = group[0] ? std::string(group) + "/" : std::string();
-    return base + "InputFiles/" + kind + "/" + g + name + "/" + name + "_1p/" + op + name + "_1p-0" + ext;
+    const std::string np = std::to_string(nodo.partCount()) + "p";
+    return base + "InputFiles/" + kind + "/" + g + name + "/" + name + "_" + np + "/" + op + name + "_" + np + "-" +
+           std::to_string(nodo.partRank()) + ext;
 }
 
 template <class Matrix, class Vector, class Node, class NonLinear>

@@ -3,6 +3,10 @@
 #include <cstdio>
 #include <cstdlib>
 
+#include "GIP_PartitionHIP.h"
+
+int GIP_ArchHIP::partCount() const { return group_ ? group_->size() : 1; }
+
 void gip_check(int rc, const char *what) {
     if (rc != GIP_OK) {
         std::fprintf(stderr, "gip_hip: %s failed (%d): %s\n", what, rc, gip_last_error());

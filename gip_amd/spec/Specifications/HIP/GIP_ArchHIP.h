@@ -29,6 +29,16 @@ class GIP_ArchHIP : public GIP_Arch {
     int device() const { return devId; }
     void setDevice(int d) { devId = d; }
 
+    // membership in a group of partitions stepped together on one device
+    // (GIP_PartitionHIP.h); without one the node is the only partition
+    void setPartition(class GIP_PartitionGroup *group, int rank) {
+        group_ = group;
+        rank_ = rank;
+    }
+    class GIP_PartitionGroup *partition() { return group_; }
+    int partRank() const { return rank_; }
+    int partCount() const;
+
     // device buffer pool
     void *acquire(size_t bytes);
     void release(void *ptr, size_t bytes);
@@ -39,6 +49,8 @@ class GIP_ArchHIP : public GIP_Arch {
    protected:
     int devId = 0;
     gip_ctx *ctx_ = nullptr;
+    class GIP_PartitionGroup *group_ = nullptr;
+    int rank_ = 0;
     std::multimap<size_t, void *> pool_;
 };
 

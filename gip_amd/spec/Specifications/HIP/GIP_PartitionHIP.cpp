@@ -1,0 +1,122 @@
+#include "GIP_PartitionHIP.h"
+
+#include <string>
+
+#include "GIP_ArchHIP.h"
+#include "../../BasicInterfaces/GIP_Topo.h"
+
+GIP_PartitionGroup::GIP_PartitionGroup(int nparts, int device)
+    : n_(nparts), device_(device), vals_((size_t)nparts, 0.0), topo_((size_t)nparts, nullptr),
+      send_off_((size_t)nparts + 1, 0), maps_((size_t)nparts) {
+    if (nparts < 1) throw std::runtime_error("GIP_PartitionGroup: at least one partition");
+    gip_check(gip_init(device, &ctx_), "gip_init (partition group)");
+}
+
+GIP_PartitionGroup::~GIP_PartitionGroup() {
+    for (Maps &m : maps_) {
+        if (m.send_indx) gip_free(ctx_, m.send_indx);
+        if (m.recv_indx) gip_free(ctx_, m.recv_indx);
+        if (m.src) gip_free(ctx_, m.src);
+    }
+    if (d_send_) gip_free(ctx_, d_send_);
+    gip_finalize(ctx_);
+}
+
+void GIP_PartitionGroup::abort() {
+    std::lock_guard<std::mutex> l(mu_);
+    aborted_ = true;
+    cv_.notify_all();
+}
+
+void GIP_PartitionGroup::barrier() {
+    std::unique_lock<std::mutex> l(mu_);
+    if (aborted_) throw std::runtime_error("another partition failed");
+    const long gen = generation_;
+    if (++arrived_ == n_) {
+        arrived_ = 0;
+        ++generation_;
+        cv_.notify_all();
+        return;
+    }
+    cv_.wait(l, [&] { return generation_ != gen || aborted_; });
+    if (generation_ == gen) throw std::runtime_error("another partition failed");
+}
+
+double GIP_PartitionGroup::allreduce(int rank, double v, int op) {
+    if (n_ == 1) return v;
+    vals_[(size_t)rank] = v;
+    barrier();
+    double r = vals_[0];
+    for (int q = 1; q < n_; ++q) {
+        const double x = vals_[(size_t)q];
+        if (op == SUM)
+            r = r + x;
+        else if (op == MAX)
+            r = x > r ? x : r;  // a NaN never replaces the running value
+        else
+            r = x < r ? x : r;
+    }
+    barrier();  // nobody overwrites vals_ before every partition has read it
+    return r;
+}
+
+static int32_t *upload_ints(gip_ctx *c, const int *v, size_t n) {
+    void *p = nullptr;
+    gip_check(gip_malloc(c, sizeof(int32_t) * (n ? n : 1), &p), "gip_malloc (halo map)");
+    if (n) gip_check(gip_memcpy_h2d(c, p, v, sizeof(int32_t) * n), "halo map upload");
+    return static_cast<int32_t *>(p);
+}
+
+// Pairs every halo slot of every partition with its sender's entry: block q
+// of rank r's receive list and block r of rank q's send list name the same
+// cells in the same order (GIP_Topo.cpp:124-181).
+void GIP_PartitionGroup::registerTopo(int rank, GIP_Topo *topo, gip_ctx *ctx) {
+    topo_[(size_t)rank] = topo;
+    barrier();
+    if (rank == 0) {
+        for (int q = 0; q < n_; ++q) send_off_[(size_t)q + 1] = send_off_[(size_t)q] + topo_[(size_t)q]->getSendSize();
+        void *p = nullptr;
+        gip_check(gip_malloc(ctx_, sizeof(double) * (size_t)(send_off_[(size_t)n_] ? send_off_[(size_t)n_] : 1), &p),
+                  "gip_malloc (halo send buffer)");
+        d_send_ = static_cast<double *>(p);
+    }
+    barrier();
+    GIP_Topo *t = topo;
+    const int *rrow = t->getRecvRow();
+    std::vector<int> src((size_t)t->getRecvSize());
+    for (int q = 0; q < n_; ++q) {
+        const int *srow = topo_[(size_t)q]->getSendRow();
+        const int nrecv = rrow[q + 1] - rrow[q], nsend = srow[rank + 1] - srow[rank];
+        if (nrecv != nsend) {
+            abort();
+            throw std::runtime_error("partition " + std::to_string(rank) + ": halo block from partition " +
+                                     std::to_string(q) + " has " + std::to_string(nrecv) + " slots, its sender lists " +
+                                     std::to_string(nsend));
+        }
+        for (int k = 0; k < nrecv; ++k) src[(size_t)(rrow[q] + k)] = (int)(send_off_[(size_t)q] + srow[rank] + k);
+    }
+    Maps &m = maps_[(size_t)rank];
+    m.ctx = ctx;
+    m.nsend = t->getSendSize();
+    m.nrecv = t->getRecvSize();
+    m.send_indx = upload_ints(ctx_, t->getSendIndx(), (size_t)m.nsend);
+    m.recv_indx = upload_ints(ctx_, t->getRecvIndx(), (size_t)m.nrecv);
+    m.src = upload_ints(ctx_, src.data(), src.size());
+    gip_check(gip_sync(ctx_), "halo maps");
+}
+
+void GIP_PartitionGroup::exchange(int rank, GIP_Topo *topo, gip_ctx *ctx, double *dvec) {
+    if (n_ == 1) return;
+    if (!maps_[(size_t)rank].ctx) registerTopo(rank, topo, ctx);  // every partition's first exchange
+    if (topo != topo_[(size_t)rank]) throw std::runtime_error("halo exchange of a vector on another topology");
+    const Maps &m = maps_[(size_t)rank];
+    // pack this partition's send list into its block
+    gip_check(gip_dcopy_indexed(ctx, m.nsend, m.send_indx, dvec, nullptr, d_send_ + send_off_[(size_t)rank]),
+              "halo pack");
+    gip_check(gip_sync(ctx), "halo pack");
+    barrier();
+    // gather the halo slots from the senders' blocks
+    gip_check(gip_dcopy_indexed(ctx, m.nrecv, m.src, d_send_, m.recv_indx, dvec), "halo unpack");
+    gip_check(gip_sync(ctx), "halo unpack");
+    barrier();  // nobody repacks before every partition has gathered
+}

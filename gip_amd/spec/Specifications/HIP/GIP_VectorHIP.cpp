@@ -6,6 +6,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "GIP_PartitionHIP.h"
 #include "gip_cavity.h"
 
 GIP_VectorHIP::GIP_VectorHIP(int n, GIP_Arch *arch) : GIP_Vector(n, arch) { allocDevice(); }
@@ -83,15 +84,22 @@ void GIP_VectorHIP::axpy(GIP_Vector *x, double alpha, double beta, enum RUN_DOMA
     gip_check(gip_daxpby(ctx(), covered(x, range(rdom)), alpha, x->getDevicePtr(), beta, dvec), "axpby");
 }
 
+// the partition's sum, combined over the partitions (MPI_Allreduce SUM in
+// GIP_VectorMulticore.cpp:125)
 double GIP_VectorHIP::dot(GIP_Vector *x, GIP_Vector *, enum RUN_DOMAIN rdom) {
     double r = 0.0;
     gip_check(gip_ddot(ctx(), covered(x, range(rdom)), dvec, x->getDevicePtr(), &r), "dot");
+    GIP_ArchHIP *node = GIP_ArchHIP::of(myArch);
+    if (GIP_PartitionGroup *g = node->partition()) r = g->allreduce(node->partRank(), r, GIP_PartitionGroup::SUM);
     return r;
 }
 
+// (MPI_Allreduce MAX in GIP_VectorMulticore.cpp:161)
 double GIP_VectorHIP::norm(GIP_Vector *, enum RUN_DOMAIN rdom) {
     double r = 0.0;
     gip_check(gip_dnrm_inf(ctx(), range(rdom), dvec, &r), "norm");
+    GIP_ArchHIP *node = GIP_ArchHIP::of(myArch);
+    if (GIP_PartitionGroup *g = node->partition()) r = g->allreduce(node->partRank(), r, GIP_PartitionGroup::MAX);
     return r;
 }
 
@@ -100,8 +108,14 @@ void GIP_VectorHIP::copyTo(GIP_Vector *x, enum RUN_DOMAIN rdom) {
               "copyTo");
 }
 
+// the halo refresh of GIP_VectorMulticore.cpp:197-216 (GIP_Topo.cpp:175-181):
+// device pack and unpack kernels among the partitions of one device; a
+// lone partition has no halo
 void GIP_VectorHIP::update() {
-    // one partition per device: no halo to refresh (GIP_Topo has none)
+    GIP_ArchHIP *node = GIP_ArchHIP::of(myArch);
+    GIP_PartitionGroup *g = node->partition();
+    if (!g || !myTopo) return;
+    g->exchange(node->partRank(), myTopo, ctx(), dvec);
 }
 
 void GIP_VectorHIP::operator=(int value) { gip_check(gip_dfill(ctx(), size, dvec, (double)value), "fill"); }

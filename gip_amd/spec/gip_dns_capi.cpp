@@ -5,8 +5,11 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "GIP_DNS.h"
+#include "GIP_PartitionHIP.h"
 #include "HIP.h"
 #include "gip_dns.h"
 #include "gip_markers.h"
@@ -168,6 +171,120 @@ int gip_dns_phase_ms(gip_dns *s, double *ms) {
 }
 
 int gip_dns_close(gip_dns *s) {
+    delete s;
+    return GIP_OK;
+}
+
+}  // extern "C"
+
+// ---- several partitions on one device -----------------------------------------
+
+struct gip_dns_parts {
+    int n, device;
+    std::unique_ptr<GIP_PartitionGroup> group;  // destroyed after the partitions
+    std::vector<std::unique_ptr<HipDNS>> sims;
+    ~gip_dns_parts() {
+        sims.clear();
+        group.reset();
+    }
+    // fn(rank) on one host thread per partition (device current on each);
+    // the first failure aborts the group so that no partition waits forever
+    int run(const std::function<void(int)> &fn) {
+        std::vector<std::string> err((size_t)n);
+        std::vector<std::thread> th;
+        for (int r = 0; r < n; ++r)
+            th.emplace_back([&, r] {
+                try {
+                    gip_check(gip_use_device(device), "gip_use_device");
+                    fn(r);
+                } catch (const std::exception &e) {
+                    err[(size_t)r] = e.what();
+                    group->abort();
+                }
+            });
+        for (auto &t : th) t.join();
+        for (int r = 0; r < n; ++r)
+            if (!err[(size_t)r].empty()) return fail(GIP_EINVAL, "partition " + std::to_string(r) + ": " + err[(size_t)r]);
+        return GIP_OK;
+    }
+};
+
+extern "C" {
+
+int gip_dns_open_parts(const char *root, const char *name, int32_t nparts, const gip_dns_config *cfg,
+                       gip_dns_parts **out) {
+    if (!name || !out || nparts < 1) return fail(GIP_EINVAL, "name, out and a positive partition count required");
+    *out = nullptr;
+    try {
+        std::unique_ptr<gip_dns_parts> s(new gip_dns_parts());
+        s->n = nparts;
+        s->device = cfg ? cfg->device : 0;
+        s->group.reset(new GIP_PartitionGroup(nparts, s->device));
+        for (int r = 0; r < nparts; ++r) s->sims.emplace_back(new HipDNS(name));
+        const GIP_Parameters prm = params_of(cfg);
+        int rc = s->run([&](int r) {
+            HipDNS &d = *s->sims[(size_t)r];
+            d.node().setPartition(s->group.get(), r);
+            d.setRoot(root ? root : "");
+            d.configure(prm);
+            d.setUp();
+        });
+        if (rc) return rc;
+        *out = s.release();
+        return GIP_OK;
+    } catch (const std::exception &e) {
+        return fail(GIP_EINVAL, e.what());
+    }
+}
+
+int gip_dns_parts_step(gip_dns_parts *s, gip_dns_step_info *info) {
+    if (!s) return fail(GIP_EINVAL, "null simulation");
+    GIP_StepInfo st{};
+    int rc = s->run([&](int r) {
+        GIP_StepInfo mine{};
+        s->sims[(size_t)r]->step(&mine);
+        if (r == 0) st = mine;  // time, dt and massr are global: every rank holds them
+    });
+    if (rc) return rc;
+    if (info) {
+        info->time = st.time;
+        info->dt = st.dt;
+        info->massr = st.massr;
+        info->pcg_relres = st.pcg_relres;
+        info->step = st.step;
+        info->pcg_iters = st.pcg_iters;
+        info->pcg_status = st.pcg_status;
+        info->pcg_corrections = st.pcg_corrections;
+    }
+    return GIP_OK;
+}
+
+int gip_dns_parts_sizes(gip_dns_parts *s, int32_t rank, int32_t *inner, int32_t *owned, int32_t *all,
+                        int32_t *faces) {
+    if (!s || rank < 0 || rank >= s->n) return fail(GIP_EINVAL, "bad simulation or rank");
+    HipDNS &d = *s->sims[(size_t)rank];
+    if (inner) *inner = d.innerSize();
+    if (owned) *owned = d.ownedSize();
+    if (all) *all = d.allSize();
+    if (faces) *faces = d.facesSize();
+    return GIP_OK;
+}
+
+int gip_dns_parts_get_field(gip_dns_parts *s, int32_t rank, const char *field, double *host, int32_t n) {
+    if (!s || rank < 0 || rank >= s->n || !field || !host) return fail(GIP_EINVAL, "bad argument");
+    try {
+        HipDNS &d = *s->sims[(size_t)rank];
+        GIP_VectorHIP &v = d.field(field);
+        if (n != v.getSize()) return fail(GIP_EINVAL, "field length mismatch");
+        if (int rc = gip_use_device(d.node().device())) return fail(rc, gip_last_error());
+        std::memcpy(host, v.getHostPtr(), sizeof(double) * (size_t)n);
+        return GIP_OK;
+    } catch (const std::exception &e) {
+        return fail(GIP_EINVAL, e.what());
+    }
+}
+
+int gip_dns_parts_close(gip_dns_parts *s) {
     delete s;
     return GIP_OK;
 }

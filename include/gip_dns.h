@@ -41,7 +41,7 @@ typedef struct gip_dns_step_info {
     int32_t step, pcg_iters, pcg_status, pcg_corrections;
 } gip_dns_step_info;
 
-/* Case from <root>/InputFiles/... (<name>_1p-0 files). */
+/* One-partition case from <root>/InputFiles/... (<name>_1p-0 files). */
 int gip_dns_open(const char *root, const char *name, const gip_dns_config *cfg, gip_dns **out);
 /* Case from an in-memory cavity (gip_cavity_generate / gip_cavity_read). */
 int gip_dns_from_cavity(const gip_cavity *cav, const gip_dns_config *cfg, gip_dns **out);
@@ -60,6 +60,25 @@ int gip_dns_sizes(gip_dns *s, int32_t *inner, int32_t *all, int32_t *faces);
 int gip_dns_sync(gip_dns *s);
 int gip_dns_close(gip_dns *s);
 const char *gip_dns_last_error(void);
+
+/* Several partitions of one case (<name>_<nparts>p-<rank> files, the
+ * decomposition the reference's driver runs with one MPI rank per partition,
+ * DrivenCavity.cpp:30-33) resident on one device, each stepped by its own
+ * host thread with its own context and stream.  Halo refreshes
+ * (GIP_Topo.cpp:124-181) are device pack/unpack kernels among the
+ * partitions; dot products, norms and the CFL bound are combined over the
+ * partitions in rank order (MPI_Allreduce in the reference).  pcg_mode must
+ * be 0 (the reference sequence).  A step returns rank 0's numbers (time,
+ * dt and massr are global); fields are per partition (inner cells, boundary
+ * ghosts, halo). */
+typedef struct gip_dns_parts gip_dns_parts;
+int gip_dns_open_parts(const char *root, const char *name, int32_t nparts, const gip_dns_config *cfg,
+                       gip_dns_parts **out);
+int gip_dns_parts_step(gip_dns_parts *s, gip_dns_step_info *info);
+int gip_dns_parts_sizes(gip_dns_parts *s, int32_t rank, int32_t *inner, int32_t *owned, int32_t *all,
+                        int32_t *faces);
+int gip_dns_parts_get_field(gip_dns_parts *s, int32_t rank, const char *field, double *host, int32_t n);
+int gip_dns_parts_close(gip_dns_parts *s);
 
 #ifdef __cplusplus
 }

@@ -52,6 +52,9 @@ const char *gip_last_error(void);
 int gip_init(int device, gip_ctx **ctx);
 int gip_finalize(gip_ctx *ctx);
 int gip_sync(gip_ctx *ctx);
+/* Make `device` the calling host thread's current device (threads that
+ * drive contexts they did not create, e.g. one thread per partition). */
+int gip_use_device(int device);
 void *gip_stream(gip_ctx *ctx); /* the hipStream_t the context launches on */
 int gip_device_count(int *count);
 
@@ -93,6 +96,15 @@ int gip_mat_create(gip_ctx *ctx, int32_t nrows, int32_t ncols, int32_t nnz, cons
 int gip_mat_destroy(gip_mat *A);
 /* nrows, nnz, chosen format, stored (padded) entries */
 int gip_mat_info(const gip_mat *A, int32_t *nrows, int32_t *nnz, int32_t *format, int64_t *stored);
+/* Bytes one product y = A x must move in the operator's chosen layout
+ * (fp32 != 0: the FP32 values of a GIP_MAT_FP32 operator): the stored
+ * values with their per-entry or per-slot indices and per-slice metadata,
+ * the input vector once over its columns, the output once over the rows.
+ * For CSR that is the CSR count 12 nnz + 4 (rows+1) + 8 rows + 8 cols; the
+ * stencil layout stores no column per entry (one offset per slot of 64
+ * rows), so it moves fewer bytes than the CSR count of the same operator.
+ * No counterpart in the reference: the unit of a layout's roofline. */
+int gip_mat_product_bytes(const gip_mat *A, int32_t fp32, int64_t *bytes);
 /* Largest column index + 1: the length an input vector of a product must have. */
 int gip_mat_cols_needed(const gip_mat *A, int32_t *ncols);
 /* CSR operators: lanes per row (1, 2, 4, 8 = 64, 32, 16, 8 rows per
@@ -146,6 +158,11 @@ int gip_dfill(gip_ctx *ctx, int32_t n, double *x, double value);                
 int gip_daxpy(gip_ctx *ctx, int32_t n, double alpha, const double *x, double *y);      /* y += alpha x  */
 int gip_daxpby(gip_ctx *ctx, int32_t n, double alpha, const double *x, double beta, double *y); /* y = ax+by */
 int gip_dcopy(gip_ctx *ctx, int32_t n, const double *x, double *y);                  /* y = x, copyTo (GIP_VectorMulticore.cpp:167-181) */
+/* y[dst ? dst[i] : i] = x[src ? src[i] : i] for i < n: the halo pack
+ * (send list -> send buffer) and unpack (receive buffer -> halo slots) of
+ * GIP_Topo.cpp:101-122, done on the device (the CUDA specification stages
+ * them on the host, GIP_VectorCuda.cpp:224-247).  Indices are trusted. */
+int gip_dcopy_indexed(gip_ctx *ctx, int32_t n, const int32_t *src, const double *x, const int32_t *dst, double *y);
 int gip_ddot(gip_ctx *ctx, int32_t n, const double *x, const double *y, double *result);         /* host result */
 int gip_dnrm_inf(gip_ctx *ctx, int32_t n, const double *x, double *result);                    /* max |x_i|  */
 int gip_cfl(gip_ctx *ctx, int32_t n, const double *u, const double *v, const double *w, const double *dxs,
@@ -153,6 +170,21 @@ int gip_cfl(gip_ctx *ctx, int32_t n, const double *u, const double *v, const dou
 /* Same reductions with the result left in device memory (no host wait). */
 int gip_ddot_dev(gip_ctx *ctx, int32_t n, const double *x, const double *y, double *dresult);
 int gip_dnrm_inf_dev(gip_ctx *ctx, int32_t n, const double *x, double *dresult);
+
+/* ---- streaming calibration -----------------------------------------------
+ * No counterpart in the reference: the practical HBM ceiling the
+ * bandwidth-bound kernels are measured against.  Times `reps` launches of
+ * one access mix over n doubles (n even; a, b, c device arrays of n
+ * doubles, b and c unused where the mix needs fewer): GIP_STREAM_COPY
+ * (b = a), GIP_STREAM_READ (sum of a; one partial per workgroup into part,
+ * part_len slots), GIP_STREAM_WRITE (a = 1), GIP_STREAM_MIX (three reads,
+ * two writes: c += 0.5 a, a = b + 0.25 a).  vec: 1 (8-byte) or 2 (16-byte)
+ * accesses; nontemporal: 1 for nontemporal loads and stores; grid: 0 for
+ * one access per thread over the array, else that many workgroups striding
+ * over it.  *ms receives the mean time of one launch. */
+enum { GIP_STREAM_COPY = 0, GIP_STREAM_READ = 1, GIP_STREAM_WRITE = 2, GIP_STREAM_MIX = 3 };
+int gip_stream_probe(gip_ctx *ctx, int32_t kind, int32_t vec, int32_t nontemporal, int32_t grid, size_t n, double *a,
+                     double *b, double *c, double *part, size_t part_len, int32_t reps, float *ms);
 
 /* ---- fused Jacobi-PCG ----------------------------------------------------
  * GIP_LinearSolver::solve (GIP_LinearSolver.h:45-117) with every scalar

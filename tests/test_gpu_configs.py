@@ -68,6 +68,23 @@ def _true_rel(L, x, b, n, nall):
     return np.linalg.norm(res) / np.linalg.norm(b[:n])
 
 
+def _errors(got, want, n_inner):
+    """Errors of u, v, w against the norm of the whole velocity field, and of
+    p's mean-free part over the inner cells against its own norm.  Solves
+    stopped at the same residual differ by an e with ||L e|| <= 2 tol ||b||:
+    after a few steps from rest that shows in v and w (nothing but -dt G p
+    yet, so of tiny norm) at a relative size far above tol, even between two
+    FP64 solves; and the constant component of p, the near-null mode of the
+    pinned Laplacian (eigenvalue ~ pin / N), is fixed by the residual only
+    loosely, while no velocity depends on it (G 1 = 0)."""
+    vel = np.sqrt(sum(np.linalg.norm(want[f]) ** 2 for f in ("u", "v", "w")))
+    out = {f: np.linalg.norm(got[f] - want[f]) / vel for f in ("u", "v", "w")}
+    g, w = got["p"][:n_inner], want["p"][:n_inner]
+    g, w = g - g.mean(), w - w.mean()
+    out["p"] = np.linalg.norm(g - w) / np.linalg.norm(w)
+    return out
+
+
 def _steps_vs_oracle(sim, orc, steps, tag):
     for k in range(steps):
         g, w = sim.step(), orc.step()
@@ -140,7 +157,11 @@ def test_clustered128_ns_tolerance_step_fp64_and_mixed_vs_oracle(dev, clu128):
     """One step with the pressure solved to 1e-10, as in the configuration:
     the FP64 device solve stops within one iteration of the oracle's; the
     mixed-precision run's error per field against the FP64 oracle is at the
-    solver tolerance, not at FP32 round-off."""
+    solver tolerance, not at FP32 round-off.
+
+    The bound is on the errors of _errors (velocities against the whole
+    velocity field, p mean-free); the per-field relative errors are printed
+    as well."""
     orc = pyoracle.Dns(cavity=clu128, gamma=1e-3, pcg_maxit=20000, pcg_tol=1e-10)
     w = orc.step()
     want = {f: orc.field(f) for f in ("u", "v", "w", "p")}
@@ -155,10 +176,11 @@ def test_clustered128_ns_tolerance_step_fp64_and_mixed_vs_oracle(dev, clu128):
             assert _rel(g[key], w[key]) <= REL
         if mode == "fused":
             assert abs(g["pcg_iters"] - w["pcg_iters"]) <= 1
-        for f in ("u", "v", "w", "p"):
-            err = _field_rel(sim.field(f), want[f])
-            print(f"[128^3 clustered tol 1e-10 {mode} field {f}] rel L2 vs FP64 oracle {err:.3e}", flush=True)
-            assert err <= 1e-7, (mode, f, err)
+        got = {f: sim.field(f) for f in ("u", "v", "w", "p")}
+        for f, bound_err in _errors(got, want, clu128.n_inner).items():
+            print(f"[128^3 clustered tol 1e-10 {mode} field {f}] rel L2 vs FP64 oracle "
+                  f"{_field_rel(got[f], want[f]):.3e}, bounded: {bound_err:.3e}", flush=True)
+            assert bound_err <= 1e-7, (mode, f, bound_err)
         sim.close()
 
 
@@ -232,8 +254,8 @@ def test_batch64_uneven_vs_single_cavity_and_oracle(dev):
         for _ in range(steps):
             s = sim.step()
         assert _rel(info[i]["time"], s["time"]) <= 1e-8, (i, info[i]["time"], s["time"])
-        for f in ("u", "v", "w", "p"):
-            err = _field_rel(got[i][f], sim.field(f))
+        want = {f: sim.field(f) for f in ("u", "v", "w", "p")}
+        for f, err in _errors(got[i], want, c.n_inner).items():
             worst = max(worst, err)
             assert err <= 1e-6, (i, sizes[i], f, err)
         sim.close()
@@ -246,9 +268,10 @@ def test_batch64_uneven_vs_single_cavity_and_oracle(dev):
         for _ in range(steps):
             w = orc.step()
         assert _rel(info[i]["time"], w["time"]) <= 1e-8
-        for f in ("u", "v", "w", "p"):
-            err = _field_rel(got[i][f], orc.field(f))
-            print(f"[batch64 inst {i} n={sizes[i]} Re={re[i]:.0f} {f}] rel L2 vs FP64 oracle {err:.3e}", flush=True)
+        want = {f: orc.field(f) for f in ("u", "v", "w", "p")}
+        for f, err in _errors(got[i], want, cavs[i].n_inner).items():
+            print(f"[batch64 inst {i} n={sizes[i]} Re={re[i]:.0f} {f}] vs FP64 oracle {err:.3e} "
+                  f"(own norm {_field_rel(got[i][f], want[f]):.3e})", flush=True)
             assert err <= 1e-6, (i, f, err)
         orc.close()
 
