@@ -28,12 +28,17 @@ Other suites (one JSON line each, not the default):
     --suite clustered 128^3 tanh-clustered mesh (beta 1.5), Re=1000, 100 steps,
                       PCG to 1e-10: steps/s FP64 vs mixed, the L SpMV rate and
                       the FP64/mixed PCG on a zero-mean N(0,1) right-hand side
+    --suite ns256     the 256^3 Navier-Stokes steps (Re=1000, pressure to 1e-10)
+                      in mixed precision and FP64 on the GPU against the FP64
+                      oracle, error per field (default 2 steps)
     --suite cpu       the CPU-baseline plan: the reference integrator at 1 and 4
                       ranks, as shipped (-O0) and -O3, on the generated 46^3
                       cavity and C100K, beside the GPU with 1 and 4 partitions
     --probe spmv      a few launches of the FP64 SpMV, eight iterations of each
                       PCG, and a plain copy of known size, for rocprofv3 --pmc
                       traffic runs
+    --probe barrier   the cost of a device-wide barrier (cooperative launch)
+                      among 8 to 1024 workgroups, relaxed and fenced
     --probe stream    the practical HBM ceiling: copy, read, write and the PCG
                       direction sweep's mix (3 reads, 2 writes) over 2 GiB
                       arrays, in 8- and 16-byte, plain and nontemporal
@@ -223,24 +228,35 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=60)
     ap.add_argument("--skip-ns", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--suite", choices=("pcg", "batch", "adjoint", "clustered", "reference", "cpu"), default="pcg")
-    ap.add_argument("--probe", choices=("spmv", "stream"), default=None)
-    ap.add_argument("--summarize", choices=("traffic", "stream"), default=None,
+    ap.add_argument("--suite", choices=("pcg", "batch", "adjoint", "clustered", "reference", "cpu", "ns256"),
+                    default="pcg")
+    ap.add_argument("--batch-eager", action="store_true",
+                    help="--suite batch: launch every kernel from the host instead of replaying captured graphs")
+    ap.add_argument("--probe", choices=("spmv", "stream", "barrier"), default=None)
+    ap.add_argument("--summarize", choices=("traffic", "stream", "kernels", "barrier"), default=None,
                     help="write a profiles/ summary from rocprofv3 --pmc outputs or a --probe stream JSON line")
     ap.add_argument("--fetch", default=None, help="--summarize traffic: the FETCH_SIZE run's output directory")
     ap.add_argument("--write", default=None, help="--summarize traffic: the WRITE_SIZE run's output directory")
-    ap.add_argument("--json", default=None, help="--summarize stream: the probe's JSON output")
+    ap.add_argument("--json", default=None, help="--summarize stream/barrier: the probe's JSON output")
+    ap.add_argument("--db", default=None, help="--summarize kernels: a rocprofv3 --kernel-trace results database")
+    ap.add_argument("--title", default="", help="--summarize kernels: the workload and the command that profiled it")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build runs one partition on one device "
+        print(f"bench.py measures one GPU: every partition of a case runs on one device "
               f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
         sys.exit(2)
     if args.summarize == "traffic":
         return summary_traffic(args)
     if args.summarize == "stream":
         return summary_stream(args)
+    if args.summarize == "kernels":
+        return summary_kernels(args)
+    if args.summarize == "barrier":
+        return summary_barrier(args)
     if args.probe == "stream":
         return probe_stream(args)
+    if args.probe == "barrier":
+        return probe_barrier(args)
     if args.probe:
         return probe_spmv(args)
     if args.suite == "batch":
@@ -253,6 +269,8 @@ def main():
         return reference_suite(args)
     if args.suite == "cpu":
         return cpu_suite(args)
+    if args.suite == "ns256":
+        return ns256_suite(args)
 
     from gip_amd import cavity, dns, hip
 
@@ -343,23 +361,25 @@ def main():
     # achieved = the CSR algorithmic bytes 12 nnz + 4 (rows+1) + 8 rows +
     # 8 cols per launch; the layout's own bytes beside it.
     us64, us32 = fp64["product_us"], mixed["product_us"]
+    # operators of at most 2^19 rows form the direction inside the product
+    k64, k32 = ("k_iter_dirprod", "k_m_dirprod") if info["nrows"] <= 1 << 19 else ("k_iter_spmv", "k_m_spmv")
     ceiling = stream_ceiling()
     ach64 = alg / (us64 * 1e-6) / 1e9
     roofline = {
         "bound": "hbm", "achieved": round(ach64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(ach64 / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n, "k_iter_spmv"),
+        "frac": round(ach64 / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n, k64),
         "algorithmic_bytes": alg, "us_per_launch": round(us64, 2),
         "layout_bytes": lay64, "frac_layout": round(lay64 / (us64 * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
-        "kernel": f"k_iter_spmv (FP64 stencil-layout product of the FP64 PCG, timed inside the solves), {n}^3"}
+        "kernel": f"{k64} (FP64 stencil-layout product of the FP64 PCG, timed inside the solves), {n}^3"}
     # The dominant kernel of the timed mixed solve, its FP32 product: achieved
     # = the layout's bytes (the FP32 CSR count includes column indices this
     # layout never reads, and would put the rate above the peak).
     ach32 = lay32 / (us32 * 1e-6) / 1e9
     roofline_solve = {
         "bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n, "k_m_spmv"),
+        "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n, k32),
         "algorithmic_bytes": lay32, "us_per_launch": round(us32, 2), "csr_bytes": alg32,
-        "kernel": f"k_m_spmv (FP32 stencil-layout product of the mixed PCG, timed inside the solves), {n}^3"}
+        "kernel": f"{k32} (FP32 stencil-layout product of the mixed PCG, timed inside the solves), {n}^3"}
 
     out = {
         "metric": METRIC,
@@ -373,7 +393,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32-inner/f64-outer",
-        "data": "synthetic: generated 256^3 uniform cavity Laplacian, seeded N(0,1) zero-mean RHS",
+        "data": f"synthetic: generated {n}^3 uniform cavity Laplacian, seeded N(0,1) zero-mean RHS",
         "config": {"workload": f"{n}^3 cavity Poisson, Jacobi-PCG to 1e-10 FP64 relative true residual "
                                f"(mixed precision), one partition", "n": n, "nnz": info["nnz"],
                    "layout": hip.FORMAT_NAMES.get(info["format"], "?")},
@@ -434,8 +454,8 @@ def summarize_pmc(fetch_dir, write_dir, kernels):
     return out, f_scale, w_scale
 
 
-PCG_KERNELS = ("k_iter_dir", "k_iter_spmv", "k_fold_dn", "k_iter_update", "k_m_dir", "k_m_spmv", "k_m_update",
-               "k_stencil<0")
+PCG_KERNELS = ("k_iter_dir_rows", "k_iter_spmv", "k_fold_two", "k_alpha_fin", "k_iter_update_rows", "k_beta_fin",
+               "k_m_dir", "k_m_spmv", "k_fold_dn", "k_m_update", "k_stencil<0")
 
 
 def summary_traffic(args):
@@ -474,6 +494,58 @@ def summary_traffic(args):
     print("\n".join(lines))
 
 
+def summary_kernels(args):
+    """Per-kernel totals (calls, total/average/min/max duration, share) of a
+    rocprofv3 --kernel-trace run, from its results database."""
+    import sqlite3
+    con = sqlite3.connect(args.db)
+    rows = con.execute("select name, count(*), sum(duration), avg(duration), min(duration), max(duration) "
+                       "from kernels group by name order by sum(duration) desc").fetchall()
+    total = sum(r[2] for r in rows)
+    lines = [f"# {line}" for line in args.title.splitlines()] + [
+        "#",
+        f"# summary: python3 bench.py --summarize kernels --db {args.db}",
+        "# durations in microseconds (rocprofv3 kernel-trace timestamps)",
+        "#",
+        "# share   calls      total_us    avg_us    min_us    max_us  kernel"]
+    for name, calls, tot, avg, mn, mx in rows:
+        lines.append(f"{100 * tot / total:6.2f}% {calls:7d} {tot / 1e3:13.1f} {avg / 1e3:9.2f} {mn / 1e3:9.2f} "
+                     f"{mx / 1e3:9.2f}  {name}")
+    # roctx ranges (a --marker-trace run): host time per range message
+    ranges = {}
+    try:
+        for ext, dur in con.execute("select extdata, duration from regions where category like 'MARKER%'"):
+            msg = json.loads(ext or "{}").get("message", "?")
+            ranges.setdefault(msg, []).append(dur / 1e3)
+    except sqlite3.OperationalError:
+        pass
+    if ranges:
+        lines += ["#", "# roctx ranges (host time, microseconds)", "#",
+                  "# calls      total_us    avg_us    min_us    max_us  range"]
+        for msg, d in ranges.items():
+            lines.append(f"{len(d):7d} {sum(d):13.1f} {sum(d) / len(d):9.2f} {min(d):9.2f} {max(d):9.2f}  {msg}")
+    print("\n".join(lines))
+
+
+def summary_barrier(args):
+    """profiles/device_barrier_cost.txt from a `--probe barrier` JSON line."""
+    d = json.loads(pathlib.Path(args.json).read_text().strip().splitlines()[-1])
+    lines = [
+        "# Cost of one device-wide barrier on this MI355X (cooperative launch, every workgroup resident,",
+        "# 2000 barriers per measurement, HIP events): relaxed device-scope atomic counter; with a device-scope",
+        "# fence before arriving and after leaving (needed to publish data written before the barrier);",
+        "# fenced with one double per workgroup stored before and all of them summed after.",
+        "#",
+        "# commands (run from the repository root on one MI355X):",
+        "#   python3 bench.py --probe barrier > gpurun_out/barrier.json",
+        "#   python3 bench.py --summarize barrier --json gpurun_out/barrier.json",
+        "#",
+        "# workgroups   relaxed_us   fenced_us   fenced_exchange_us"]
+    for r in d["rows"]:
+        lines.append(f"#   {r['workgroups']:6d} {r['relaxed_us']:12.3f} {r['fenced_us']:11.3f} {r['fenced_exchange_us']:14.3f}")
+    print("\n".join(lines))
+
+
 def summary_stream(args):
     """profiles/stream_ceiling.txt from a `--probe stream` JSON line."""
     d = json.loads(pathlib.Path(args.json).read_text().strip().splitlines()[-1])
@@ -493,6 +565,22 @@ def summary_stream(args):
     for k, r in d["best"].items():
         lines.append(f"best_gbs[{k}]: {r['gbs']}")
     print("\n".join(lines))
+
+
+def probe_barrier(args):
+    """Microseconds per device-wide barrier (gip_barrier_probe) for 8 to
+    1024 resident workgroups: relaxed counter, with fences, with fences and
+    one value per workgroup exchanged."""
+    from gip_amd import hip
+    dev = hip.Device(0)
+    rows = []
+    for grid in (8, 64, 128, 256, 512, 1024):
+        r = {"workgroups": grid}
+        for mode, name in ((0, "relaxed_us"), (1, "fenced_us"), (2, "fenced_exchange_us")):
+            r[name] = round(hip.barrier_probe(dev, grid, 2000, mode), 3)
+        rows.append(r)
+        log(f"barrier, {grid:5d} workgroups: {r}")
+    print(json.dumps({"probe": "barrier", "rows": rows}), flush=True)
 
 
 def probe_stream(args):
@@ -572,14 +660,18 @@ def batch_suite(args):
     log(f"generated {B} cavities ({cells} cells) in {time.perf_counter() - t0:.1f} s")
     dev = hip.Device(0)
     bt = gbatch.Batch(dev, cavs, 1.0 / re, pcg_tol=1e-8, pcg_maxit=20000, precision=hip.PCG_MIXED, poll=16,
-                      use_graph=True, autotune=True)
+                      use_graph=not args.batch_eager, autotune=True)
     bt.step()  # warm-up: graph capture
     dev.sync()
+    phases = dict.fromkeys(bt.PHASES, 0.0)
     t0 = time.perf_counter()
     for _ in range(steps - 1):
         bt.step()
+        for k, v in bt.phase_ms().items():
+            phases[k] += v
     dev.sync()
     t_batch = time.perf_counter() - t0
+    log("batch phases, ms per step: " + ", ".join(f"{k} {v / (steps - 1):.2f}" for k, v in phases.items()))
     bt_u = [bt.field(i, "u") for i in range(B)]
     bt_info = [bt.info(i) for i in range(B)]
     bt.close()
@@ -607,6 +699,8 @@ def batch_suite(args):
         "speedup_vs_sequential": t_seq / t_batch, "max_rel_u_diff_vs_sequential": max_diff,
         "cells_total": cells, "sizes": sizes, "re": [round(float(v), 1) for v in re],
         "pcg_iters_last_step": [i["pcg_iters"] for i in bt_info],
+        "graph_replay": not args.batch_eager,
+        "phase_ms_per_step": {k: v / (steps - 1) for k, v in phases.items()},
         "data": "synthetic: generated uniform cavities, lid u=1, from rest"}), flush=True)
 
 
@@ -804,6 +898,69 @@ def cpu_suite(args):
                 "timer": "MPI_Wtime around integrate()"},
         "data": "generated 46^3 cavity (1 and 4 partitions) and the shipped C100K (4 partitions; 1 merged)"}),
         flush=True)
+
+
+def ns256_suite(args):
+    """The mixed-precision configuration's Navier-Stokes check: --ns-steps
+    steps (default here 2) of the 256^3 cavity at Re=1000 with the pressure
+    solved to 1e-10, on the GPU with the mixed-precision and the FP64
+    device PCG, and on the FP64 oracle (the C restatement of the reference
+    CPU path, its row loops on the host's cores and its dot products summed
+    in 4096-term blocks); errors per field against the oracle: u, v, w
+    against the norm of the whole velocity field and against their own, p
+    mean-free over the inner cells (its constant mode is the pinned
+    Laplacian's near-null mode, fixed only loosely by any solver stopped at
+    1e-10; no velocity depends on it)."""
+    from gip_amd import cavity, dns
+    sys.path.insert(0, str(ROOT))
+    from oracle import pyoracle
+    n, steps = args.n, args.ns_steps if args.ns_steps != 20 else 2
+    t0 = time.perf_counter()
+    cav = cavity.generate(n, gamma=1e-3)
+    N = cav.n_inner
+    log(f"generated {n}^3 cavity in {time.perf_counter() - t0:.1f} s")
+    out = {}
+    for mode in ("mixed", "fused"):
+        sim = dns.Simulation(cavity=cav, gamma=1e-3, pcg_mode=mode, pcg_tol=1e-10, pcg_maxit=20000)
+        rows = []
+        for k in range(steps):
+            g = sim.step()
+            rows.append(g)
+            log(f"gpu {mode} step {k + 1}: pcg {g['pcg_iters']} massr {g['massr']:.3e}")
+        out[mode] = {"steps": rows, "fields": {f: sim.field(f) for f in ("u", "v", "w", "p")}}
+        sim.close()
+    pyoracle.set_threads(min(16, len(os.sched_getaffinity(0))))
+    pyoracle.set_blocked_dot(True)
+    orc = pyoracle.Dns(cavity=cav, gamma=1e-3, pcg_maxit=20000, pcg_tol=1e-10)
+    orows = []
+    for k in range(steps):
+        t0 = time.perf_counter()
+        w = orc.step()
+        orows.append(w)
+        log(f"oracle step {k + 1}: pcg {w['pcg_iters']} massr {w['massr']:.3e} ({time.perf_counter() - t0:.0f} s)")
+    want = {f: orc.field(f) for f in ("u", "v", "w", "p")}
+    orc.close()
+    vel = float(np.sqrt(sum(np.linalg.norm(want[f]) ** 2 for f in ("u", "v", "w"))))
+    wp = want["p"][:N] - want["p"][:N].mean()
+    res = {}
+    for mode in ("mixed", "fused"):
+        got = out[mode]["fields"]
+        gp = got["p"][:N] - got["p"][:N].mean()
+        res[mode] = {
+            "vs_velocity_norm": {f: float(np.linalg.norm(got[f] - want[f]) / vel) for f in ("u", "v", "w")},
+            "own_norm": {f: float(np.linalg.norm(got[f] - want[f]) / np.linalg.norm(want[f])) for f in ("u", "v", "w")},
+            "p_mean_free": float(np.linalg.norm(gp - wp) / np.linalg.norm(wp)),
+            "time_rel": abs(out[mode]["steps"][-1]["time"] - orows[-1]["time"]) / orows[-1]["time"],
+            "pcg_iters": [r["pcg_iters"] for r in out[mode]["steps"]]}
+        log(f"{mode} vs FP64 oracle: {res[mode]}")
+    print(json.dumps({
+        "metric": f"{n}^3 cavity, Re=1000, pressure to 1e-10: fields after {steps} steps against the FP64 oracle",
+        "value": res["mixed"]["p_mean_free"], "unit": "relative L2 (p, mean-free)", "n_gpus": 1,
+        "higher_is_better": False, "steps": steps, "errors": res,
+        "oracle_pcg_iters": [r["pcg_iters"] for r in orows],
+        "oracle": "C restatement of the reference CPU path; row loops threaded (bitwise the serial ones), dot "
+                  "products in 4096-term blocks",
+        "data": f"synthetic: generated {n}^3 uniform cavity, lid u=1, from rest"}), flush=True)
 
 
 def adjoint_suite(args):

@@ -28,6 +28,7 @@ def _lib():
         L.gip_batch_instance.argtypes = [c_void_p, c_int32, POINTER(BatchInfo)]
         L.gip_batch_get_field.argtypes = [c_void_p, c_int32, ctypes.c_char_p, c_void_p, c_int32]
         L.gip_batch_layouts.argtypes = [c_void_p, POINTER(c_int32), POINTER(c_int32)]
+        L.gip_batch_phase_ms.argtypes = [c_void_p, POINTER(ctypes.c_float)]
         L.gip_batch_destroy.argtypes = [c_void_p]
         L._gip_batch_typed = True
     return L
@@ -51,6 +52,17 @@ class Batch:
 
     def step(self):
         hip.check(_lib().gip_batch_step(self.h))
+
+    PHASES = ("predictor", "pcg", "post")
+
+    def phase_ms(self):
+        """Device time (ms) of the last step's phases, keyed by PHASES: the
+        predictor with the Poisson right-hand side, the PCG iterations (with
+        the host's reads between poll groups), and correction + mass fluxes
+        + CFL."""
+        out = (ctypes.c_float * 3)()
+        hip.check(_lib().gip_batch_phase_ms(self.h, out))
+        return dict(zip(self.PHASES, [float(x) for x in out]))
 
     def info(self, i):
         s = BatchInfo()

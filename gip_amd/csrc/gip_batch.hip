@@ -610,6 +610,9 @@ struct gip_batch {
     int *state = nullptr, *iters = nullptr, *active = nullptr, *h_active = nullptr;
     bool started = false;
     hipGraphExec_t g_pre = nullptr, g_iter = nullptr, g_post = nullptr;
+    // events around the step's three phases (recorded outside any capture)
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    float phase_ms[3] = {0.0f, 0.0f, 0.0f};
     int grid_in = 0, grid_f = 0, grid_all = 0;
     std::vector<void *> allocs;
 };
@@ -965,6 +968,8 @@ int gip_batch_create(gip_ctx *c, int32_t ninst, const gip_cavity *const *cavs, c
     if (!rc) {
         hipError_t e = hipHostMalloc((void **)&b->h_active, sizeof(int), hipHostMallocDefault);
         if (e != hipSuccess) rc = hip_fail(e, "hipHostMalloc");
+        for (int k = 0; k < 4 && e == hipSuccess; ++k) e = hipEventCreate(&b->ev[k]);
+        if (e != hipSuccess && !rc) rc = hip_fail(e, "hipEventCreate");
     }
     // fields from the cavities: cells (inner + ghosts), gamma
     if (!rc) {
@@ -1069,7 +1074,9 @@ int gip_batch_step(gip_batch *b) {
         GIP_LAUNCH_CHECK("batch initial CFL");
         b->started = true;
     }
+    GIP_HIP_TRY(hipEventRecord(b->ev[0], st));
     if ((rc = run_phase(b, enqueue_pre, &b->g_pre))) return rc;
+    GIP_HIP_TRY(hipEventRecord(b->ev[1], st));
     const int max_groups = (b->o.pcg_maxit + b->o.poll - 1) / b->o.poll + 2;
     for (int g = 0; g < max_groups; ++g) {
         if ((rc = run_phase(b, enqueue_iter, &b->g_iter))) return rc;
@@ -1077,8 +1084,17 @@ int gip_batch_step(gip_batch *b) {
         GIP_HIP_TRY(hipStreamSynchronize(st));
         if (*b->h_active <= 0) break;
     }
+    GIP_HIP_TRY(hipEventRecord(b->ev[2], st));
     if ((rc = run_phase(b, enqueue_post, &b->g_post))) return rc;
+    GIP_HIP_TRY(hipEventRecord(b->ev[3], st));
     GIP_HIP_TRY(hipStreamSynchronize(st));
+    for (int k = 0; k < 3; ++k) GIP_HIP_TRY(hipEventElapsedTime(&b->phase_ms[k], b->ev[k], b->ev[k + 1]));
+    return GIP_OK;
+}
+
+int gip_batch_phase_ms(gip_batch *b, float *ms) {
+    if (!b || !ms) return set_error(GIP_EINVAL, "gip_batch_phase_ms: bad argument");
+    for (int k = 0; k < 3; ++k) ms[k] = b->phase_ms[k];
     return GIP_OK;
 }
 
@@ -1144,6 +1160,8 @@ int gip_batch_destroy(gip_batch *b) {
     if (b->C) gip_mat_destroy(b->C);
     for (void *p : b->allocs) hipFree(p);
     if (b->h_active) hipHostFree(b->h_active);
+    for (hipEvent_t e : b->ev)
+        if (e) hipEventDestroy(e);
     delete b;
     return GIP_OK;
 }

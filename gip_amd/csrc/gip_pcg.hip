@@ -1,39 +1,35 @@
 // Device-resident Jacobi-PCG of libgip_hip (GIP_LinearSolver.h:45-117).
 //
-// FP64 iteration k, three kernels, the reference recurrences and roundings:
-//   K0  rho_k = sum of the r.s partials; convergence test ||r|| <= tol ||b||;
-//       x += alpha_{k-1} p (the previous iteration's update, :95, moved here
-//       so that p is read once per iteration); beta = rho_k / rho_{k-1};
-//       p = 1.0 s + beta p on the solved rows (GIP_LinearSolver.h:107-113;
-//       iteration 0 uses p from the setup; the last iteration's x update is
-//       applied by a flush kernel)
-//   K1  Ap = A p, partials of p.Ap                             (:87-91)
-//       (one row per thread over the whole matrix, one partial per 256-row
-//       chunk; beyond 2048 chunks a small kernel folds each 256 of them)
-//   K2  dn = sum of the p.Ap partials, alpha = rho_k / dn;
-//       r += (-alpha) Ap ; s = Dinv r ; partials of r.s, r.r
-// Cross-workgroup communication is fence-free: every workgroup stores its
-// partial sums with plain stores, and every workgroup of the NEXT kernel
-// re-reduces all partials in its prologue in the same fixed order, so all
-// workgroups compute bit-identical scalars.  The kernel boundary orders the
-// stores before the loads; no atomics, no device-scope fences (a fence per
-// workgroup costs an L2 write-back on every XCD).  Scalars that must survive
-// a kernel (rho and alpha of the last two iterations, the state word) are
-// written by workgroup 0 only: rho and alpha into the slots of their
-// iteration's parity, which no workgroup of the same kernel reads (a later
-// kernel that needs alpha reads its slot instead of re-reducing the p.Ap
-// partials); the state word as described at running().
-// The direction update has its own streaming kernel so that the product
-// gathers one vector (p) per neighbour; forming p inside the product would
-// gather s and the old p for every neighbour.  Every kernel reads the
-// solver's state word and returns when the solve has finished, so the host
-// queues many iterations and polls the state only now and then.  The
-// streaming kernels request the state word, the partials and their first
-// row's operands together, so that a small operator's kernel costs one
-// memory round trip before it stores.
+// FP64 iteration k, the reference recurrences and roundings
+// (GIP_LinearSolver.h:85-116):
+//   direction  x += alpha_{k-1} p (the previous iteration's update, :95,
+//              moved here so that p is read once per iteration);
+//              beta = rho_k / rho_{k-1}; p = 1.0 s + beta p   (:107-113)
+//   product    Ap = A p, partials of p.Ap                     (:87-91)
+//   update     alpha = rho_k / p.Ap; r += (-alpha) Ap; s = Dinv r;
+//              partials of r.s and r.r                        (:93-105)
+// Operators of at most 2^19 rows (2048 chunks of 256 rows) take two
+// kernels per iteration: the direction is formed inside the product, and
+// every workgroup of a kernel re-reduces the previous kernel's partials in
+// its prologue (fence-free: plain stores, ordered by the kernel boundary;
+// every workgroup computes bit-identical scalars), so a kernel costs about
+// one memory round trip before it stores.  Longer operators sweep one row
+// per thread in dispatch order (the pace of a plain stream of each kernel's
+// access mix) and take their scalars from one-workgroup kernels (fold +
+// finalize) instead, since that many workgroups cannot each re-reduce the
+// partials; see the long-path section below.  Scalars that must survive a
+// kernel (rho and alpha of the last two iterations, beta, the state word)
+// are written by one workgroup: rho and alpha into the slots of their
+// iteration's parity, which no workgroup of the same kernel reads.  Every
+// kernel reads the solver's state word and returns when the solve has
+// finished, so the host queues many iterations and polls the state only now
+// and then.
 //
-// MIXED: the same three kernels in FP32 (operator values, vectors and SpMV;
-// dot products accumulate in FP64) on the correction equation, inside an
+// MIXED: the same iteration in FP32 (operator values, vectors and SpMV;
+// dot products accumulate in FP64; on long operators three kernels whose
+// grid-stride sweeps re-reduce the partials in their prologues: the
+// row-per-thread form measured 18% slower per FP32 iteration at 256^3)
+// on the correction equation, inside an
 // FP64 defect-correction loop: when the FP32 residual has dropped by
 // inner_tol (or reaches the target) the FP32 correction is added to the
 // FP64 x, the FP64 true residual b - A x replaces the FP32 residual (the
@@ -72,7 +68,7 @@ enum {
     S_ZAZ = 8,      // 1.A1 over the solved rows (MIXED coarse step)
     S_ALPHA0 = 10,  // alpha of even iterations, as the update kernel formed it
     S_ALPHA1 = 11,  // alpha of odd iterations
-    S_BETA = 12,    // beta of the next direction update (long FP64 operators)
+    S_BETA = 12,    // beta of the next direction update (FP64, long operators)
     S_COUNT = 16
 };
 // State word: ST_RUN, ST_BREAK, or ST_CONV; the direction kernel of
@@ -315,49 +311,25 @@ struct SelfOperand {
 // partial arrays are summed one at a time, which keeps the register count,
 // and with it the occupancy of the sweep, low.
 
-// K0 (k >= 1): rho_k, the convergence test, x += alpha_{k-1} p,
-// beta, p = 1.0 s + beta p
-__global__ __launch_bounds__(kBlock) void k_iter_dir(PcgDev d, double *__restrict__ p, int k) {
-    double *__restrict__ x = d.x;
-    const double *__restrict__ s = d.s;
-    const int st = state_load(d.state);
-    if (st != ST_RUN && st != conv_stamp(k)) return;
-    const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);  // :107
-    const double alpha = alpha_slot(d, k - 1);
-    const double tol2 = d.sc[S_TOL2];
-    if (tol2 >= 0.0) {
-        const double rr = sum_partials<RED_SUM>(d.P(P_RR), d.G);
-        if (rr <= tol2) {  // every workgroup reaches the same decision
-            x_update(d, p, alpha);
-            if (leader()) state_store(d.state, conv_stamp(k));
-            return;
-        }
-    }
-    const double beta = rho / rho_slot(d, k - 1);  // :109
-    if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
-    FOR_ROWS(d.n, row) {
-        const double xo = x[row], si = s[row], po = p[row];
-        x[row] = xo + alpha * po;       // :95 of iteration k-1
-        p[row] = 1.0 * si + beta * po;  // :113
-    }
-}
-
 // x += alpha_k p of the last iteration k
 __global__ __launch_bounds__(kBlock) void k_x_flush(PcgDev d, const double *__restrict__ p, int k) {
     if (!running(d)) return;
     x_update(d, p, alpha_slot(d, k));
 }
 
-// K1: Ap = A p (empty rows keep Ap), one partial of p.Ap per 256-row chunk;
-// one row per thread, workgroups in XCD-contiguous order.  SMALL: the state
-// word is tested after the row's loads instead of before them, taking its
-// latency off the kernel's critical path; a finished solve then still reads
-// the operator, but stores nothing.
+// K1: Ap = A p (empty rows keep Ap), p.Ap partials; one row per thread.
+// Long operators: workgroups in dispatch order, one partial per wavefront
+// (with the direction and update sweeps in dispatch order too, 2% faster
+// per FP64 iteration at 256^3 than the XCD-contiguous order, which keeps
+// the neighbour gathers in one XCD's L2).  SMALL: XCD-contiguous order,
+// one partial per chunk; the state word is tested after the row's loads
+// instead of before them, taking its latency off the kernel's critical
+// path; a finished solve then still reads the operator, but stores nothing.
 template <int W, bool SMALL>
 __device__ inline void iter_spmv(const EllView &A, const PcgDev &d, const double *__restrict__ p) {
     if constexpr (!SMALL) {
         if (!running(d)) return;
-        const int c = xcd_block(blockIdx.x, gridDim.x);
+        const int c = blockIdx.x;
         const int row = c * kBlock + threadIdx.x;
         double acc = 0.0;
         if (row < d.n) {
@@ -527,47 +499,15 @@ __global__ __launch_bounds__(kBlock) void k_fold_dn(PcgDev d) {
     }
 }
 
-// K2: alpha_k, r += (-alpha) Ap, s = Dinv r, partials of r.s and r.r; each
-// row's operands are loaded together (one round trip per row)
-template <bool SMALL>
+// K2 (operators of at most 2^19 rows): alpha_k, r += (-alpha) Ap, s = Dinv r,
+// partials of r.s and r.r; each row's operands are loaded together (one
+// round trip per row), requested before the prologue's sums
 __global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, int k) {
     double *__restrict__ r = d.r, *__restrict__ s = d.s;
     const double *__restrict__ Ap = d.Ap, *__restrict__ dinv = d.dinv;
     const bool precond = d.precond != 0;
     double rs = 0.0, rr = 0.0;
-    if constexpr (!SMALL) {
-        if (!running(d)) return;
-        double dsum[1];
-        if (d.Gdn <= kBlock)  // one round of the uniform loop (up to 256^3)
-            sum_partials_n<1>({d.P(P_DN)}, d.Gdn, dsum);
-        else
-            dsum[0] = sum_partials<RED_SUM>(d.P(P_DN), d.Gdn);
-        const double dn = dsum[0];
-        const double alpha = rho_slot(d, k) / dn;  // :93
-        if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) {
-            if (leader()) state_store(d.state, ST_BREAK);
-            return;
-        }
-        store_alpha(d, k, alpha);
-        const double malpha = -1.0 * alpha;
-        FOR_ROWS(d.n, row) {
-            // every operand first, so the row costs one memory round trip
-            const double ro = r[row], apr = Ap[row];
-            const double di = precond ? dinv[row] : 0.0;
-            const double ri = ro + malpha * apr;  // :97
-            r[row] = ri;
-            double si;
-            if (precond) {  // :99-105
-                si = 0.0;
-                si = si + ri * di;
-            } else {
-                si = ri;
-            }
-            s[row] = si;
-            rs = rs + ri * si;
-            rr = rr + ri * ri;
-        }
-    } else {
+    {
         ThreadRows t(d.n);
         double ro = 0.0, apr = 0.0, di = 0.0;
         const int st = state_load(d.state);
@@ -1383,8 +1323,8 @@ static int poll(gip_pcg *S, bool scalars) {
     return GIP_OK;
 }
 
-// The solve on a solver's workspace (allocated by gip_pcg_create);
-// arguments already checked.
+// The solve on a solver's workspace (allocated by gip_pcg_create, or the
+// no-alloc scratch); arguments already checked.
 static int pcg_solve_impl(gip_pcg *S, const double *b, double *x, const double *x0, const gip_pcg_opts *o,
                           gip_pcg_stats *stats) {
     gip_ctx *c = S->ctx;
@@ -1473,7 +1413,7 @@ static int pcg_solve_impl(gip_pcg *S, const double *b, double *x, const double *
                     else
                         DISPATCH_WG(A, S->nchunk, k_iter_spmv_small, v, d, S->p);
                     sample_end(smp);
-                    k_iter_update<true><<<S->grid, kBlock, 0, st>>>(d, k);
+                    k_iter_update<<<S->grid, kBlock, 0, st>>>(d, k);
                 }
             }
             GIP_LAUNCH_CHECK("pcg iteration");

@@ -101,9 +101,92 @@ static void dispatch_stream(hipStream_t s, int vec, bool nt, unsigned grid, size
            : launch_stream<KIND, 1, false>(s, grid, m, a, b, c, part);
 }
 
+// A device-wide barrier, `iters` times, across a cooperative launch (every
+// workgroup resident): each workgroup's thread 0 adds itself to a counter
+// with a relaxed device-scope atomic and waits until the counter reaches
+// the barrier's target.  FENCE adds a device-scope fence before the arrival
+// and after the wait (what a barrier that publishes data written before it
+// needs); PARTIALS also stores one value per workgroup before the barrier
+// and sums every workgroup's value after it.  The wait is bounded: past
+// `limit` polls (or once another workgroup has given up) a workgroup sets
+// *failed and stops waiting, so every wave exits.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_barrier_probe(unsigned *count, double *part, int iters, int *failed,
+                                                          unsigned limit) {
+    double acc = 0.0;
+    for (int k = 1; k <= iters; ++k) {
+        if (MODE == 2 && threadIdx.x == 0) part[blockIdx.x] = (double)(k + blockIdx.x);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (MODE >= 1) __threadfence();
+            __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned target = (unsigned)k * gridDim.x;
+            unsigned polls = 0;
+            while (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                if (++polls > limit || __hip_atomic_load(failed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    __hip_atomic_store(failed, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (MODE >= 1) __threadfence();
+        }
+        __syncthreads();
+        if (MODE == 2) {
+            double v = 0.0;
+            for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock)
+                v += __hip_atomic_load(&part[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            acc += v;
+            __syncthreads();
+        }
+    }
+    if (MODE == 2 && blockIdx.x == 0 && threadIdx.x == 0) part[gridDim.x] = acc;
+}
+
 }  // namespace gip
 
 using namespace gip;
+
+extern "C" int gip_barrier_probe(gip_ctx *c, int32_t grid, int32_t iters, int32_t mode, float *us_per_barrier) {
+    if (!c || !us_per_barrier || grid < 1 || iters < 1 || mode < 0 || mode > 2)
+        return set_error(GIP_EINVAL, "gip_barrier_probe: bad argument");
+    int per_cu = 0;
+    const void *fn = mode == 0 ? (const void *)k_barrier_probe<0>
+                     : mode == 1 ? (const void *)k_barrier_probe<1>
+                                 : (const void *)k_barrier_probe<2>;
+    GIP_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, 0));
+    if (grid > per_cu * c->num_cu) return set_error(GIP_EINVAL, "gip_barrier_probe: more workgroups than can be resident");
+    unsigned *count = nullptr;
+    int *failed = nullptr;
+    double *part = nullptr;
+    GIP_HIP_TRY(hipMalloc(&count, sizeof(unsigned)));
+    GIP_HIP_TRY(hipMalloc(&failed, sizeof(int)));
+    GIP_HIP_TRY(hipMalloc(&part, sizeof(double) * ((size_t)grid + 1)));
+    hipError_t e = hipMemsetAsync(count, 0, sizeof(unsigned), c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(failed, 0, sizeof(int), c->stream);
+    unsigned limit = 200000;
+    void *args[] = {&count, &part, &iters, &failed, &limit};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+    if (e == hipSuccess) e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kBlock), args, 0, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    int f = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess) e = hipMemcpy(&f, failed, sizeof(int), hipMemcpyDeviceToHost);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(count);
+    hipFree(failed);
+    hipFree(part);
+    if (e != hipSuccess) return hip_fail(e, "gip_barrier_probe");
+    if (f) return set_error(GIP_EHIP, "gip_barrier_probe: a barrier wait gave up");
+    *us_per_barrier = 1e3f * ms / (float)iters;
+    return GIP_OK;
+}
 
 extern "C" int gip_stream_probe(gip_ctx *c, int32_t kind, int32_t vec, int32_t nontemporal, int32_t grid, size_t n,
                                 double *a, double *b, double *cc, double *part, size_t part_len, int32_t reps,

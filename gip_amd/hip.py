@@ -87,6 +87,7 @@ SIGNATURES = {
     "gip_pcg_destroy": ([c_void_p], c_int),
     "gip_stream_probe": ([c_void_p, c_int32, c_int32, c_int32, c_int32, c_size_t, c_void_p, c_void_p, c_void_p,
                           c_void_p, c_size_t, c_int32, POINTER(c_float)], c_int),
+    "gip_barrier_probe": ([c_void_p, c_int32, c_int32, c_int32, POINTER(c_float)], c_int),
 }
 
 
@@ -417,6 +418,13 @@ def stream_probe(dev: Device, kind, n, *, vec=1, nontemporal=False, grid=0, reps
     check(lib().gip_stream_probe(dev.ctx, STREAM_KINDS[kind], vec, 1 if nontemporal else 0, grid, n, a.ptr, b.ptr,
                                  c.ptr, part.ptr, part.n, reps, byref(ms)))
     return ms.value
+
+
+def barrier_probe(dev: Device, grid, iters=2000, mode=1):
+    """Microseconds per device-wide barrier among `grid` workgroups (gip_barrier_probe)."""
+    us = c_float()
+    check(lib().gip_barrier_probe(dev.ctx, grid, iters, mode, byref(us)))
+    return us.value
 
 
 class Event:

@@ -62,6 +62,13 @@ int gip_batch_create(gip_ctx *ctx, int32_t ninst, const gip_cavity *const *cavs,
                      const gip_batch_opts *opts, gip_batch **out);
 /* One time step of every instance. */
 int gip_batch_step(gip_batch *b);
+/* Device time (ms, HIP events on the library stream) of the last step's three
+ * phases: ms[0] predictor and Poisson right-hand side, ms[1] the PCG
+ * iterations including the host's reads of the running count between poll
+ * groups, ms[2] correction, mass fluxes and CFL.  No counterpart in the
+ * reference (its GIP_Timer brackets whole phases on the host,
+ * GIP_Timer.h:6-26); the batch has no per-instance phase boundary. */
+int gip_batch_phase_ms(gip_batch *b, float *ms);
 int gip_batch_instance(gip_batch *b, int32_t inst, gip_batch_info *info);
 /* Instance field in its own layout ("u","v","w","p": n_all values; "mf": n_faces). */
 int gip_batch_get_field(gip_batch *b, int32_t inst, const char *field, double *host, int32_t n);

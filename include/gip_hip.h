@@ -185,6 +185,13 @@ int gip_dnrm_inf_dev(gip_ctx *ctx, int32_t n, const double *x, double *dresult);
 enum { GIP_STREAM_COPY = 0, GIP_STREAM_READ = 1, GIP_STREAM_WRITE = 2, GIP_STREAM_MIX = 3 };
 int gip_stream_probe(gip_ctx *ctx, int32_t kind, int32_t vec, int32_t nontemporal, int32_t grid, size_t n, double *a,
                      double *b, double *c, double *part, size_t part_len, int32_t reps, float *ms);
+/* Mean cost of a device-wide barrier among `grid` resident workgroups
+ * (cooperative launch, `iters` barriers): mode 0 a relaxed atomic counter,
+ * 1 with a device-scope fence before arriving and after leaving (what a
+ * barrier that publishes data needs), 2 also one value stored per
+ * workgroup before and every value summed after.  The measurement behind
+ * keeping the PCG's global reductions at kernel boundaries. */
+int gip_barrier_probe(gip_ctx *ctx, int32_t grid, int32_t iters, int32_t mode, float *us_per_barrier);
 
 /* ---- fused Jacobi-PCG ----------------------------------------------------
  * GIP_LinearSolver::solve (GIP_LinearSolver.h:45-117) with every scalar

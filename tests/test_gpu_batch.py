@@ -7,6 +7,8 @@ orders differ) and PCG iteration counts to +-1; graph replay equals eager
 execution bitwise; the mixed-precision batch matches the FP64 oracle to the
 solver tolerance.
 """
+import time
+
 import numpy as np
 import pytest
 
@@ -109,7 +111,13 @@ def test_batch_graph_replay_equals_eager(dev, tmp_path_factory):
         b = batch.Batch(dev, cavs, GAMMAS, pcg_tol=1e-8, precision=hip.PCG_MIXED, poll=4, use_graph=graph)
         out = []
         for _ in range(4):
+            t0 = time.perf_counter()
             b.step()
+            wall_ms = 1e3 * (time.perf_counter() - t0)
+            ph = b.phase_ms()
+            print(f"[batch graph={graph}] phases {ph} ms, step wall {wall_ms:.2f} ms", flush=True)
+            assert all(v > 0.0 for v in ph.values())
+            assert sum(ph.values()) <= wall_ms  # the events bracket device work inside the step
             dev.sync()
             out.append([b.info(k) for k in range(len(cavs))])
         runs.append((out, [b.field(k, "p") for k in range(len(cavs))]))
