@@ -89,4 +89,7 @@ struct gip_mat {
 
 namespace gip {
 int reduce_grid(int n);
+// Looks up the no-alloc PCG scratch array's address on `device` (current);
+// called once per device by gip_init.
+int pcg_scratch_setup(int device);
 }

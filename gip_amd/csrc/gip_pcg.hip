@@ -47,6 +47,7 @@
 // constant mode is left to the FP64 step.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -1108,6 +1109,44 @@ __global__ __launch_bounds__(kBlock) void k_az32(int n, const double *__restrict
     }
 }
 
+// ---- the solve without allocations -------------------------------------------
+//
+// The workspace of one FP64 solve on an operator of at most kScratchRows
+// rows lives in `scratch`, a file-scope device array (one copy per GPU, held
+// until the process ends): r, s, Ap, Dinv (kScratchRows doubles each), the
+// two direction buffers (kScratchAll each), the partial-sum arrays, the
+// scalars and the state words.  Host code never passes `scratch` itself
+// (in host code the name is a stand-in, not a device address): gip_init
+// looks up its address on each device with hipGetSymbolAddress and keeps it
+// in g_scratch, and the entry point builds its workspace from that pointer.
+// Every call on a GPU shares its array; what must start at zero is cleared
+// by a kernel at the start of each call, so calls that overlap on one GPU
+// race.
+constexpr int kScratchRows = 1 << 18;
+constexpr int kScratchAll = (1 << 18) + (1 << 16);
+constexpr size_t kScratchZero = (size_t)P_COUNT * kMaxRedBlocks + S_COUNT + 1;  // partials, scalars, state words
+constexpr size_t kScratchDoubles = 4 * (size_t)kScratchRows + 2 * (size_t)kScratchAll + kScratchZero;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kScratchDevices = 64;
+static double *g_scratch[kScratchDevices];
+static std::mutex g_scratch_mu;
+
+// Called by gip_init with `device` current.
+int pcg_scratch_setup(int device) {
+    if (device < 0 || device >= kScratchDevices) return GIP_OK;  // no no-alloc solves on this device
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return GIP_OK;
+    double *ptr = nullptr;
+    GIP_HIP_TRY(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+    g_scratch[device] = ptr;
+    return GIP_OK;
+}
+
+__global__ __launch_bounds__(kBlock) void k_zero(size_t n, double *__restrict__ a) {
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) a[i] = 0.0;
+}
+
 }  // namespace gip
 
 using namespace gip;
@@ -1511,6 +1550,58 @@ int gip_pcg_solve(gip_pcg *S, const double *b, double *x, const double *x0, cons
     if (!S || !b || !x || !x0 || !o || o->maxit < 0) return set_error(GIP_EINVAL, "gip_pcg_solve: bad argument");
     if (o->precond && S->A->diag_missing) return set_error(GIP_EINVAL, "gip_pcg_solve: Jacobi needs every diagonal entry");
     return pcg_solve_impl(S, b, x, x0, o, stats);
+}
+
+int gip_pcg_solve_noalloc(gip_ctx *c, const gip_mat *A, int32_t n_all, const double *b, double *x, const double *x0,
+                          const gip_pcg_opts *o, gip_pcg_stats *stats) {
+    if (!c || !A || !b || !x || !x0 || !o || o->maxit < 0)
+        return set_error(GIP_EINVAL, "gip_pcg_solve_noalloc: bad argument");
+    if (A->ctx != c) return set_error(GIP_EINVAL, "gip_pcg_solve_noalloc: operator belongs to another context");
+    if (o->precision != GIP_PCG_FP64) return set_error(GIP_EINVAL, "gip_pcg_solve_noalloc: FP64 only");
+    if (o->time_products) return set_error(GIP_EINVAL, "gip_pcg_solve_noalloc: no product timing");
+    if (A->format != GIP_FMT_ELL && A->format != GIP_FMT_CSR && A->format != GIP_FMT_STENCIL)
+        return set_error(GIP_EINVAL, "gip_pcg_solve_noalloc: operator must be ELL, stencil or CSR");
+    if (A->nrows > kScratchRows || n_all > kScratchAll)
+        return set_error(GIP_EINVAL, "gip_pcg_solve_noalloc: at most 262144 rows and 327680 vector entries");
+    if (n_all < A->nrows || n_all < A->cols_needed)
+        return set_error(GIP_EINVAL, "gip_pcg_solve_noalloc: vectors shorter than the operator's columns");
+    if (o->precond && A->diag_missing)
+        return set_error(GIP_EINVAL, "gip_pcg_solve_noalloc: Jacobi needs every diagonal entry");
+    double *base = (c->device >= 0 && c->device < kScratchDevices) ? g_scratch[c->device] : nullptr;
+    if (!base) return set_error(GIP_EINVAL, "gip_pcg_solve_noalloc: no scratch on this device (gip_init sets it up)");
+    GIP_HIP_TRY(hipSetDevice(c->device));
+    int h_state[2] = {0, 0};
+    double h_sc[S_COUNT] = {};
+    gip_pcg S{};
+    S.ctx = c;
+    S.A = A;
+    S.n = A->nrows;
+    S.n_all = n_all;
+    S.grid = pcg_grid(A->nrows);
+    S.nchunk = std::max(1, (A->nrows + kBlock - 1) / kBlock);
+    S.fold = false;  // at most 1024 chunks: the fused two-kernel iteration
+    S.gdn = S.nchunk;
+    S.r = base;
+    S.s = S.r + kScratchRows;
+    S.Ap = S.s + kScratchRows;
+    S.dinv = S.Ap + kScratchRows;
+    S.p = S.dinv + kScratchRows;
+    S.p2 = S.p + kScratchAll;
+    double *zero = S.p2 + kScratchAll;
+    S.part = zero;
+    S.sc = S.part + (size_t)P_COUNT * kMaxRedBlocks;
+    S.state = reinterpret_cast<int *>(S.sc + S_COUNT);
+    S.h_state = h_state;
+    S.h_sc = h_sc;
+    // partial slots past each producer's grid, the scalars and the state
+    // words start at zero (a previous call may have left other values)
+    k_zero<<<(unsigned)((kScratchZero + kBlock - 1) / kBlock), kBlock, 0, c->stream>>>(kScratchZero, zero);
+    GIP_LAUNCH_CHECK("gip_pcg_solve_noalloc: clear");
+    if (!A->diag_missing) {
+        int rc = gip_mat_diag_inv(A, S.dinv);
+        if (rc) return rc;
+    }
+    return pcg_solve_impl(&S, b, x, x0, o, stats);
 }
 
 }  // extern "C"

@@ -69,6 +69,11 @@ int gip_init(int device, gip_ctx **out) {
         gip_finalize(c);
         return rc;
     }
+    int rc = pcg_scratch_setup(device);  // the no-alloc solve's scratch on this device
+    if (rc) {
+        gip_finalize(c);
+        return rc;
+    }
     *out = c;
     return GIP_OK;
 }

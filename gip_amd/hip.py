@@ -88,6 +88,8 @@ SIGNATURES = {
     "gip_stream_probe": ([c_void_p, c_int32, c_int32, c_int32, c_int32, c_size_t, c_void_p, c_void_p, c_void_p,
                           c_void_p, c_size_t, c_int32, POINTER(c_float)], c_int),
     "gip_barrier_probe": ([c_void_p, c_int32, c_int32, c_int32, POINTER(c_float)], c_int),
+    "gip_pcg_solve_noalloc": ([c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, POINTER(PcgOpts),
+                               POINTER(PcgStats)], c_int),
 }
 
 
@@ -425,6 +427,17 @@ def barrier_probe(dev: Device, grid, iters=2000, mode=1):
     us = c_float()
     check(lib().gip_barrier_probe(dev.ctx, grid, iters, mode, byref(us)))
     return us.value
+
+
+def pcg_solve_noalloc(A: Matrix, n_all, b, x, x0, *, maxit=100, tol=0.0, precond=True, poll=16):
+    """The FP64 solve of PCG.solve without allocations (gip_pcg_solve_noalloc):
+    the workspace is the library's per-device scratch array."""
+    if b.n < A.nrows or x.n < n_all or x0.n < n_all:
+        raise ValueError("PCG vectors shorter than n_all")
+    o = PcgOpts(maxit, tol, 1 if precond else 0, PCG_FP64, 0.1, poll, 0, 0)
+    st = PcgStats()
+    check(lib().gip_pcg_solve_noalloc(A.dev.ctx, A.ptr, int(n_all), b.ptr, x.ptr, x0.ptr, byref(o), byref(st)))
+    return PcgResult(st.iters, st.corrections, st.relres, st.status, st.product_ms, st.product_samples)
 
 
 class Event:

@@ -244,6 +244,20 @@ int gip_pcg_solve(gip_pcg *solver, const double *b, double *x, const double *x0,
                   gip_pcg_stats *stats);
 int gip_pcg_destroy(gip_pcg *solver);
 
+/* The FP64 solve of gip_pcg_solve without any allocation: no counterpart in
+ * the reference; for callers that cannot allocate device or pinned memory
+ * (code under stream capture, tight serving loops).  The solver workspace
+ * (r, s, Ap, Dinv, two directions, partial sums, scalars, state) is a
+ * device array of about 13.8 MB, one per GPU, set up by gip_init and held
+ * until the process ends; its limit is 262,144 rows (nrows(A)) and 327,680
+ * vector entries (n_all): larger operators, GIP_PCG_MIXED and
+ * time_products are rejected with GIP_EINVAL.  Same kernels, iterates and
+ * stopping rule as gip_pcg_solve on a solver created for A.  Every call on
+ * a GPU shares that GPU's workspace: calls that overlap on one GPU (other
+ * host threads, other streams) race. */
+int gip_pcg_solve_noalloc(gip_ctx *ctx, const gip_mat *A, int32_t n_all, const double *b, double *x,
+                          const double *x0, const gip_pcg_opts *opts, gip_pcg_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif
