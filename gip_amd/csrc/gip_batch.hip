@@ -178,6 +178,38 @@ __global__ __launch_bounds__(kBlock) void k_b_cfl_fin(const double *part, Seg S,
     }
 }
 
+// The PCG iteration kernels (direction, product, update) instead run one
+// workgroup per chunk in dispatch order (grid = the chunk count: the pace of
+// a plain stream) and store one partial per wavefront, with no barrier
+// (seg_store_w); their finalize kernels sum an instance's wavefront
+// partials in chunk order (seg_total_w).  A chunk of an instance that has
+// left the PCG returns after two loads.
+template <int K>
+__device__ inline void seg_store_w(double (&v)[K], int c, double *part) {
+    const size_t w = (size_t)c * kWaves + (threadIdx.x >> 6);
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) part[w * K + k] = v[k];
+    }
+}
+
+template <int K>
+__device__ inline void seg_total_w(const double *part, int inst, const Seg &S, double (&out)[K]) {
+    const size_t w0 = (size_t)S.inst_c0[inst] * kWaves;
+    const int nw = S.inst_nch[inst] * kWaves;
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = 0.0;
+    for (int j = threadIdx.x; j < nw; j += kBlock)
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] = acc[k] + part[(w0 + j) * K + k];
+    block_reduce_n<K>(acc);
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = acc[k];
+}
+
 // ---- batched PCG ------------------------------------------------------------
 
 struct BPcg {
@@ -189,7 +221,7 @@ struct BPcg {
     const double *one, *az;  // MIXED coarse step: 1 on real inner rows (0 on padding), A 1
     double *sc;              // B * B_COUNT
     int *state, *iters, *active;
-    double *part;  // chunk totals
+    double *part;  // chunk totals or wavefront partials
     int maxit;
     double inner2;
 };
@@ -269,35 +301,31 @@ __global__ __launch_bounds__(kBlock) void k_bp_setup_fin(BPcg d, Seg S, double t
 // direction of the running instances: p = 1.0 s + beta p (GIP_LinearSolver.h:113;
 // the first iteration has beta = 0 and p = 0 on the solved rows, so p = s)
 __global__ __launch_bounds__(kBlock) void k_bp_dir(BPcg d, Seg S, double *__restrict__ p) {
-    const double *__restrict__ s = d.s;
-    FOR_CHUNKS(S, c) {
-        const int inst = S.chunk_inst[c];
-        if (d.state[inst] != BS_RUN) continue;
-        const int row = c * kChunk + threadIdx.x;
-        const double beta = bsc(d, inst)[B_BETA];
-        const double si = s[row], po = p[row];
-        p[row] = 1.0 * si + beta * po;
-    }
+    const int c = blockIdx.x;
+    const int inst = S.chunk_inst[c];
+    if (d.state[inst] != BS_RUN) return;
+    const int row = c * kChunk + threadIdx.x;
+    const double beta = bsc(d, inst)[B_BETA];
+    const double si = d.s[row], po = p[row];
+    p[row] = 1.0 * si + beta * po;
 }
 
 // Ap = A p (one gather of p per neighbour), chunk partials of p.Ap
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_bp_spmv(EllView A, BPcg d, Seg S, const double *__restrict__ p) {
-    double *__restrict__ Ap = d.Ap;
+    const int c = blockIdx.x;
+    const int inst = S.chunk_inst[c];
+    if (d.state[inst] != BS_RUN) return;
+    const int row = c * kChunk + threadIdx.x;
     Operand<double> po{p};
-    FOR_CHUNKS(S, c) {
-        const int inst = S.chunk_inst[c];
-        if (d.state[inst] != BS_RUN) continue;
-        const int row = c * kChunk + threadIdx.x;
-        const double pi = p[row];
-        double sum;
-        if (row_dot<W, double, double>(A, A.val, po, row, sum))
-            Ap[row] = sum;
-        else
-            sum = Ap[row];
-        double v[1] = {pi * sum};
-        seg_store<1, RED_SUM>(v, c, d.part);
-    }
+    const double pi = p[row];
+    double sum;
+    if (row_dot<W, double, double>(A, A.val, po, row, sum))
+        d.Ap[row] = sum;
+    else
+        sum = d.Ap[row];
+    double v[1] = {pi * sum};
+    seg_store_w<1>(v, c, d.part);
 }
 
 // alpha = rho / p.Ap of the running instances (FP64 and MIXED)
@@ -305,7 +333,7 @@ __global__ __launch_bounds__(kBlock) void k_bp_spmv_fin(BPcg d, Seg S) {
     const int inst = blockIdx.x;
     if (d.state[inst] != BS_RUN) return;
     double out[1];
-    seg_total<1, RED_SUM>(d.part, inst, S, out);
+    seg_total_w<1>(d.part, inst, S, out);
     if (threadIdx.x == 0) {
         double *q = bsc(d, inst);
         const double alpha = q[B_RHO] / out[0];
@@ -315,35 +343,34 @@ __global__ __launch_bounds__(kBlock) void k_bp_spmv_fin(BPcg d, Seg S) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_bp_update(BPcg d, Seg S, const double *__restrict__ p, int precond) {
-    FOR_CHUNKS(S, c) {
-        const int inst = S.chunk_inst[c];
-        if (d.state[inst] != BS_RUN) continue;
-        const int row = c * kChunk + threadIdx.x;
-        const double alpha = bsc(d, inst)[B_ALPHA];
-        // every operand first: one memory round trip per row
-        const double xo = d.x[row], pr = p[row], ro = d.r[row], apr = d.Ap[row];
-        const double di = precond ? d.dinv[row] : 0.0;
-        d.x[row] = xo + alpha * pr;
-        const double ri = ro + (-1.0 * alpha) * apr;
-        d.r[row] = ri;
-        double si;
-        if (precond) {
-            si = 0.0;
-            si = si + ri * di;
-        } else {
-            si = ri;
-        }
-        d.s[row] = si;
-        double v[2] = {ri * si, ri * ri};
-        seg_store<2, RED_SUM>(v, c, d.part);
+    const int c = blockIdx.x;
+    const int inst = S.chunk_inst[c];
+    if (d.state[inst] != BS_RUN) return;
+    const int row = c * kChunk + threadIdx.x;
+    const double alpha = bsc(d, inst)[B_ALPHA];
+    // every operand first: one memory round trip per row
+    const double xo = d.x[row], pr = p[row], ro = d.r[row], apr = d.Ap[row];
+    const double di = precond ? d.dinv[row] : 0.0;
+    d.x[row] = xo + alpha * pr;
+    const double ri = ro + (-1.0 * alpha) * apr;
+    d.r[row] = ri;
+    double si;
+    if (precond) {
+        si = 0.0;
+        si = si + ri * di;
+    } else {
+        si = ri;
     }
+    d.s[row] = si;
+    double v[2] = {ri * si, ri * ri};
+    seg_store_w<2>(v, c, d.part);
 }
 
 __global__ __launch_bounds__(kBlock) void k_bp_update_fin(BPcg d, Seg S) {
     const int inst = blockIdx.x;
     if (d.state[inst] != BS_RUN) return;
     double out[2];
-    seg_total<2, RED_SUM>(d.part, inst, S, out);
+    seg_total_w<2>(d.part, inst, S, out);
     if (threadIdx.x == 0) {
         double *q = bsc(d, inst);
         const double r0 = q[B_RHO];
@@ -424,57 +451,52 @@ __global__ __launch_bounds__(kBlock) void k_bpm_setup_fin(BPcg d, Seg S, double 
 }
 
 __global__ __launch_bounds__(kBlock) void k_bpm_dir(BPcg d, Seg S, float *__restrict__ p) {
-    const float *__restrict__ s = d.s32;
-    FOR_CHUNKS(S, c) {
-        const int inst = S.chunk_inst[c];
-        if (d.state[inst] != BS_RUN) continue;
-        const int row = c * kChunk + threadIdx.x;
-        const float beta = (float)bsc(d, inst)[B_BETA];
-        const float si = s[row], po = p[row];
-        p[row] = 1.0f * si + beta * po;
-    }
+    const int c = blockIdx.x;
+    const int inst = S.chunk_inst[c];
+    if (d.state[inst] != BS_RUN) return;
+    const int row = c * kChunk + threadIdx.x;
+    const float beta = (float)bsc(d, inst)[B_BETA];
+    const float si = d.s32[row], po = p[row];
+    p[row] = 1.0f * si + beta * po;
 }
 
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_bpm_spmv(EllView A, BPcg d, Seg S, const float *__restrict__ p) {
-    float *__restrict__ Ap = d.Ap32;
+    const int c = blockIdx.x;
+    const int inst = S.chunk_inst[c];
+    if (d.state[inst] != BS_RUN) return;
+    const int row = c * kChunk + threadIdx.x;
     Operand<float> po{p};
-    FOR_CHUNKS(S, c) {
-        const int inst = S.chunk_inst[c];
-        if (d.state[inst] != BS_RUN) continue;
-        const int row = c * kChunk + threadIdx.x;
-        const float pi = p[row];
-        float sum;
-        row_dot<W, float, float>(A, A.val32, po, row, sum);
-        Ap[row] = sum;
-        double v[1] = {(double)pi * (double)sum};
-        seg_store<1, RED_SUM>(v, c, d.part);
-    }
+    const float pi = p[row];
+    float sum;
+    row_dot<W, float, float>(A, A.val32, po, row, sum);
+    d.Ap32[row] = sum;
+    double v[1] = {(double)pi * (double)sum};
+    seg_store_w<1>(v, c, d.part);
 }
 
 __global__ __launch_bounds__(kBlock) void k_bpm_update(BPcg d, Seg S, const float *__restrict__ p, int precond) {
-    FOR_CHUNKS(S, c) {
-        const int inst = S.chunk_inst[c];
-        if (d.state[inst] != BS_RUN) continue;
-        const int row = c * kChunk + threadIdx.x;
-        const float a32 = (float)bsc(d, inst)[B_ALPHA];
-        const float xo = d.xc32[row], pr = p[row], ro = d.r32[row], apr = d.Ap32[row];
-        const float di = precond ? d.dinv32[row] : 0.0f;
-        d.xc32[row] = xo + a32 * pr;
-        const float ri = ro - a32 * apr;
-        d.r32[row] = ri;
-        const float si = precond ? ri * di : ri;
-        d.s32[row] = si;
-        double v[2] = {(double)ri * (double)si, (double)ri * (double)ri};
-        seg_store<2, RED_SUM>(v, c, d.part);
-    }
+    const int c = blockIdx.x;
+    const int inst = S.chunk_inst[c];
+    if (d.state[inst] != BS_RUN) return;
+    const int row = c * kChunk + threadIdx.x;
+    const float a32 = (float)bsc(d, inst)[B_ALPHA];
+    const float xo = d.xc32[row], pr = p[row], ro = d.r32[row], apr = d.Ap32[row];
+    const float di = precond ? d.dinv32[row] : 0.0f;
+    d.xc32[row] = xo + a32 * pr;
+    const float ri = ro - a32 * apr;
+    d.r32[row] = ri;
+    const float si = precond ? ri * di : ri;
+    d.s32[row] = si;
+    double v[2] = {(double)ri * (double)si, (double)ri * (double)ri};
+    seg_store_w<2>(v, c, d.part);
 }
 
 __global__ __launch_bounds__(kBlock) void k_bpm_update_fin(BPcg d, Seg S) {
     const int inst = blockIdx.x;
     if (d.state[inst] != BS_RUN) return;
     double out[2];
-    seg_total<2, RED_SUM>(d.part, inst, S, out);
+    seg_total_w<2>(d.part, inst, S, out);
     if (threadIdx.x == 0) {
         double *q = bsc(d, inst);
         const double r0 = q[B_RHO];
@@ -795,18 +817,19 @@ int enqueue_iter(gip_batch *b) {
     BPcg d = pcg_view(b);
     const gip_mat *L = b->op[OP_L];
     const bool mixed = b->o.precision == GIP_PCG_MIXED;
+    const unsigned nch = (unsigned)b->seg_in.nchunk;  // one workgroup per chunk
     for (int k = 0; k < b->o.poll; ++k) {
         if (mixed) {
-            k_bpm_dir<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, b->pq);
-            BDISPATCH(L, b->grid_in, k_bpm_spmv, view_of(L), d, b->seg_in, b->pq);
+            k_bpm_dir<<<nch, kBlock, 0, st>>>(d, b->seg_in, b->pq);
+            BDISPATCH(L, nch, k_bpm_spmv, view_of(L), d, b->seg_in, b->pq);
             k_bp_spmv_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in);
-            k_bpm_update<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, b->pq, 1);
+            k_bpm_update<<<nch, kBlock, 0, st>>>(d, b->seg_in, b->pq, 1);
             k_bpm_update_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in);
         } else {
-            k_bp_dir<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, b->pp);
-            BDISPATCH(L, b->grid_in, k_bp_spmv, view_of(L), d, b->seg_in, b->pp);
+            k_bp_dir<<<nch, kBlock, 0, st>>>(d, b->seg_in, b->pp);
+            BDISPATCH(L, nch, k_bp_spmv, view_of(L), d, b->seg_in, b->pp);
             k_bp_spmv_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in);
-            k_bp_update<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, b->pp, 1);
+            k_bp_update<<<nch, kBlock, 0, st>>>(d, b->seg_in, b->pp, 1);
             k_bp_update_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in);
         }
     }
@@ -961,7 +984,8 @@ int gip_batch_create(gip_ctx *c, int32_t ninst, const gip_cavity *const *cavs, c
         if (!rc) rc = dalloc_t(b, &b->caz, NPs);
     }
     if (!rc) rc = dalloc_t(b, &b->sc, (size_t)B * B_COUNT);
-    if (!rc) rc = dalloc_t(b, &b->part, (size_t)std::max(b->NP, b->FP) / kChunk * 3 + 8);
+    // chunk totals (up to 3 per chunk) or wavefront partials (up to 2 per wavefront)
+    if (!rc) rc = dalloc_t(b, &b->part, (size_t)std::max(b->NP, b->FP) / kChunk * (2 * kWaves) + 8);
     if (!rc) rc = dalloc_t(b, &b->state, (size_t)B);
     if (!rc) rc = dalloc_t(b, &b->iters, (size_t)B);
     if (!rc) rc = dalloc_t(b, &b->active, 1);

@@ -37,6 +37,15 @@ struct RedOp<RED_MIN> {
     __device__ static double term(const double *x, const double *, int i) { return x[i]; }
 };
 
+constexpr int kWaves = kBlock / 64;  // wavefronts per workgroup
+
+// Sum over the wavefront (shuffle tree, no LDS or barrier); valid in lane 0.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = v + __shfl_down(v, off, 64);
+    return v;
+}
+
 // Reduce v over the workgroup; the result is valid in thread 0.
 template <int OP>
 __device__ inline double block_reduce(double v) {

@@ -13,8 +13,10 @@
 // every workgroup of a kernel re-reduces the previous kernel's partials in
 // its prologue (fence-free: plain stores, ordered by the kernel boundary;
 // every workgroup computes bit-identical scalars), so a kernel costs about
-// one memory round trip before it stores.  Longer operators sweep one row
-// per thread in dispatch order (the pace of a plain stream of each kernel's
+// one memory round trip before it stores.  Up to kRowsMin rows, four
+// kernels: grid-stride sweeps whose prologues re-reduce the partials, the
+// product's partials folded first.  Longer operators sweep one row per
+// thread in dispatch order (the pace of a plain stream of each kernel's
 // access mix) and take their scalars from one-workgroup kernels (fold +
 // finalize) instead, since that many workgroups cannot each re-reduce the
 // partials; see the long-path section below.  Scalars that must survive a
@@ -102,7 +104,7 @@ struct PcgDev {
     double *chunk; // partials of the product: with the fold, kWaves per
                    // 256-row chunk (one per wavefront); without it, one per
                    // chunk, and these are the P_DN partials themselves
-    double *chunk_s;  // long operators: per-wavefront partials of 1.Ap (MIXED deflation)
+    double *chunk_s;  // long operators: a second array of per-wavefront partials (FP64: r.r; MIXED: 1.Ap)
     double *x, *r, *s, *Ap;
     const double *b, *dinv;
     float *r32, *s32, *Ap32, *xc32;
@@ -142,12 +144,6 @@ __device__ inline double early_load(const double *p) {
 }
 __device__ inline int conv_stamp(int k) { return ST_CONV | (k << 2); }
 __device__ inline bool leader() { return blockIdx.x == 0 && threadIdx.x == 0; }
-// Sum over the wavefront (shuffle tree, no LDS or barrier); valid in lane 0.
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = v + __shfl_down(v, off, 64);
-    return v;
-}
 
 // v of lane i - (CTRL - 0x110) within the lane's row of 16 (0 past the
 // row's start): a DPP row shift, moved as two 32-bit halves
@@ -176,7 +172,6 @@ __device__ inline double wave_sum_dpp(double v) {
     v = v + dpp_row_shr<0x118>(v);
     return ((read_lane(v, 15) + read_lane(v, 31)) + read_lane(v, 47)) + read_lane(v, 63);
 }
-constexpr int kWaves = kBlock / 64;  // wavefronts per workgroup
 __device__ inline double rho_slot(const PcgDev &d, int k) { return d.sc[S_RHO0 + (k & 1)]; }
 // p.Ap partials of iteration k in the fused (small-operator) iteration
 __device__ inline double *dn_part(const PcgDev &d, int k) { return d.P((k & 1) ? P_DN2 : P_DN); }
@@ -560,7 +555,93 @@ __global__ __launch_bounds__(kBlock) void k_iter_update(PcgDev d, int k) {
     if (leader()) *d.iters = k + 1;
 }
 
-// ---- FP64 iteration on long operators (more than 2^19 rows) -----------------
+// ---- FP64 iteration on mid-size operators (2^19 to kRowsMin rows) ----------
+//
+// A sweep of 2048 (or fewer) workgroups striding over the rows in
+// XCD-contiguous order, whose prologue re-reduces the previous kernel's
+// partials (every workgroup forms the same scalars): four launches per
+// iteration (direction, product, fold, update).  At these sizes the three
+// extra launches of the row-per-thread iteration cost more than its faster
+// sweeps save (FP64 solve to 1e-10, grid-stride against row-per-thread:
+// 128^3 94.9 against 107.0 ms, 160^3 205.8 / 227.7, 192^3 419.2 / 424.6;
+// past the switch 224^3 834.0 / 776.4, 256^3 1418 / 1286).
+constexpr int kRowsMin = 1 << 23;
+
+// K0 (k >= 1): rho_k, the convergence test, x += alpha_{k-1} p,
+// beta, p = 1.0 s + beta p
+__global__ __launch_bounds__(kBlock) void k_iter_dir(PcgDev d, double *__restrict__ p, int k) {
+    double *__restrict__ x = d.x;
+    const double *__restrict__ s = d.s;
+    const int st = state_load(d.state);
+    if (st != ST_RUN && st != conv_stamp(k)) return;
+    const double rho = sum_partials<RED_SUM>(d.P(P_RS), d.G);  // :107
+    const double alpha = alpha_slot(d, k - 1);
+    const double tol2 = d.sc[S_TOL2];
+    if (tol2 >= 0.0) {
+        const double rr = sum_partials<RED_SUM>(d.P(P_RR), d.G);
+        if (rr <= tol2) {  // every workgroup reaches the same decision
+            x_update(d, p, alpha);
+            if (leader()) state_store(d.state, conv_stamp(k));
+            return;
+        }
+    }
+    const double beta = rho / rho_slot(d, k - 1);  // :109
+    if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
+    FOR_ROWS(d.n, row) {
+        const double xo = x[row], si = s[row], po = p[row];
+        x[row] = xo + alpha * po;       // :95 of iteration k-1
+        p[row] = 1.0 * si + beta * po;  // :113
+    }
+}
+
+// K2: alpha_k from the folded p.Ap partials, r += (-alpha) Ap, s = Dinv r,
+// partials of r.s and r.r (one per workgroup); each row's operands are
+// loaded together (one round trip per row)
+__global__ __launch_bounds__(kBlock) void k_iter_update_grid(PcgDev d, int k) {
+    if (!running(d)) return;
+    double *__restrict__ r = d.r, *__restrict__ s = d.s;
+    const double *__restrict__ Ap = d.Ap, *__restrict__ dinv = d.dinv;
+    const bool precond = d.precond != 0;
+    double dsum[1];
+    if (d.Gdn <= kBlock)  // one round of the uniform loop
+        sum_partials_n<1>({d.P(P_DN)}, d.Gdn, dsum);
+    else
+        dsum[0] = sum_partials<RED_SUM>(d.P(P_DN), d.Gdn);
+    const double dn = dsum[0];
+    const double alpha = rho_slot(d, k) / dn;  // :93
+    if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha)) {
+        if (leader()) state_store(d.state, ST_BREAK);
+        return;
+    }
+    store_alpha(d, k, alpha);
+    const double malpha = -1.0 * alpha;
+    double rs = 0.0, rr = 0.0;
+    FOR_ROWS(d.n, row) {
+        const double ro = r[row], apr = Ap[row];
+        const double di = precond ? dinv[row] : 0.0;
+        const double ri = ro + malpha * apr;  // :97
+        r[row] = ri;
+        double si;
+        if (precond) {  // :99-105
+            si = 0.0;
+            si = si + ri * di;
+        } else {
+            si = ri;
+        }
+        s[row] = si;
+        rs = rs + ri * si;
+        rr = rr + ri * ri;
+    }
+    double red[2] = {rs, rr};
+    block_reduce_n<2>(red);
+    if (threadIdx.x == 0) {
+        d.P(P_RS)[blockIdx.x] = red[0];
+        d.P(P_RR)[blockIdx.x] = red[1];
+    }
+    if (leader()) *d.iters = k + 1;
+}
+
+// ---- FP64 iteration on long operators (more than kRowsMin rows) -------------
 //
 // One row per thread over the whole operator, workgroups in dispatch order
 // (consecutive 256-row chunks on consecutive XCDs, the whole chip streaming
@@ -622,16 +703,18 @@ __global__ __launch_bounds__(kBlock) void k_fold_two(PcgDev d, const double *__r
 }
 
 // F1 (one workgroup): alpha_k = rho_k / p.Ap, or a breakdown
+__device__ inline void alpha_finish(const PcgDev &d, int k, double dn) {
+    const double alpha = rho_slot(d, k) / dn;  // :93
+    if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha))
+        state_store(d.state, ST_BREAK);
+    else
+        d.sc[S_ALPHA0 + (k & 1)] = alpha;
+}
+
 __global__ __launch_bounds__(kBlock) void k_alpha_fin(PcgDev d, int k) {
     if (!running(d)) return;
     const double dn = sum_partials<RED_SUM>(d.P(P_DN), d.Gdn);
-    const double alpha = rho_slot(d, k) / dn;  // :93
-    if (threadIdx.x == 0) {
-        if (dn == 0.0 || !isfinite(dn) || !isfinite(alpha))
-            state_store(d.state, ST_BREAK);
-        else
-            d.sc[S_ALPHA0 + (k & 1)] = alpha;
-    }
+    if (threadIdx.x == 0) alpha_finish(d, k, dn);
 }
 
 // K2 of iteration k
@@ -669,12 +752,8 @@ __global__ __launch_bounds__(kBlock) void k_iter_update_rows(PcgDev d, int k) {
 
 // F2 (one workgroup) after K2 of iteration k - 1: rho_k, r.r, the
 // convergence test (stamped with k), beta_k
-__global__ __launch_bounds__(kBlock) void k_beta_fin(PcgDev d, int k) {
-    if (!running(d)) return;
-    double sums[2];
-    sum_partials_n<2>({d.P(P_RS), d.P(P_RR)}, d.Gdn, sums);
-    if (threadIdx.x != 0) return;
-    const double rho = sums[0], rr = sums[1], tol2 = d.sc[S_TOL2];
+__device__ inline void beta_finish(const PcgDev &d, int k, double rho, double rr) {
+    const double tol2 = d.sc[S_TOL2];
     d.sc[S_RR] = rr;
     if (tol2 >= 0.0 && rr <= tol2) {
         state_store(d.state, conv_stamp(k));
@@ -683,6 +762,14 @@ __global__ __launch_bounds__(kBlock) void k_beta_fin(PcgDev d, int k) {
     d.sc[S_BETA] = rho / rho_slot(d, k - 1);  // :107-111
     d.sc[S_RHO0 + (k & 1)] = rho;
 }
+
+__global__ __launch_bounds__(kBlock) void k_beta_fin(PcgDev d, int k) {
+    if (!running(d)) return;
+    double sums[2];
+    sum_partials_n<2>({d.P(P_RS), d.P(P_RR)}, d.Gdn, sums);
+    if (threadIdx.x == 0) beta_finish(d, k, sums[0], sums[1]);
+}
+
 
 // x += alpha_k p of the last iteration k, unless a breakdown stopped the solve
 __global__ __launch_bounds__(kBlock) void k_x_flush_rows(PcgDev d, const double *__restrict__ p, int k) {
@@ -1166,6 +1253,7 @@ struct gip_pcg {
     int nchunk, gdn;
     bool fold;  // nchunk > kMaxRedBlocks: the chunk partials need the fold kernel;
                 // otherwise the fused two-kernel iteration runs
+    bool rows;  // fold and more than kRowsMin rows: the row-per-thread iteration
     int *state;    // state, iters
     int *h_state;  // pinned copy
     double *h_sc;  // pinned scalars
@@ -1247,6 +1335,7 @@ int gip_pcg_create(gip_ctx *c, const gip_mat *A, int32_t n_all, gip_pcg **out) {
     // per-chunk partials directly; beyond, a fold kernel sums each run of
     // 256 chunks first
     S->fold = S->nchunk > kMaxRedBlocks;
+    S->rows = S->fold && S->n > kRowsMin;
     S->gdn = S->fold ? (S->nchunk + kBlock - 1) / kBlock : S->nchunk;
     if (S->gdn > kMaxRedBlocks) {
         delete S;
@@ -1432,7 +1521,14 @@ static int pcg_solve_impl(gip_pcg *S, const double *b, double *x, const double *
         while (k < o->maxit && S->h_state[0] == ST_RUN) {
             int chunk = std::min(fixed ? 4096 : pollk, o->maxit - k);
             for (int j = 0; j < chunk; ++j, ++k) {
-                if (S->fold) {
+                if (S->fold && !S->rows) {
+                    if (k > 0) k_iter_dir<<<S->grid, kBlock, 0, st>>>(d, S->p, k);
+                    const bool smp = sample_begin(k);
+                    DISPATCH_WG(A, S->nchunk, k_iter_spmv, v, d, S->p);
+                    sample_end(smp);
+                    k_fold_dn<<<S->gdn, kBlock, 0, st>>>(d);
+                    k_iter_update_grid<<<S->grid, kBlock, 0, st>>>(d, k);
+                } else if (S->fold) {
                     // (partials: p.Ap in chunk, then r.s in chunk and r.r in chunk_s)
                     if (k > 0) k_iter_dir_rows<<<S->nchunk, kBlock, 0, st>>>(d, S->p, k);
                     const bool smp = sample_begin(k);
@@ -1461,7 +1557,7 @@ static int pcg_solve_impl(gip_pcg *S, const double *b, double *x, const double *
                 sample_read();
             }
         }
-        if (S->fold) {
+        if (S->rows) {
             if (k > 0) k_x_flush_rows<<<S->nchunk, kBlock, 0, st>>>(d, S->p, k - 1);
             k_fin_rows<<<1, kBlock, 0, st>>>(d);
         } else {

@@ -114,12 +114,15 @@ def test_dns_96_reference_sequence_vs_oracle(tmp_path):
         assert err <= 1e-10, (f, err)
 
 
-@pytest.mark.parametrize("n", [2 ** 19, 2 ** 19 + 1])
-def test_pcg_either_side_of_the_partial_fold(dev, n):
-    """2^19 rows: 2048 product chunks, read directly by the next kernels;
-    one row more: the chunk partials go through the fold kernel.  A shifted
-    1-D Laplacian: 40 fixed FP64 iterations and a tolerance stop against the
-    oracle, and a mixed-precision solve to a 1e-10 true residual."""
+@pytest.mark.parametrize("n", [2 ** 19, 2 ** 19 + 1, 2 ** 23, 2 ** 23 + 1])
+def test_pcg_either_side_of_the_iteration_switches(dev, n):
+    """2^19 rows: 2048 product chunks, read directly by the next kernels (two
+    kernels per FP64 iteration); one row more: the chunk partials go through
+    the fold kernel and the sweeps re-reduce them in their prologues (four
+    kernels); 2^23 + 1 rows: the row-per-thread sweeps with one-workgroup
+    alpha and beta kernels (seven).  A shifted 1-D Laplacian: 40 fixed FP64
+    iterations and a tolerance stop against the oracle, and a
+    mixed-precision solve to a 1e-10 true residual."""
     T = sp.diags([-1.0, 2.05, -1.0], [-1, 0, 1], shape=(n, n)).tocsr()
     T.sort_indices()
     b = np.random.default_rng(n).standard_normal(n)
