@@ -178,12 +178,12 @@ __global__ __launch_bounds__(kBlock) void k_b_cfl_fin(const double *part, Seg S,
     }
 }
 
-// The PCG iteration kernels (direction, product, update) instead run one
-// workgroup per chunk in dispatch order (grid = the chunk count: the pace of
-// a plain stream) and store one partial per wavefront, with no barrier
-// (seg_store_w); their finalize kernels sum an instance's wavefront
-// partials in chunk order (seg_total_w).  A chunk of an instance that has
-// left the PCG returns after two loads.
+// The PCG product and update kernels instead run one workgroup per chunk in
+// dispatch order (grid = the chunk count: the pace of a plain stream) and
+// store one partial per wavefront, with no barrier (seg_store_w); their
+// finalize kernels sum an instance's wavefront partials in chunk order
+// (seg_total_w).  A chunk of an instance that has left the PCG returns
+// after two loads.
 template <int K>
 __device__ inline void seg_store_w(double (&v)[K], int c, double *part) {
     const size_t w = (size_t)c * kWaves + (threadIdx.x >> 6);
@@ -202,6 +202,7 @@ __device__ inline void seg_total_w(const double *part, int inst, const Seg &S, d
     double acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = 0.0;
+#pragma unroll 8
     for (int j = threadIdx.x; j < nw; j += kBlock)
 #pragma unroll
         for (int k = 0; k < K; ++k) acc[k] = acc[k] + part[(w0 + j) * K + k];
@@ -300,14 +301,18 @@ __global__ __launch_bounds__(kBlock) void k_bp_setup_fin(BPcg d, Seg S, double t
 
 // direction of the running instances: p = 1.0 s + beta p (GIP_LinearSolver.h:113;
 // the first iteration has beta = 0 and p = 0 on the solved rows, so p = s)
+// (a grid-stride sweep: at 12-24 B per row a workgroup per chunk spends
+// more on its launch than on its rows)
 __global__ __launch_bounds__(kBlock) void k_bp_dir(BPcg d, Seg S, double *__restrict__ p) {
-    const int c = blockIdx.x;
-    const int inst = S.chunk_inst[c];
-    if (d.state[inst] != BS_RUN) return;
-    const int row = c * kChunk + threadIdx.x;
-    const double beta = bsc(d, inst)[B_BETA];
-    const double si = d.s[row], po = p[row];
-    p[row] = 1.0 * si + beta * po;
+    const double *__restrict__ s = d.s;
+    FOR_CHUNKS(S, c) {
+        const int inst = S.chunk_inst[c];
+        if (d.state[inst] != BS_RUN) continue;
+        const int row = c * kChunk + threadIdx.x;
+        const double beta = bsc(d, inst)[B_BETA];
+        const double si = s[row], po = p[row];
+        p[row] = 1.0 * si + beta * po;
+    }
 }
 
 // Ap = A p (one gather of p per neighbour), chunk partials of p.Ap
@@ -451,13 +456,15 @@ __global__ __launch_bounds__(kBlock) void k_bpm_setup_fin(BPcg d, Seg S, double 
 }
 
 __global__ __launch_bounds__(kBlock) void k_bpm_dir(BPcg d, Seg S, float *__restrict__ p) {
-    const int c = blockIdx.x;
-    const int inst = S.chunk_inst[c];
-    if (d.state[inst] != BS_RUN) return;
-    const int row = c * kChunk + threadIdx.x;
-    const float beta = (float)bsc(d, inst)[B_BETA];
-    const float si = d.s32[row], po = p[row];
-    p[row] = 1.0f * si + beta * po;
+    const float *__restrict__ s = d.s32;
+    FOR_CHUNKS(S, c) {
+        const int inst = S.chunk_inst[c];
+        if (d.state[inst] != BS_RUN) continue;
+        const int row = c * kChunk + threadIdx.x;
+        const float beta = (float)bsc(d, inst)[B_BETA];
+        const float si = s[row], po = p[row];
+        p[row] = 1.0f * si + beta * po;
+    }
 }
 
 template <int W>
@@ -820,13 +827,13 @@ int enqueue_iter(gip_batch *b) {
     const unsigned nch = (unsigned)b->seg_in.nchunk;  // one workgroup per chunk
     for (int k = 0; k < b->o.poll; ++k) {
         if (mixed) {
-            k_bpm_dir<<<nch, kBlock, 0, st>>>(d, b->seg_in, b->pq);
+            k_bpm_dir<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, b->pq);
             BDISPATCH(L, nch, k_bpm_spmv, view_of(L), d, b->seg_in, b->pq);
             k_bp_spmv_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in);
             k_bpm_update<<<nch, kBlock, 0, st>>>(d, b->seg_in, b->pq, 1);
             k_bpm_update_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in);
         } else {
-            k_bp_dir<<<nch, kBlock, 0, st>>>(d, b->seg_in, b->pp);
+            k_bp_dir<<<b->grid_in, kBlock, 0, st>>>(d, b->seg_in, b->pp);
             BDISPATCH(L, nch, k_bp_spmv, view_of(L), d, b->seg_in, b->pp);
             k_bp_spmv_fin<<<b->B, kBlock, 0, st>>>(d, b->seg_in);
             k_bp_update<<<nch, kBlock, 0, st>>>(d, b->seg_in, b->pp, 1);
@@ -1167,7 +1174,7 @@ int gip_batch_layouts(gip_batch *b, int32_t *format, int32_t *lanes) {
     if (!b || !format || !lanes) return set_error(GIP_EINVAL, "gip_batch_layouts: bad argument");
     for (int k = 0; k < OP_N; ++k) {
         format[k] = b->op[k]->format;
-        lanes[k] = b->op[k]->lanes;
+        lanes[k] = b->op[k]->format == GIP_FMT_ELL ? 1 << b->op[k]->ell_lg : b->op[k]->lanes;
     }
     return GIP_OK;
 }

@@ -28,7 +28,6 @@ int hip_fail(hipError_t e, const char *what);
 
 constexpr int kBlock = 256;        // threads per workgroup for streaming kernels
 constexpr int kMaxRedBlocks = 2048; // partial sums per reduction
-constexpr int kSlice = 64;          // ELL slice height = one wavefront
 
 // Device scalar slots of a context (doubles), used by host-returning reductions.
 enum { kSlotResult = 0, kSlotCount = 8 };
@@ -55,8 +54,10 @@ struct gip_mat {
     int32_t *d_col;
     double *d_val;
     float *d_val32;
-    // sliced ELL: slice s covers rows [64 s, 64 s + 64); entry k of row
-    // 64 s + l sits at slice_off[s] + 64 k + l; padding has col = -1.
+    // sliced ELL with H = 2^ell_lg rows per slice (32, 64 or 128): slice s
+    // covers rows [H s, H s + H); entry k of row H s + l sits at
+    // slice_off[s] + H k + l; padding has col = -1.
+    int32_t ell_lg;
     int32_t nslices;
     int32_t uniform_width; // > 0 when every slice has this width and no padding
     int64_t ell_stored;

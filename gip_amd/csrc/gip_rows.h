@@ -19,6 +19,7 @@ struct EllView {
     const StencilHdr *st_hdr;
     const StencilTail *st_tail;
     int st_w, tail_base, cmax;
+    int ell_lg;  // sliced ELL: log2 of the rows per slice
 };
 
 static EllView view(const gip_mat *A) {
@@ -37,6 +38,7 @@ static EllView view(const gip_mat *A) {
     v.st_w = A->st_w;
     v.tail_base = (int)A->st_tail_base;
     v.cmax = A->cols_needed > 0 ? A->cols_needed - 1 : 0;
+    v.ell_lg = A->ell_lg;
     return v;
 }
 
@@ -190,24 +192,27 @@ __device__ inline bool row_dot(const EllView &A, const V *__restrict__ val, cons
     } else if (W == 0 && A.fmt == GIP_FMT_STENCIL) {
         return stencil_row<0, T, V, X>(A, val, x, row, sum);
     } else if (W > 0) {
-        const int base = A.soff[row >> 6] + (row & 63);
+        const int lg = A.ell_lg;
+        const int base = A.soff[row >> lg] + (row & ((1 << lg) - 1));
         int c[W > 0 ? W : 1];
         V v[W > 0 ? W : 1];
 #pragma unroll
         for (int k = 0; k < W; ++k) {
-            c[k] = __builtin_nontemporal_load(&A.col[base + 64 * k]);
-            v[k] = __builtin_nontemporal_load(&val[base + 64 * k]);
+            c[k] = __builtin_nontemporal_load(&A.col[base + (k << lg)]);
+            v[k] = __builtin_nontemporal_load(&val[base + (k << lg)]);
         }
 #pragma unroll
         for (int k = 0; k < W; ++k) sum = sum + x.at(c[k]) * v[k];
         return true;
     } else if (A.fmt == GIP_FMT_ELL) {
-        const int s = row >> 6, base = A.soff[s] + (row & 63), width = (A.soff[s + 1] - A.soff[s]) >> 6;
+        const int lg = A.ell_lg;
+        const int s = row >> lg, base = A.soff[s] + (row & ((1 << lg) - 1));
+        const int width = (A.soff[s + 1] - A.soff[s]) >> lg;
         bool any = false;
         for (int k = 0; k < width; ++k) {
-            const int cc = A.col[base + 64 * k];
+            const int cc = A.col[base + (k << lg)];
             if (cc < 0) break;
-            sum = sum + x.at(cc) * val[base + 64 * k];
+            sum = sum + x.at(cc) * val[base + (k << lg)];
             any = true;
         }
         return any;

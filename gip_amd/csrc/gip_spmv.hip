@@ -6,10 +6,11 @@
 // starting from 0.0, and leaves empty rows untouched: results are bitwise
 // those of the CPU specification.
 //
-// Sliced ELL: slice s holds rows [64 s, 64 s + 64) (one wavefront); entry k
-// of row 64 s + l is stored at slice_off[s] + 64 k + l, so the 64 lanes of a
-// wavefront read 64 consecutive values and indices per entry.  Rows shorter
-// than the slice width are padded with column -1.
+// Sliced ELL: slice s holds rows [H s, H s + H), H = 2^lg = 32, 64 (one
+// wavefront, the default) or 128 rows; entry k of row H s + l is stored at
+// slice_off[s] + H k + l, so the lanes of a wavefront read consecutive
+// values and indices per entry.  Rows shorter than the slice width are
+// padded with column -1.
 //
 // Stencil: the rows of a slice share per-slot column offsets (see
 // gip_layout.h), so a slot stores only values; the wavefront reads 64
@@ -27,13 +28,13 @@
 namespace gip {
 
 template <int MODE, int W>
-__global__ __launch_bounds__(kBlock) void k_ell(int nrows, const int32_t *__restrict__ soff,
+__global__ __launch_bounds__(kBlock) void k_ell(int nrows, int lg, const int32_t *__restrict__ soff,
                                                const int32_t *__restrict__ col, const double *__restrict__ val,
                                                const double *__restrict__ x, double *__restrict__ y, double a,
                                                double b, const int32_t *__restrict__ scatter) {
     const int row = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (row >= nrows) return;
-    const int s = row >> 6, l = row & 63;
+    const int s = row >> lg, l = row & ((1 << lg) - 1);
     const int base = soff[s] + l;
     double sum = 0.0;
     bool any;
@@ -42,19 +43,19 @@ __global__ __launch_bounds__(kBlock) void k_ell(int nrows, const int32_t *__rest
         double v[W > 0 ? W : 1];
 #pragma unroll
         for (int k = 0; k < W; ++k) {
-            c[k] = __builtin_nontemporal_load(&col[base + 64 * k]);
-            v[k] = __builtin_nontemporal_load(&val[base + 64 * k]);
+            c[k] = __builtin_nontemporal_load(&col[base + (k << lg)]);
+            v[k] = __builtin_nontemporal_load(&val[base + (k << lg)]);
         }
 #pragma unroll
         for (int k = 0; k < W; ++k) sum = sum + x[c[k]] * v[k];
         any = true;
     } else {
-        const int width = (soff[s + 1] - soff[s]) >> 6;
+        const int width = (soff[s + 1] - soff[s]) >> lg;
         any = false;
         for (int k = 0; k < width; ++k) {
-            const int cc = col[base + 64 * k];
+            const int cc = col[base + (k << lg)];
             if (cc < 0) break;
-            sum = sum + x[cc] * val[base + 64 * k];
+            sum = sum + x[cc] * val[base + (k << lg)];
             any = true;
         }
     }
@@ -169,18 +170,18 @@ __global__ __launch_bounds__(kBlock) void k_diag(int nrows, const double *__rest
 
 // FP32 products for the mixed-precision inner solver (no empty-row rule needed:
 // rows without entries produce 0).
-__global__ __launch_bounds__(kBlock) void k_ell32(int nrows, const int32_t *__restrict__ soff,
+__global__ __launch_bounds__(kBlock) void k_ell32(int nrows, int lg, const int32_t *__restrict__ soff,
                                                  const int32_t *__restrict__ col, const float *__restrict__ val,
                                                  const float *__restrict__ x, float *__restrict__ y) {
     const int row = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (row >= nrows) return;
-    const int s = row >> 6, l = row & 63;
-    const int base = soff[s] + l, width = (soff[s + 1] - soff[s]) >> 6;
+    const int s = row >> lg, l = row & ((1 << lg) - 1);
+    const int base = soff[s] + l, width = (soff[s + 1] - soff[s]) >> lg;
     float sum = 0.0f;
     for (int k = 0; k < width; ++k) {
-        const int cc = col[base + 64 * k];
+        const int cc = col[base + (k << lg)];
         if (cc < 0) break;
-        sum = sum + x[cc] * val[base + 64 * k];
+        sum = sum + x[cc] * val[base + (k << lg)];
     }
     y[row] = sum;
 }
@@ -244,11 +245,11 @@ int spmv_launch(gip_ctx *c, const gip_mat *A, const double *x, double *y, double
 #define GIP_ELL_CASE(Wc)                                                                                        \
     case Wc:                                                                                                    \
         if (mode == 0)                                                                                          \
-            k_ell<0, Wc><<<g, kBlock, 0, st>>>(A->nrows, A->d_slice_off, A->d_ell_col, A->d_ell_val, x, y, a, b, \
-                                               scatter);                                                        \
+            k_ell<0, Wc><<<g, kBlock, 0, st>>>(A->nrows, A->ell_lg, A->d_slice_off, A->d_ell_col, A->d_ell_val, x,  \
+                                               y, a, b, scatter);                                               \
         else                                                                                                    \
-            k_ell<1, Wc><<<g, kBlock, 0, st>>>(A->nrows, A->d_slice_off, A->d_ell_col, A->d_ell_val, x, y, a, b, \
-                                               scatter);                                                        \
+            k_ell<1, Wc><<<g, kBlock, 0, st>>>(A->nrows, A->ell_lg, A->d_slice_off, A->d_ell_col, A->d_ell_val, x,  \
+                                               y, a, b, scatter);                                               \
         break;
         switch (A->uniform_width) {
             GIP_ELL_CASE(1)
@@ -261,11 +262,11 @@ int spmv_launch(gip_ctx *c, const gip_mat *A, const double *x, double *y, double
             GIP_ELL_CASE(8)
             default:
                 if (mode == 0)
-                    k_ell<0, 0><<<g, kBlock, 0, st>>>(A->nrows, A->d_slice_off, A->d_ell_col, A->d_ell_val, x, y, a,
-                                                     b, scatter);
+                    k_ell<0, 0><<<g, kBlock, 0, st>>>(A->nrows, A->ell_lg, A->d_slice_off, A->d_ell_col, A->d_ell_val,
+                                                     x, y, a, b, scatter);
                 else
-                    k_ell<1, 0><<<g, kBlock, 0, st>>>(A->nrows, A->d_slice_off, A->d_ell_col, A->d_ell_val, x, y, a,
-                                                     b, scatter);
+                    k_ell<1, 0><<<g, kBlock, 0, st>>>(A->nrows, A->ell_lg, A->d_slice_off, A->d_ell_col, A->d_ell_val,
+                                                     x, y, a, b, scatter);
         }
 #undef GIP_ELL_CASE
     } else if (A->format == GIP_FMT_STENCIL) {
@@ -327,7 +328,8 @@ int spmv32_launch(gip_ctx *c, const gip_mat *A, const float *x, float *y) {
     if (A->nrows == 0) return GIP_OK;
     if (A->format == GIP_FMT_ELL) {
         if (!A->d_ell_val32) return set_error(GIP_EINVAL, "operator has no FP32 values");
-        k_ell32<<<nblocks(A->nrows), kBlock, 0, c->stream>>>(A->nrows, A->d_slice_off, A->d_ell_col, A->d_ell_val32, x, y);
+        k_ell32<<<nblocks(A->nrows), kBlock, 0, c->stream>>>(A->nrows, A->ell_lg, A->d_slice_off, A->d_ell_col,
+                                                             A->d_ell_val32, x, y);
     } else if (A->format == GIP_FMT_STENCIL) {
         if (!A->d_ell_val32) return set_error(GIP_EINVAL, "operator has no FP32 values");
         const EllView v = view(A);
@@ -392,21 +394,24 @@ int gip_mat_create(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, const 
         if (!found) A->diag_missing = 1;
         if (is_diag && !(rowptr[i] == i && col[i] == i)) is_diag = false;
     }
-    // slice widths
-    int32_t ns = (nrows + kSlice - 1) / kSlice;
+    // slice widths (H rows per slice)
+    const int lg = (flags & GIP_MAT_ELL_SLICE32) ? 5 : (flags & GIP_MAT_ELL_SLICE128) ? 7 : 6;
+    const int32_t H = 1 << lg;
+    A->ell_lg = lg;
+    int32_t ns = (nrows + H - 1) / H;
     std::vector<int32_t> soff((size_t)ns + 1, 0);
     int32_t uw = -1;
     bool uniform = true;
     int64_t stored = 0;
     for (int32_t s = 0; s < ns; ++s) {
         int32_t w = 0;
-        for (int32_t r = s * kSlice; r < std::min(nrows, (s + 1) * kSlice); ++r) {
+        for (int32_t r = s * H; r < std::min(nrows, (s + 1) * H); ++r) {
             int32_t len = rowptr[r + 1] - rowptr[r];
             w = std::max(w, len);
             if (uw < 0) uw = len;
             if (len != uw) uniform = false;
         }
-        stored += (int64_t)w * kSlice;
+        stored += (int64_t)w * H;
         if (stored > INT32_MAX) uniform = false;
         soff[(size_t)s + 1] = (int32_t)std::min<int64_t>(stored, INT32_MAX);
     }
@@ -419,7 +424,7 @@ int gip_mat_create(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, const 
     int32_t fmt = want;
     if (fmt == GIP_FMT_STENCIL && !stencil_ok) fmt = GIP_FMT_AUTO;
     if (fmt == GIP_FMT_AUTO) {
-        const bool ell_ok = stored <= INT32_MAX && stored <= (int64_t)(1.2 * nnz) + kSlice * 8;
+        const bool ell_ok = stored <= INT32_MAX && stored <= (int64_t)(1.2 * nnz) + H * 8;
         const int64_t csr_bytes = 12 * (int64_t)nnz + 8 * (int64_t)nrows;
         const int64_t ell_bytes = ell_ok ? 12 * stored + 4 * (int64_t)ns : csr_bytes;
         if (is_diag && nrows > 0)
@@ -441,15 +446,15 @@ int gip_mat_create(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, const 
     if (!rc && fmt == GIP_FMT_ELL) {
         A->nslices = ns;
         A->ell_stored = stored;
-        A->uniform_width = (uniform && uw > 0 && uw <= 8 && (int64_t)uw * kSlice * ns == stored) ? uw : 0;
-        if (uniform && uw > 0 && uw <= 8 && nrows % kSlice != 0) A->uniform_width = uw;  // tail rows never run
+        A->uniform_width = (uniform && uw > 0 && uw <= 8 && (int64_t)uw * H * ns == stored) ? uw : 0;
+        if (uniform && uw > 0 && uw <= 8 && nrows % H != 0) A->uniform_width = uw;  // tail rows never run
         std::vector<int32_t> ecol((size_t)stored, -1);
         std::vector<double> evalv((size_t)stored, 0.0);
         std::vector<int32_t> c2e((size_t)nnz);
         for (int32_t r = 0; r < nrows; ++r) {
-            int32_t s = r / kSlice, l = r % kSlice;
+            int32_t s = r / H, l = r % H;
             for (int32_t j = rowptr[r]; j < rowptr[r + 1]; ++j) {
-                int64_t pos = soff[(size_t)s] + (int64_t)kSlice * (j - rowptr[r]) + l;
+                int64_t pos = soff[(size_t)s] + (int64_t)H * (j - rowptr[r]) + l;
                 ecol[(size_t)pos] = col[j];
                 evalv[(size_t)pos] = val[j];
                 c2e[(size_t)j] = (int32_t)pos;
@@ -571,9 +576,12 @@ int gip_mat_autotune(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, cons
                      gip_tune_report *rep) {
     if (!c || !out || !x || !y || reps < 1) return set_error(GIP_EINVAL, "gip_mat_autotune: bad argument");
     *out = nullptr;
-    struct Cand { int fmt, lanes; };
-    const Cand cands[6] = {{GIP_FMT_ELL, 1}, {GIP_FMT_CSR, 1}, {GIP_FMT_CSR, 2},
-                           {GIP_FMT_CSR, 4}, {GIP_FMT_CSR, 8}, {GIP_FMT_STENCIL, 1}};
+    struct Cand { int fmt, lanes, slice_flag; };
+    const Cand cands[8] = {{GIP_FMT_ELL, 1, GIP_MAT_ELL_SLICE32}, {GIP_FMT_ELL, 1, 0},
+                           {GIP_FMT_ELL, 1, GIP_MAT_ELL_SLICE128}, {GIP_FMT_CSR, 1, 0},
+                           {GIP_FMT_CSR, 2, 0}, {GIP_FMT_CSR, 4, 0},
+                           {GIP_FMT_CSR, 8, 0}, {GIP_FMT_STENCIL, 1, 0}};
+    const int32_t base_flags = flags & ~0xff & ~(GIP_MAT_ELL_SLICE32 | GIP_MAT_ELL_SLICE128);
     gip_tune_report r{};
     r.chosen_format = -1;
     float best = 0.0f;
@@ -582,20 +590,21 @@ int gip_mat_autotune(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, cons
     GIP_HIP_TRY(hipEventCreate(&e0));
     GIP_HIP_TRY(hipEventCreate(&e1));
     int rc = GIP_OK;
-    for (int k = 0; k < 6 && !rc; ++k) {
+    for (int k = 0; k < 8 && !rc; ++k) {
         r.us[k] = -1.0f;
         gip_mat *A = nullptr;
         if (cands[k].fmt == GIP_FMT_ELL || cands[k].fmt == GIP_FMT_STENCIL) {
-            rc = gip_mat_create(c, nrows, ncols, nnz, rowptr, col, val, cands[k].fmt | (flags & ~0xff), &A);
+            rc = gip_mat_create(c, nrows, ncols, nnz, rowptr, col, val, cands[k].fmt | base_flags | cands[k].slice_flag,
+                                &A);
             if (rc) break;
             // not a candidate: the layout does not fit, or ELL would pad heavily
             if (A->format != cands[k].fmt ||
-                (A->format == GIP_FMT_ELL && A->ell_stored > (int64_t)(1.2 * nnz) + kSlice * 8)) {
+                (A->format == GIP_FMT_ELL && A->ell_stored > (int64_t)(1.2 * nnz) + (8 << A->ell_lg))) {
                 gip_mat_destroy(A);
                 continue;
             }
         } else {
-            if (!csr) rc = gip_mat_create(c, nrows, ncols, nnz, rowptr, col, val, GIP_FMT_CSR | (flags & ~0xff), &csr);
+            if (!csr) rc = gip_mat_create(c, nrows, ncols, nnz, rowptr, col, val, GIP_FMT_CSR | base_flags, &csr);
             if (rc) break;
             A = csr;
             A->lanes = cands[k].lanes;
@@ -611,6 +620,7 @@ int gip_mat_autotune(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, cons
                 best = r.us[k];
                 r.chosen_format = cands[k].fmt;
                 r.chosen_lanes = cands[k].lanes;
+                r.chosen_slice = cands[k].fmt == GIP_FMT_ELL ? 1 << A->ell_lg : 0;
                 if (keep && keep != csr) gip_mat_destroy(keep);
                 keep = A;
             } else if (A != csr) {

@@ -17,6 +17,9 @@ from ._lib import POINTER, byref, c_double, c_float, c_int, c_int32, c_int64, c_
 FMT_AUTO, FMT_CSR, FMT_ELL, FMT_DIAG, FMT_STENCIL = 0, 1, 2, 3, 4
 FORMAT_NAMES = {FMT_CSR: "csr", FMT_ELL: "sliced-ell", FMT_DIAG: "diag", FMT_STENCIL: "stencil"}
 MAT_FP32 = 1 << 8
+MAT_ELL_SLICE32 = 1 << 9
+MAT_ELL_SLICE128 = 1 << 10
+ELL_SLICE_FLAGS = {32: MAT_ELL_SLICE32, 64: 0, 128: MAT_ELL_SLICE128}
 PCG_FP64, PCG_MIXED = 0, 1
 OK, EINVAL, EHIP, ENOMEM, EBREAKDOWN, ENOCONV = 0, -1, -2, -3, -4, -5
 
@@ -32,7 +35,8 @@ class PcgStats(ctypes.Structure):
 
 
 class TuneReport(ctypes.Structure):
-    _fields_ = [("chosen_format", c_int32), ("chosen_lanes", c_int32), ("us", c_float * 6)]
+    _fields_ = [("chosen_format", c_int32), ("chosen_lanes", c_int32), ("chosen_slice", c_int32),
+                ("us", c_float * 8)]
 
 
 # exported entry points and their argument types (kept in step with include/gip_hip.h)
@@ -191,7 +195,7 @@ class DeviceArray:
 class Matrix:
     """A device operator built from host CSR arrays (int32 rowptr/col, float64 val)."""
 
-    def __init__(self, dev: Device, rowptr, col, val, nrows=None, ncols=0, fmt=FMT_AUTO, fp32=False):
+    def __init__(self, dev: Device, rowptr, col, val, nrows=None, ncols=0, fmt=FMT_AUTO, fp32=False, slice=64):
         self.dev = dev
         rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
         col = np.ascontiguousarray(col, dtype=np.int32)
@@ -199,7 +203,9 @@ class Matrix:
         self.nrows = int(nrows if nrows is not None else rowptr.size - 1)
         self.nnz = int(val.size)
         self.ptr = c_void_p()
-        flags = fmt | (MAT_FP32 if fp32 else 0)
+        if slice not in ELL_SLICE_FLAGS:
+            raise ValueError("slice: rows per ELL slice, 32, 64 or 128")
+        flags = fmt | (MAT_FP32 if fp32 else 0) | ELL_SLICE_FLAGS[slice]
         check(lib().gip_mat_create(dev.ctx, self.nrows, int(ncols), self.nnz, rowptr.ctypes.data, col.ctypes.data,
                                    val.ctypes.data, flags, byref(self.ptr)))
         dev._adopt(self)
@@ -225,7 +231,8 @@ class Matrix:
                                      col.ctypes.data, val.ctypes.data, MAT_FP32 if fp32 else 0, x.ptr, y.ptr, reps,
                                      byref(self.ptr), ctypes.cast(byref(rep), c_void_p)))
         dev._adopt(self)
-        return self, {"format": rep.chosen_format, "lanes": rep.chosen_lanes, "us": list(rep.us)}
+        return self, {"format": rep.chosen_format, "lanes": rep.chosen_lanes, "slice": rep.chosen_slice,
+                      "us": list(rep.us)}
 
     def set_lanes(self, lanes):
         check(lib().gip_mat_set_lanes(self.ptr, lanes))

@@ -72,8 +72,9 @@ int gip_batch_phase_ms(gip_batch *b, float *ms);
 int gip_batch_instance(gip_batch *b, int32_t inst, gip_batch_info *info);
 /* Instance field in its own layout ("u","v","w","p": n_all values; "mf": n_faces). */
 int gip_batch_get_field(gip_batch *b, int32_t inst, const char *field, double *host, int32_t n);
-/* Chosen SpMV layout per operator (format, lanes) in the order
- * D, Ec, L, Lb, Mx, My, Mz, Gx, Gy, Gz, SFx, SFy, SFz, SO, Mres. */
+/* Chosen SpMV layout per operator in the order D, Ec, L, Lb, Mx, My, Mz,
+ * Gx, Gy, Gz, SFx, SFy, SFz, SO, Mres: format, and in `lanes` the CSR lanes
+ * per row, or for sliced ELL its rows per slice (32, 64 or 128). */
 int gip_batch_layouts(gip_batch *b, int32_t *format, int32_t *lanes);
 int gip_batch_destroy(gip_batch *b);
 

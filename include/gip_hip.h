@@ -91,6 +91,10 @@ int gip_event_destroy(gip_event *ev);
  * `ncols` is informational (columns may exceed it, as in the reference). */
 enum { GIP_FMT_AUTO = 0, GIP_FMT_CSR = 1, GIP_FMT_ELL = 2, GIP_FMT_DIAG = 3, GIP_FMT_STENCIL = 4 };
 enum { GIP_MAT_FP32 = 1 << 8 }; /* also keep an FP32 copy of the values (mixed precision) */
+/* Rows per sliced-ELL slice (default 64, one wavefront): 32 pads less on
+ * irregular rows (two slices per wavefront), 128 amortises the slice
+ * offsets; the autotuner tries all three. */
+enum { GIP_MAT_ELL_SLICE32 = 1 << 9, GIP_MAT_ELL_SLICE128 = 1 << 10 };
 int gip_mat_create(gip_ctx *ctx, int32_t nrows, int32_t ncols, int32_t nnz, const int32_t *rowptr, const int32_t *col,
                    const double *val, int32_t flags, gip_mat **A);
 int gip_mat_destroy(gip_mat *A);
@@ -112,15 +116,15 @@ int gip_mat_cols_needed(const gip_mat *A, int32_t *ncols);
  * stored order. */
 int gip_mat_set_lanes(gip_mat *A, int32_t lanes);
 /* In-project SpMV autotuner: builds the operator in each candidate layout
- * (sliced ELL, 64 rows per wavefront; CSR with 1, 2, 4 or 8 lanes per row;
- * stencil),
- * times `reps` products y = A x on the device (x, y: device vectors covering
- * the operator) and returns the fastest; the rest are freed.  `report`
- * (optional) gets each candidate's mean time in microseconds (-1: not a
- * candidate) in that order. */
+ * (sliced ELL with 32, 64 or 128 rows per slice; CSR with 1, 2, 4 or 8
+ * lanes per row; stencil), times `reps` products y = A x on the device
+ * (x, y: device vectors covering the operator) and returns the fastest; the
+ * rest are freed.  `report` (optional) gets each candidate's mean time in
+ * microseconds (-1: not a candidate) in that order, and for an ELL choice
+ * its rows per slice. */
 typedef struct gip_tune_report {
-    int32_t chosen_format, chosen_lanes;
-    float us[6];
+    int32_t chosen_format, chosen_lanes, chosen_slice;
+    float us[8];
 } gip_tune_report;
 int gip_mat_autotune(gip_ctx *ctx, int32_t nrows, int32_t ncols, int32_t nnz, const int32_t *rowptr, const int32_t *col,
                      const double *val, int32_t flags, const double *x, double *y, int32_t reps, gip_mat **A,

@@ -39,12 +39,12 @@ def _ncols(c, A):
     return max(c.n_all, c.n_faces, int(A.col.max()) + 1 if A.nnz else 1)
 
 
-def _check_op(dev, c, name, fmt=hip.FMT_AUTO, seed=0):
+def _check_op(dev, c, name, fmt=hip.FMT_AUTO, seed=0, slice=64):
     A = c.op(name)
     rng = np.random.default_rng(seed)
     x = rng.standard_normal(_ncols(c, A))
     y0 = rng.standard_normal(A.nrows)
-    M = hip.Matrix.from_csr(dev, A, fmt=fmt)
+    M = hip.Matrix.from_csr(dev, A, fmt=fmt, slice=slice)
     dx, dy = dev.to_device(x), dev.to_device(y0)
     M.spmv(dx, dy)
     got = dy.download()
@@ -83,6 +83,42 @@ def test_spmv_forced_layouts(dev, cav46, fmt):
             assert info["format"] == fmt, (name, info)
         else:
             assert info["format"] != hip.FMT_STENCIL, (name, info)
+
+
+@pytest.mark.parametrize("slice", [32, 128])
+def test_ell_slice_heights_bitwise(dev, cav46, slice):
+    """Sliced ELL with 32 or 128 rows per slice (64 is the default): every
+    product variant bitwise against the oracle, the FP32 product equal to the
+    64-row layout's, and the PCG iterates on L equal to the stencil
+    layout's (the solver reads ELL through the same row sum)."""
+    for name in ("L", "Ec", "Gx", "Lb", "SO"):
+        info = _check_op(dev, cav46, name, fmt=hip.FMT_ELL, slice=slice)
+        assert info["format"] == hip.FMT_ELL, (name, info)
+    A = cav46.op("Ec")
+    x = np.random.default_rng(7).standard_normal(_ncols(cav46, A)).astype(np.float32)
+    outs = []
+    for sl in (64, slice):
+        M = hip.Matrix.from_csr(dev, A, fmt=hip.FMT_ELL, fp32=True, slice=sl)
+        y32 = dev.array(A.nrows, np.float32)
+        M.spmv32(dev.to_device(x), y32)
+        outs.append(y32.download())
+    assert np.array_equal(outs[0], outs[1])
+    L = cav46.op("L")
+    n, nall = cav46.n_inner, cav46.n_all
+    b = np.zeros(nall)
+    b[:n] = cavity.rhs_normal(n, 5)
+    z = dev.to_device(np.zeros(nall))
+    xs = []
+    for fmt, sl in ((hip.FMT_STENCIL, 64), (hip.FMT_ELL, slice)):
+        M = hip.Matrix.from_csr(dev, L, fmt=fmt, slice=sl)
+        assert M.info()["format"] == fmt
+        solver = hip.PCG(M, nall)
+        dx = dev.array(nall)
+        r = solver.solve(dev.to_device(b), dx, z, maxit=30)
+        assert r.iters == 30 and r.status == 0
+        xs.append(dx.download())
+        solver.destroy()
+    assert np.array_equal(xs[0], xs[1])
 
 
 def _banded(n, offsets, seed, extra=0.02, empty=(), shuffle_rows=0):
@@ -159,9 +195,13 @@ def test_autotuner_picks_a_timed_candidate(dev, cav46):
         A = cav46.op(name)
         M, rep = hip.Matrix.autotune(dev, A, reps=3)
         timed = [u for u in rep["us"] if u > 0]
-        idx = {hip.FMT_ELL: 0, hip.FMT_STENCIL: 5}.get(rep["format"])
-        if idx is None:
-            idx = {1: 1, 2: 2, 4: 3, 8: 4}[rep["lanes"]]
+        # candidate order: ELL 32/64/128 rows per slice, CSR 1/2/4/8 lanes, stencil
+        if rep["format"] == hip.FMT_ELL:
+            idx = {32: 0, 64: 1, 128: 2}[rep["slice"]]
+        elif rep["format"] == hip.FMT_STENCIL:
+            idx = 7
+        else:
+            idx = {1: 3, 2: 4, 4: 5, 8: 6}[rep["lanes"]]
         assert timed and min(timed) == pytest.approx(rep["us"][idx], rel=1e-6)
         x = np.random.default_rng(0).standard_normal(_ncols(cav46, A))
         dy = dev.to_device(np.zeros(A.nrows))
@@ -193,13 +233,13 @@ def test_spmv_random_with_empty_rows(dev):
     A.data = rng.standard_normal(A.nnz)
     x = rng.standard_normal(777)
     y0 = rng.standard_normal(1000)
-    for fmt in (hip.FMT_AUTO, hip.FMT_CSR, hip.FMT_ELL):
-        M = hip.Matrix(dev, A.indptr, A.indices, A.data, ncols=777, fmt=fmt)
+    for fmt, sl in ((hip.FMT_AUTO, 64), (hip.FMT_CSR, 64), (hip.FMT_ELL, 32), (hip.FMT_ELL, 64), (hip.FMT_ELL, 128)):
+        M = hip.Matrix(dev, A.indptr, A.indices, A.data, ncols=777, fmt=fmt, slice=sl)
         dy = dev.to_device(y0)
         M.spmv(dev.to_device(x), dy)
         want = y0.copy()
         pyoracle.spmv(pyoracle.Csr(A.indptr, A.indices, A.data), x, want)
-        assert np.array_equal(dy.download(), want), fmt
+        assert np.array_equal(dy.download(), want), (fmt, sl)
 
 
 def test_spmv_into_values_rebuilds_convection(dev, cav46):
