@@ -1,14 +1,18 @@
 #include "GIP_PartitionHIP.h"
 
 #include <string>
+#include <thread>
 
 #include "GIP_ArchHIP.h"
 #include "../../BasicInterfaces/GIP_Topo.h"
 
 GIP_PartitionGroup::GIP_PartitionGroup(int nparts, int device)
-    : n_(nparts), device_(device), vals_((size_t)nparts, 0.0), topo_((size_t)nparts, nullptr),
-      send_off_((size_t)nparts + 1, 0), maps_((size_t)nparts) {
+    : n_(nparts), device_(device), topo_((size_t)nparts, nullptr), send_off_((size_t)nparts + 1, 0),
+      maps_((size_t)nparts) {
     if (nparts < 1) throw std::runtime_error("GIP_PartitionGroup: at least one partition");
+    vals_[0].assign((size_t)nparts, 0.0);
+    vals_[1].assign((size_t)nparts, 0.0);
+    parity_.assign((size_t)nparts, 0u);
     gip_check(gip_init(device, &ctx_), "gip_init (partition group)");
 }
 
@@ -22,33 +26,36 @@ GIP_PartitionGroup::~GIP_PartitionGroup() {
     gip_finalize(ctx_);
 }
 
-void GIP_PartitionGroup::abort() {
-    std::lock_guard<std::mutex> l(mu_);
-    aborted_ = true;
-    cv_.notify_all();
-}
+void GIP_PartitionGroup::abort() { aborted_.store(true, std::memory_order_release); }
 
+// Central counter with a generation number: the last partition to arrive
+// resets the counter and then advances the generation (release), which the
+// others wait for (acquire), so everything written before the rendezvous is
+// visible after it.
 void GIP_PartitionGroup::barrier() {
-    std::unique_lock<std::mutex> l(mu_);
-    if (aborted_) throw std::runtime_error("another partition failed");
-    const long gen = generation_;
-    if (++arrived_ == n_) {
-        arrived_ = 0;
-        ++generation_;
-        cv_.notify_all();
+    if (aborted_.load(std::memory_order_acquire)) throw std::runtime_error("another partition failed");
+    if (n_ == 1) return;
+    const unsigned gen = generation_.load(std::memory_order_acquire);
+    if (arrived_.fetch_add(1, std::memory_order_acq_rel) + 1 == n_) {
+        arrived_.store(0, std::memory_order_relaxed);
+        generation_.store(gen + 1, std::memory_order_release);
         return;
     }
-    cv_.wait(l, [&] { return generation_ != gen || aborted_; });
-    if (generation_ == gen) throw std::runtime_error("another partition failed");
+    for (unsigned polls = 0; generation_.load(std::memory_order_acquire) == gen; ++polls) {
+        if (aborted_.load(std::memory_order_acquire)) throw std::runtime_error("another partition failed");
+        if (polls > 4096) std::this_thread::yield();
+    }
 }
 
 double GIP_PartitionGroup::allreduce(int rank, double v, int op) {
     if (n_ == 1) return v;
-    vals_[(size_t)rank] = v;
+    std::vector<double> &vals = vals_[parity_[(size_t)rank]];
+    parity_[(size_t)rank] ^= 1u;
+    vals[(size_t)rank] = v;
     barrier();
-    double r = vals_[0];
+    double r = vals[0];
     for (int q = 1; q < n_; ++q) {
-        const double x = vals_[(size_t)q];
+        const double x = vals[(size_t)q];
         if (op == SUM)
             r = r + x;
         else if (op == MAX)
@@ -56,7 +63,6 @@ double GIP_PartitionGroup::allreduce(int rank, double v, int op) {
         else
             r = x < r ? x : r;
     }
-    barrier();  // nobody overwrites vals_ before every partition has read it
     return r;
 }
 
