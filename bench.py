@@ -222,7 +222,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--ns-steps", type=int, default=20)
+    ap.add_argument("--ns-steps", type=int, default=None,
+                    help="Navier-Stokes steps (default 20 in the default run, 2 in --suite ns256)")
     ap.add_argument("--ns-n", type=int, default=None)
     ap.add_argument("--seed", type=int, default=20240601)
     ap.add_argument("--cpu-iters", type=int, default=60)
@@ -338,19 +339,20 @@ def main():
     dev.sync()
 
     ns = None
-    if not args.skip_ns and args.ns_steps > 0:
+    ns_steps = 20 if args.ns_steps is None else args.ns_steps
+    if not args.skip_ns and ns_steps > 0:
         ns_n = args.ns_n or n
         t0 = time.perf_counter()
         full = cavity.generate(ns_n, gamma=1e-3)
         log(f"generated full {ns_n}^3 cavity operators in {time.perf_counter() - t0:.1f} s")
-        r64, f64 = ns_leg(dns, full, "fused", args.ns_steps, 1)
-        r32, f32 = ns_leg(dns, full, "mixed", args.ns_steps, 1)
+        r64, f64 = ns_leg(dns, full, "fused", ns_steps, 1)
+        r32, f32 = ns_leg(dns, full, "mixed", ns_steps, 1)
         # error of the mixed-precision run against the FP64 run, per field
         # (max abs difference over the inner cells / max abs FP64 value)
         N = full.n_inner
         err = {k: float(np.abs(f32[k][:N] - f64[k][:N]).max() / max(1e-300, np.abs(f64[k][:N]).max()))
                for k in f64}
-        ns = {"n": ns_n, "re": 1000, "steps": args.ns_steps, "fp64": r64, "mixed": r32,
+        ns = {"n": ns_n, "re": 1000, "steps": ns_steps, "fp64": r64, "mixed": r32,
               "mixed_vs_fp64_rel_err": err}
         del f64, f32
         log(f"ns steps/s fp64 {ns['fp64']['steps_per_s']:.3f}, mixed {ns['mixed']['steps_per_s']:.3f}")
@@ -922,7 +924,7 @@ def ns256_suite(args):
     from gip_amd import cavity, dns
     sys.path.insert(0, str(ROOT))
     from oracle import pyoracle
-    n, steps = args.n, args.ns_steps if args.ns_steps != 20 else 2
+    n, steps = args.n, 2 if args.ns_steps is None else args.ns_steps
     t0 = time.perf_counter()
     cav = cavity.generate(n, gamma=1e-3)
     N = cav.n_inner
