@@ -29,17 +29,14 @@ int hip_fail(hipError_t e, const char *what);
 constexpr int kBlock = 256;        // threads per workgroup for streaming kernels
 constexpr int kMaxRedBlocks = 2048; // partial sums per reduction
 
-// Device scalar slots of a context (doubles), used by host-returning reductions.
-enum { kSlotResult = 0, kSlotCount = 8 };
-
 }  // namespace gip
 
 struct gip_ctx {
     int device;
     hipStream_t stream;
     double *d_partials;     // kMaxRedBlocks * 4 doubles of reduction partials (one per workgroup)
-    double *d_scalars;      // kSlotCount device scalars
     double *h_result;       // pinned host landing slot for reduction results
+    double *h_result_dev;   // its device address: the finalize kernels store the result there
     int num_cu;
 };
 

@@ -318,25 +318,28 @@ __global__ __launch_bounds__(kBlock) void k_x_flush(PcgDev d, const double *__re
 // (with the direction and update sweeps in dispatch order too, 2% faster
 // per FP64 iteration at 256^3 than the XCD-contiguous order, which keeps
 // the neighbour gathers in one XCD's L2).  SMALL: XCD-contiguous order,
-// one partial per chunk; the state word is tested after the row's loads
+// one partial per chunk.  Both test the state word after the row's loads
 // instead of before them, taking its latency off the kernel's critical
-// path; a finished solve then still reads the operator, but stores nothing.
+// path (FP64 solve at 256^3: 1256 -> 1238 ms); a finished solve then still
+// reads the operator, but stores nothing.  (The same reordering in the
+// long direction and update sweeps made the solve slower: 1303 ms.)
 template <int W, bool SMALL>
 __device__ inline void iter_spmv(const EllView &A, const PcgDev &d, const double *__restrict__ p) {
     if constexpr (!SMALL) {
-        if (!running(d)) return;
+        const int st = state_load(d.state);  // tested after the row's loads, before any store
         const int c = blockIdx.x;
         const int row = c * kBlock + threadIdx.x;
-        double acc = 0.0;
+        double acc = 0.0, sum = 0.0, pr = 0.0;
+        bool any = false;
         if (row < d.n) {
             const SelfOperand<double> po{p, row, 0.0, false};
-            double *__restrict__ Ap = d.Ap;
-            double sum;
-            if (row_dot<W, double, double>(A, A.val, po, row, sum))
-                Ap[row] = sum;
-            else
-                sum = Ap[row];  // empty row: Ap keeps its value
-            const double pr = po.found ? po.self : p[row];
+            any = row_dot<W, double, double>(A, A.val, po, row, sum);
+            if (!any) sum = d.Ap[row];  // empty row: Ap keeps its value
+            pr = po.found ? po.self : p[row];
+        }
+        if (st != ST_RUN) return;
+        if (row < d.n) {
+            if (any) d.Ap[row] = sum;
             acc = pr * sum;
         }
         // one partial per wavefront: no barrier, each wavefront retires alone
@@ -811,13 +814,10 @@ __global__ __launch_bounds__(kBlock) void k_m_dir(PcgDev d, float *__restrict__ 
     const float a32 = pending ? (float)alpha_slot(d, k - 1) : 0.0f;  // alpha_{k-1}, as K2 of k-1 formed it
     const float beta = (float)(rho / rho_slot(d, k - 1));
     if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
-    // nontemporal accesses: each array is streamed once per sweep (mixed
-    // solve at 256^3: 403 -> 392 ms)
     FOR_ROWS(d.n, row) {
-        const float xo = __builtin_nontemporal_load(xc + row), si = __builtin_nontemporal_load(s + row),
-                    po = __builtin_nontemporal_load(p + row);
-        if (pending) __builtin_nontemporal_store(xo + a32 * po, xc + row);
-        __builtin_nontemporal_store(si + beta * po, p + row);
+        const float xo = xc[row], si = s[row], po = p[row];
+        if (pending) xc[row] = xo + a32 * po;
+        p[row] = si + beta * po;
     }
 }
 
@@ -994,9 +994,8 @@ __global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, int k) {
         for (int j = 0; j < kAzList; ++j) li[j] = j < naz ? li[j] : -1;
         FOR_ROWS(d.n, row) {
             // every operand first, so the row costs one memory round trip
-            // (nontemporal, as in the direction sweep)
-            const float ro = __builtin_nontemporal_load(r + row), apr = __builtin_nontemporal_load(Ap + row);
-            const float di = precond ? __builtin_nontemporal_load(dinv + row) : 0.0f;
+            const float ro = r[row], apr = Ap[row];
+            const float di = precond ? dinv[row] : 0.0f;
             float azr = 0.0f;
             if (dfl && !listed) {
                 azr = az[row];
@@ -1013,9 +1012,9 @@ __global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, int k) {
             }
             const float wr = dfl ? apr - c32 * azr : apr;  // (P A p)_row
             const float ri = ro - a32 * wr;
-            __builtin_nontemporal_store(ri, r + row);
+            r[row] = ri;
             const float si = precond ? ri * di : ri;
-            __builtin_nontemporal_store(si, s + row);
+            s[row] = si;
             rs = rs + (double)ri * (double)si;
             rr = rr + (double)ri * (double)ri;
         }

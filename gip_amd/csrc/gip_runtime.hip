@@ -60,9 +60,8 @@ int gip_init(int device, gip_ctx **out) {
     // default-priority, non-blocking stream
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->d_partials, sizeof(double) * kMaxRedBlocks * 4);
-    if (e == hipSuccess) e = hipMalloc(&c->d_scalars, sizeof(double) * kSlotCount);
     if (e == hipSuccess) e = hipHostMalloc(&c->h_result, sizeof(double) * 4, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipMemsetAsync(c->d_scalars, 0, sizeof(double) * kSlotCount, c->stream);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&c->h_result_dev, c->h_result, 0);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         int rc = hip_fail(e, "context setup");
@@ -83,7 +82,6 @@ int gip_finalize(gip_ctx *c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     if (c->d_partials) hipFree(c->d_partials);
-    if (c->d_scalars) hipFree(c->d_scalars);
     if (c->h_result) hipHostFree(c->h_result);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
