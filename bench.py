@@ -11,8 +11,10 @@ time in ms.  Reported beside it: the same solve with the FP64 solver, the
 FP64 SpMV HBM rate on the Laplacian (`roofline`), the FP32 product of the
 timed mixed solves (the solve's largest kernel, timed inside the solves:
 `roofline_solve`), the time-step rate of full Navier–Stokes steps at 256^3 (Re=1000, PCG to 1e-10,
-FP64 vs mixed), and the reference CPU path's PCG on the same system
-(`cpu_baseline`).  Inputs are resident in HBM before the timed region.
+FP64 vs mixed), the reference CPU path's PCG on the same system
+(`cpu_baseline`), and ten steps of the 64-cavity batch beside the same
+instances stepped one after another (`batch`).  Inputs are resident in HBM
+before the timed region.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--ns-steps 20]
 
@@ -229,6 +231,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=60)
     ap.add_argument("--skip-ns", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-batch", action="store_true", help="default run: leave out the 64-cavity batch leg")
     ap.add_argument("--suite", choices=("pcg", "batch", "adjoint", "clustered", "reference", "cpu", "ns256"),
                     default="pcg")
     ap.add_argument("--batch-eager", action="store_true",
@@ -359,6 +362,9 @@ def main():
         del full
 
     cpu = None if args.skip_cpu else cpu_baseline(n, args.cpu_iters, args.seed, fp64["iters"])
+    # the batched workload (64 uneven cavities), 10 timed steps, beside the
+    # same instances stepped one after another
+    batch = None if args.skip_batch else batch_workload(args, 11)
 
     # The dominant kernel of the path: the FP64 product k_iter_spmv inside
     # the FP64 PCG (its median launch, timed by events inside the solves),
@@ -417,6 +423,7 @@ def main():
         "roofline_solve": roofline_solve,
         "stream_ceiling_gbs": ceiling,
         "cpu_baseline": cpu,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
 
@@ -652,10 +659,17 @@ def batch_suite(args):
     integrator (one block-diagonal system, segmented kernels, hipGraph) vs
     the same instances stepped one after another with the single-cavity
     path.  Both solve the pressure to 1e-8 with the mixed-precision PCG."""
+    print(json.dumps(batch_workload(args, 50, sequential=not args.skip_sequential, eager=args.batch_eager)),
+          flush=True)
+
+
+def batch_workload(args, steps, sequential=True, eager=False):
+    """The 64-cavity batch for `steps` steps (the first untimed), and the
+    same instances stepped one after another when `sequential`."""
     from gip_amd import batch as gbatch
     from gip_amd import cavity, dns, hip
     rng = np.random.default_rng(args.seed)
-    B, steps = 64, 50
+    B = 64
     sizes = [int(v) for v in rng.integers(32, 97, B)]
     re = np.exp(rng.uniform(np.log(100.0), np.log(3200.0), B))
     t0 = time.perf_counter()
@@ -664,7 +678,7 @@ def batch_suite(args):
     log(f"generated {B} cavities ({cells} cells) in {time.perf_counter() - t0:.1f} s")
     dev = hip.Device(0)
     bt = gbatch.Batch(dev, cavs, 1.0 / re, pcg_tol=1e-8, pcg_maxit=20000, precision=hip.PCG_MIXED, poll=16,
-                      use_graph=not args.batch_eager, autotune=True)
+                      use_graph=not eager, autotune=True)
     bt.step()  # warm-up: graph capture
     dev.sync()
     phases = dict.fromkeys(bt.PHASES, 0.0)
@@ -682,7 +696,7 @@ def batch_suite(args):
     log(f"batch: {steps - 1} steps in {t_batch:.2f} s")
     t_seq, max_diff = float("nan"), float("nan")
     for i, (c, r) in enumerate(zip(cavs, re)):
-        if args.skip_sequential:
+        if not sequential:
             break
         if i == 0:
             t_seq, max_diff = 0.0, 0.0
@@ -700,18 +714,19 @@ def batch_suite(args):
         sim.close()
     log(f"sequential: {t_seq:.2f} s; max relative u difference batch vs sequential {max_diff:.2e}")
     inst_steps = B * (steps - 1)
-    print(json.dumps({
+    dev.close()
+    return {
         "metric": "batched cavities: instance-steps/s (64 uneven cavities, PCG to 1e-8, mixed precision)",
         "value": inst_steps / t_batch, "unit": "instance-steps/s", "n_gpus": 1, "steps": steps - 1, "warmup": 1,
         "higher_is_better": True,
-        "sequential_instance_steps_per_s": None if args.skip_sequential else inst_steps / t_seq,
-        "speedup_vs_sequential": None if args.skip_sequential else t_seq / t_batch,
-        "max_rel_u_diff_vs_sequential": None if args.skip_sequential else max_diff,
+        "sequential_instance_steps_per_s": inst_steps / t_seq if sequential else None,
+        "speedup_vs_sequential": t_seq / t_batch if sequential else None,
+        "max_rel_u_diff_vs_sequential": max_diff if sequential else None,
         "cells_total": cells, "sizes": sizes, "re": [round(float(v), 1) for v in re],
         "pcg_iters_last_step": [i["pcg_iters"] for i in bt_info],
-        "graph_replay": not args.batch_eager,
+        "graph_replay": not eager,
         "phase_ms_per_step": {k: v / (steps - 1) for k, v in phases.items()},
-        "data": "synthetic: generated uniform cavities, lid u=1, from rest"}), flush=True)
+        "data": "synthetic: generated uniform cavities, lid u=1, from rest"}
 
 
 def clustered_suite(args):
