@@ -481,11 +481,11 @@ def summary_traffic(args):
         "# iterations of the FP64 PCG, 8 of the mixed PCG, 5 copies of 2^27 doubles)",
         "#",
         "# commands (run from the repository root on one MI355X):",
-        f"#   rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -o run -- python3 -u bench.py "
+        f"#   rocprofv3 --pmc FETCH_SIZE --output-format csv -d out/pmc_fetch -o run -- python3 -u bench.py "
         f"--probe spmv --n {n}",
-        f"#   rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write -o run -- python3 -u bench.py "
+        f"#   rocprofv3 --pmc WRITE_SIZE --output-format csv -d out/pmc_write -o run -- python3 -u bench.py "
         f"--probe spmv --n {n}",
-        "#   python3 bench.py --summarize traffic --fetch gpurun_out/pmc_fetch --write gpurun_out/pmc_write "
+        "#   python3 bench.py --summarize traffic --fetch out/pmc_fetch --write out/pmc_write "
         f"--n {n}",
         "#",
         "# calibration: k_copy of 2^27 doubles (1 GiB read + 1 GiB written per launch, known):",
@@ -548,8 +548,8 @@ def summary_barrier(args):
         "# fenced with one double per workgroup stored before and all of them summed after.",
         "#",
         "# commands (run from the repository root on one MI355X):",
-        "#   python3 bench.py --probe barrier > gpurun_out/barrier.json",
-        "#   python3 bench.py --summarize barrier --json gpurun_out/barrier.json",
+        "#   python3 bench.py --probe barrier > out/barrier.json",
+        "#   python3 bench.py --summarize barrier --json out/barrier.json",
         "#",
         "# workgroups   relaxed_us   fenced_us   fenced_exchange_us"]
     for r in d["rows"]:
@@ -565,8 +565,8 @@ def summary_stream(args):
         "# (8 GiB of HBM per test, far beyond the 256 MB Infinity Cache), HIP events over 10 launches.",
         "#",
         "# commands (run from the repository root on one MI355X):",
-        "#   python3 bench.py --probe stream > gpurun_out/stream.json",
-        "#   python3 bench.py --summarize stream --json gpurun_out/stream.json",
+        "#   python3 bench.py --probe stream > out/stream.json",
+        "#   python3 bench.py --summarize stream --json out/stream.json",
         "#",
         "# mix    bytes/access  nontemporal  grid   ms/launch   GB/s",
     ]

@@ -55,8 +55,8 @@ mpic++ -include cstring -O3 -march=x86-64-v3 -c "$HERE/ref_harness.cpp" -fopenmp
 g++ "$OUT/o3/ref_harness.o" $O3 -o "$OUT/gip_ref_harness_O3" $LINK -L"$REPO/gip_amd" -lgip_cavity -Wl,-rpath,'$ORIGIN/../../gip_amd'
 # Drop-in check: the reference's unmodified templates and abstract/basic
 # interfaces instantiated with this repository's HIP specification, built by
-# the recipe INTEGRATION.md documents (run verbatim in a temporary copy of
-# the reference tree).
+# the maintainer's drop-in recipe (its marked blocks run verbatim in a
+# temporary copy of the reference tree).
 if [ -f "$REPO/gip_amd/libgip_hip.so" ]; then
     python3 "$HERE/dropin_build.py" "$REF" "$OUT/DrivenCavityHIP_reftemplates"
 fi

@@ -1,8 +1,9 @@
 """TEST INFRASTRUCTURE ONLY — the reference driver with the HIP specification
-plugged in, built by the recipe INTEGRATION.md documents (section 1).
+plugged in, built by the maintainer's drop-in recipe.
 
-The three recipe blocks (copy, patch, build) are read from INTEGRATION.md and
-run verbatim, with <repo> replaced by this repository's path, in a temporary
+The three recipe blocks (copy, patch, build; fenced shell blocks marked
+`<!-- recipe:copy -->` and so on in the integration guide) are read from
+the guide and run verbatim, with <repo> replaced by this repository's path, in a temporary
 copy of the reference tree (its src/ and DrivenCavity.cpp; the reference
 itself is never written).  The resulting executable is copied to `out`.
 
@@ -29,7 +30,7 @@ def recipe_blocks(doc: pathlib.Path = REPO / "INTEGRATION.md") -> dict:
         out[m.group(1)] = m.group(2).replace("<repo>", str(REPO))
     missing = {"copy", "patch", "build"} - set(out)
     if missing:
-        raise RuntimeError(f"INTEGRATION.md lacks recipe blocks {sorted(missing)}")
+        raise RuntimeError(f"{doc.name} lacks recipe blocks {sorted(missing)}")
     return out
 
 

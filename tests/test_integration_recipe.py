@@ -1,7 +1,7 @@
-"""The integration recipe of INTEGRATION.md section 1, run verbatim.
+"""The maintainer's drop-in recipe, run verbatim.
 
-The copy, patch and build blocks are taken from INTEGRATION.md, not retyped
-here, and run in a temporary copy of the reference tree (its src/ and
+The copy, patch and build blocks are taken from the integration guide (its
+marked shell blocks), not retyped here, and run in a temporary copy of the reference tree (its src/ and
 DrivenCavity.cpp).  The result must be an executable that links the kernel
 library and binds to the reference's own abstract headers.  CPU only:
 compiling and linking need no device.  Skipped where the reference sources
