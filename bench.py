@@ -465,8 +465,10 @@ def summarize_pmc(fetch_dir, write_dir, kernels):
     return out, f_scale, w_scale
 
 
-PCG_KERNELS = ("k_iter_dir_rows", "k_iter_spmv", "k_fold_two", "k_alpha_fin", "k_iter_update_rows", "k_beta_fin",
-               "k_m_dir", "k_m_spmv", "k_fold_dn", "k_m_update", "k_stencil<0")
+# kernel-name keys (substrings of the profiled names; "k_iter_dir(" is the
+# four-launch form's direction sweep, not k_iter_dir_rows)
+PCG_KERNELS = ("k_iter_dir_rows", "k_iter_dir(", "k_iter_spmv", "k_fold_two", "k_alpha_fin", "k_iter_update_rows",
+               "k_iter_update_grid", "k_beta_fin", "k_m_dir", "k_m_spmv", "k_fold_dn", "k_m_update", "k_stencil<0")
 
 
 def summary_traffic(args):
