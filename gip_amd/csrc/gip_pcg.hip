@@ -814,10 +814,14 @@ __global__ __launch_bounds__(kBlock) void k_m_dir(PcgDev d, float *__restrict__ 
     const float a32 = pending ? (float)alpha_slot(d, k - 1) : 0.0f;  // alpha_{k-1}, as K2 of k-1 formed it
     const float beta = (float)(rho / rho_slot(d, k - 1));
     if (leader()) d.sc[S_RHO0 + (k & 1)] = rho;
+    // nontemporal accesses: each array is streamed once per sweep (mixed
+    // solve at 256^3 over four runs: 392-394 ms with them, 403-405 without;
+    // the sweeps' own times are unchanged within noise)
     FOR_ROWS(d.n, row) {
-        const float xo = xc[row], si = s[row], po = p[row];
-        if (pending) xc[row] = xo + a32 * po;
-        p[row] = si + beta * po;
+        const float xo = __builtin_nontemporal_load(xc + row), si = __builtin_nontemporal_load(s + row),
+                    po = __builtin_nontemporal_load(p + row);
+        if (pending) __builtin_nontemporal_store(xo + a32 * po, xc + row);
+        __builtin_nontemporal_store(si + beta * po, p + row);
     }
 }
 
@@ -994,8 +998,8 @@ __global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, int k) {
         for (int j = 0; j < kAzList; ++j) li[j] = j < naz ? li[j] : -1;
         FOR_ROWS(d.n, row) {
             // every operand first, so the row costs one memory round trip
-            const float ro = r[row], apr = Ap[row];
-            const float di = precond ? dinv[row] : 0.0f;
+            const float ro = __builtin_nontemporal_load(r + row), apr = __builtin_nontemporal_load(Ap + row);
+            const float di = precond ? __builtin_nontemporal_load(dinv + row) : 0.0f;
             float azr = 0.0f;
             if (dfl && !listed) {
                 azr = az[row];
@@ -1012,9 +1016,9 @@ __global__ __launch_bounds__(kBlock) void k_m_update(PcgDev d, int k) {
             }
             const float wr = dfl ? apr - c32 * azr : apr;  // (P A p)_row
             const float ri = ro - a32 * wr;
-            r[row] = ri;
+            __builtin_nontemporal_store(ri, r + row);
             const float si = precond ? ri * di : ri;
-            s[row] = si;
+            __builtin_nontemporal_store(si, s + row);
             rs = rs + (double)ri * (double)si;
             rr = rr + (double)ri * (double)ri;
         }
