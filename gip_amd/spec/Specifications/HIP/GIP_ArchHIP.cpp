@@ -5,7 +5,30 @@
 
 #include "GIP_PartitionHIP.h"
 
-int GIP_ArchHIP::partCount() const { return group_ ? group_->size() : 1; }
+GIP_RankComm *(*gip_rank_comm_factory)() = nullptr;
+
+static_assert(int(GIP_RankComm::SUM) == int(GIP_PartitionGroup::SUM) &&
+                  int(GIP_RankComm::MAX) == int(GIP_PartitionGroup::MAX) &&
+                  int(GIP_RankComm::MIN) == int(GIP_PartitionGroup::MIN),
+              "one numbering of the reduction operations");
+
+int GIP_ArchHIP::partCount() const { return group_ ? group_->size() : ranks_ ? ranks_->size() : 1; }
+
+int GIP_ArchHIP::partRank() const { return group_ ? rank_ : ranks_ ? ranks_->rank() : 0; }
+
+double GIP_ArchHIP::allreduce(double v, int op) {
+    if (group_) return group_->allreduce(rank_, v, op);
+    if (ranks_) return ranks_->allreduce(v, op);
+    return v;
+}
+
+void GIP_ArchHIP::exchange(GIP_Topo *topo, double *dvec) {
+    if (!topo) return;
+    if (group_)
+        group_->exchange(rank_, topo, ctx(), dvec);
+    else if (ranks_)
+        ranks_->exchange(topo, ctx(), dvec);
+}
 
 void gip_check(int rc, const char *what) {
     if (rc != GIP_OK) {
@@ -17,6 +40,7 @@ void gip_check(int rc, const char *what) {
 GIP_ArchHIP::GIP_ArchHIP(int cpus, int cores, int devs) : GIP_Arch(cpus, cores, devs), devId(0) {}
 
 GIP_ArchHIP::~GIP_ArchHIP() {
+    delete ranks_;  // its device buffers belong to ctx_, still alive here
     if (ctx_) {
         for (auto &kv : pool_) gip_free(ctx_, kv.second);
         gip_finalize(ctx_);
@@ -29,8 +53,16 @@ void GIP_ArchHIP::postConstruct(int cpus, int cores, int devs) {
     num_devs = devs;
 }
 
+// Several processes: one per partition, each on device localRank mod the
+// device count (the reference takes rank mod the node's CPU count,
+// GIP_ArchGPU.cpp:3-8); several processes may share one device.
 void GIP_ArchHIP::setUp() {
     if (ctx_) return;
+    if (!group_ && gip_rank_comm_factory && (ranks_ = gip_rank_comm_factory())) {
+        int ndev = 0;
+        gip_check(gip_device_count(&ndev), "gip_device_count");
+        if (ndev > 0) devId = ranks_->localRank() % ndev;
+    }
     gip_check(gip_init(devId, &ctx_), "gip_init");
 }
 

@@ -4,6 +4,12 @@
 // scratch) every vector, operator and solver of this node launches on.
 // It also keeps a pool of device buffers so that the solver temporaries
 // (GIP_LinearSolver.h:54-61) do not allocate on every solve.
+//
+// A node is the only partition, one of a group of partitions stepped by
+// threads of this process (GIP_PartitionHIP.h), or one of several processes
+// (MPI ranks, GIP_RanksMPI.cpp).  allreduce() and exchange() are the
+// collectives the vectors and the CFL bound make across the partitions in
+// either case, and the identity for a lone partition.
 #ifndef _gip_archhip_
 #define _gip_archhip_
 
@@ -13,6 +19,27 @@
 
 #include "../../AbstractInterfaces/GIP_Arch.h"
 #include "gip_hip.h"
+
+class GIP_Topo;
+
+// Collectives across processes, one partition per process.
+class GIP_RankComm {
+   public:
+    enum { SUM = 0, MAX = 1, MIN = 2 };
+    virtual ~GIP_RankComm() {}
+    virtual int rank() const = 0;
+    virtual int size() const = 0;
+    virtual int localRank() const = 0;  // among the processes of this host
+    virtual double allreduce(double v, int op) = 0;
+    // refresh the halo of a device vector laid out by `topo`
+    virtual void exchange(GIP_Topo *topo, gip_ctx *ctx, double *dvec) = 0;
+};
+
+// Set, before main, by a translation unit that links MPI (GIP_RanksMPI.cpp,
+// compiled in the MPI build of this specification; the library build leaves
+// it null).  Returns the communicator of this process when more than one
+// process runs, else nullptr.
+extern GIP_RankComm *(*gip_rank_comm_factory)();
 
 class GIP_ArchHIP : public GIP_Arch {
    public:
@@ -36,8 +63,14 @@ class GIP_ArchHIP : public GIP_Arch {
         rank_ = rank;
     }
     class GIP_PartitionGroup *partition() { return group_; }
-    int partRank() const { return rank_; }
+    GIP_RankComm *ranks() { return ranks_; }
+    int partRank() const;
     int partCount() const;
+
+    // the partitions' combined value (op: GIP_RankComm::SUM, MAX or MIN)
+    double allreduce(double v, int op);
+    // refresh the halo of a device vector of this node laid out by `topo`
+    void exchange(GIP_Topo *topo, double *dvec);
 
     // device buffer pool
     void *acquire(size_t bytes);
@@ -51,6 +84,7 @@ class GIP_ArchHIP : public GIP_Arch {
     gip_ctx *ctx_ = nullptr;
     class GIP_PartitionGroup *group_ = nullptr;
     int rank_ = 0;
+    GIP_RankComm *ranks_ = nullptr;
     std::multimap<size_t, void *> pool_;
 };
 

@@ -3,7 +3,6 @@
 #include <cstdio>
 #include <cstdlib>
 
-#include "GIP_PartitionHIP.h"
 
 double GIP_NonLinearHIP::getCFL(GIP_Topo *topo, GIP_Vector *u, GIP_Vector *v, GIP_Vector *w, GIP_Vector *dxs,
                                 GIP_Vector *, double gamma, double rho) {
@@ -20,7 +19,5 @@ double GIP_NonLinearHIP::getCFL(GIP_Topo *topo, GIP_Vector *u, GIP_Vector *v, GI
     // the smallest bound over the partitions (MPI_Allreduce MIN,
     // GIP_NonLinearMulticore.cpp:45): the local value is 0.8 min(...) capped
     // at 0.1, and the cap commutes with the minimum
-    GIP_ArchHIP *node = GIP_ArchHIP::of(u->getMyArch());
-    if (GIP_PartitionGroup *g = node->partition()) dt = g->allreduce(node->partRank(), dt, GIP_PartitionGroup::MIN);
-    return dt;
+    return GIP_ArchHIP::of(u->getMyArch())->allreduce(dt, GIP_RankComm::MIN);
 }

@@ -6,7 +6,6 @@
 #include <stdexcept>
 #include <string>
 
-#include "GIP_PartitionHIP.h"
 #include "gip_cavity.h"
 
 GIP_VectorHIP::GIP_VectorHIP(int n, GIP_Arch *arch) : GIP_Vector(n, arch) { allocDevice(); }
@@ -89,18 +88,14 @@ void GIP_VectorHIP::axpy(GIP_Vector *x, double alpha, double beta, enum RUN_DOMA
 double GIP_VectorHIP::dot(GIP_Vector *x, GIP_Vector *, enum RUN_DOMAIN rdom) {
     double r = 0.0;
     gip_check(gip_ddot(ctx(), covered(x, range(rdom)), dvec, x->getDevicePtr(), &r), "dot");
-    GIP_ArchHIP *node = GIP_ArchHIP::of(myArch);
-    if (GIP_PartitionGroup *g = node->partition()) r = g->allreduce(node->partRank(), r, GIP_PartitionGroup::SUM);
-    return r;
+    return GIP_ArchHIP::of(myArch)->allreduce(r, GIP_RankComm::SUM);
 }
 
 // (MPI_Allreduce MAX in GIP_VectorMulticore.cpp:161)
 double GIP_VectorHIP::norm(GIP_Vector *, enum RUN_DOMAIN rdom) {
     double r = 0.0;
     gip_check(gip_dnrm_inf(ctx(), range(rdom), dvec, &r), "norm");
-    GIP_ArchHIP *node = GIP_ArchHIP::of(myArch);
-    if (GIP_PartitionGroup *g = node->partition()) r = g->allreduce(node->partRank(), r, GIP_PartitionGroup::MAX);
-    return r;
+    return GIP_ArchHIP::of(myArch)->allreduce(r, GIP_RankComm::MAX);
 }
 
 void GIP_VectorHIP::copyTo(GIP_Vector *x, enum RUN_DOMAIN rdom) {
@@ -109,14 +104,9 @@ void GIP_VectorHIP::copyTo(GIP_Vector *x, enum RUN_DOMAIN rdom) {
 }
 
 // the halo refresh of GIP_VectorMulticore.cpp:197-216 (GIP_Topo.cpp:175-181):
-// device pack and unpack kernels among the partitions of one device; a
-// lone partition has no halo
-void GIP_VectorHIP::update() {
-    GIP_ArchHIP *node = GIP_ArchHIP::of(myArch);
-    GIP_PartitionGroup *g = node->partition();
-    if (!g || !myTopo) return;
-    g->exchange(node->partRank(), myTopo, ctx(), dvec);
-}
+// device pack and unpack kernels, among the partitions of one device or
+// staged through the host between processes; a lone partition has no halo
+void GIP_VectorHIP::update() { GIP_ArchHIP::of(myArch)->exchange(myTopo, dvec); }
 
 void GIP_VectorHIP::operator=(int value) { gip_check(gip_dfill(ctx(), size, dvec, (double)value), "fill"); }
 

@@ -5,14 +5,18 @@ unmerged, with the halo refreshed by device pack/unpack kernels
 over the partitions, against the reference binary's 4-rank run
 (tests/golden/ref_c100k_4p.json, made with mpirun -np 4) and the README
 lines; a generated cavity written in 4 partitions against the oracle on the
-whole case.
+whole case; and the drop-in build of the reference's own driver run as the
+reference runs its shipped case, one MPI process per partition.
 
 Bar: per-step time, dt and massr within 1e-12 of the 4-rank reference run
 (only the order in which partial sums are combined differs), the README
 lines to their printed digits, the merged fields' norms and samples within
-1e-12; the generated case within 1e-10 of the oracle.
+1e-12; the generated case within 1e-10 of the oracle; the MPI run's printed
+lines at their six printed digits.
 """
 import json
+import pathlib
+import subprocess
 
 import numpy as np
 import pytest
@@ -20,7 +24,7 @@ import pytest
 from gip_amd import cavity, dns
 from oracle import pyoracle
 
-from .conftest import GOLDEN
+from .conftest import GOLDEN, ROOT
 
 pytestmark = pytest.mark.gpu
 
@@ -82,3 +86,31 @@ def test_partitions_need_the_reference_solver_sequence(tmp_path, c100k_root):
     h = dns.c_void_p()
     rc = dns.lib().gip_dns_open_parts(str(c100k_root).encode(), b"C100K", 4, dns.byref(cfg), dns.byref(h))
     assert rc != 0 and b"reference" in dns.lib().gip_dns_last_error()
+
+
+def test_reference_driver_with_four_mpi_ranks(c100k_root):
+    """The drop-in build (the reference's unmodified DrivenCavity.cpp and
+    templates over the HIP specification, oracle/_ref/DrivenCavityHIP_reftemplates)
+    under mpiexec -n 4 on the shipped 4-partition C100K files, as the
+    reference runs it (README.md:62-68): four processes share the device,
+    the halos are staged through the host between them and the reductions
+    are MPI_Allreduce (Specifications/HIP/GIP_RanksMPI.cpp).  Rank 0 prints
+    the ten lines; each value is the six-digit rounding of the 4-rank
+    reference run's and equals the README line."""
+    exe = ROOT / "oracle" / "_ref" / "DrivenCavityHIP_reftemplates"
+    mpiexec = pathlib.Path("/opt/conda/bin/mpiexec")
+    if not exe.exists() or not mpiexec.exists():
+        pytest.skip("drop-in build or mpiexec absent")
+    ref = json.loads((GOLDEN / "ref_c100k_4p.json").read_text())
+    readme = pyoracle.parse_lines((GOLDEN / "readme_c100k_np4.txt").read_text())
+    print("[drop-in, mpiexec -n 4] C100K_4p", flush=True)
+    out = subprocess.run([str(mpiexec), "-launcher", "fork", "-n", "4", str(exe), "C100K"], cwd=c100k_root,
+                         capture_output=True, text=True, timeout=180)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    got = pyoracle.parse_lines(out.stdout)
+    assert len(got) == 10, out.stdout
+    for g, w, line in zip(got, ref["steps_out"], readme):
+        for key in ("time", "dt", "massr"):
+            assert abs(g[key] - w[key]) <= 5.01e-6 * abs(w[key]), (w["step"], key, g[key], w[key])
+            assert f"{g[key]:.6e}" == f"{line[key]:.6e}", (key, g[key], line[key])
+    print(f"[drop-in, mpiexec -n 4] last line {got[-1]}", flush=True)
