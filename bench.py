@@ -325,19 +325,21 @@ def main():
     # A1 comes from its list of nonzeros, one for this pinned Laplacian, and
     # is not streamed)
     it_mixed = spmv32_bytes(info["nnz"], N, NA) + 44 * N
-    for leg, byts in ((fp64, it_fp64), (mixed, it_mixed)):
-        ms_it = leg["ms"] / max(1, leg["iters"])
-        leg["ms_per_iter"] = ms_it
-        leg["iter_alg_bytes"] = byts
-        leg["iter_gbs"] = byts / (ms_it * 1e-3) / 1e9
-    log(f"fp64 pcg {fp64['ms']:.1f} ms ({fp64['iters']} it, true {fp64['true_relres']:.2e}); "
-        f"mixed {mixed['ms']:.1f} ms ({mixed['iters']} it, true {mixed['true_relres']:.2e}); "
-        f"speedup {fp64['ms'] / mixed['ms']:.2f}x")
     # bytes one product moves in the operator's own layout (stencil: slot
     # values, slot offsets, per-slice tail records and tail entries, the
     # input once, the output once): no column index per entry, so fewer
     # than the CSR count
     lay64, lay32 = A.product_bytes(), A.product_bytes(fp32=True)
+    for leg, byts, lay in ((fp64, it_fp64, lay64 + 88 * N), (mixed, it_mixed, lay32 + 44 * N)):
+        ms_it = leg["ms"] / max(1, leg["iters"])
+        leg["ms_per_iter"] = ms_it
+        leg["iter_alg_bytes"] = byts  # CSR count of the product + the vector streams
+        leg["iter_gbs"] = byts / (ms_it * 1e-3) / 1e9
+        leg["iter_layout_bytes"] = lay  # the layout's own bytes + the vector streams
+        leg["iter_layout_gbs"] = lay / (ms_it * 1e-3) / 1e9
+    log(f"fp64 pcg {fp64['ms']:.1f} ms ({fp64['iters']} it, true {fp64['true_relres']:.2e}); "
+        f"mixed {mixed['ms']:.1f} ms ({mixed['iters']} it, true {mixed['true_relres']:.2e}); "
+        f"speedup {fp64['ms'] / mixed['ms']:.2f}x")
     A.destroy()
     dev.sync()
 
@@ -416,8 +418,13 @@ def main():
         "mixed_iteration_ratio": round(fp64["iters"] / max(1, mixed["iters"]), 3),
         "pcg_fp64": fp64,
         "pcg_mixed": mixed,
-        "spmv_fp64": {"ms": spmv_ms, "gbs": spmv_gbs, "algorithmic_bytes": alg},
-        "spmv_fp32": {"ms": spmv32_ms, "gbs": alg32 / (spmv32_ms * 1e-3) / 1e9, "algorithmic_bytes": alg32},
+        # FP64: the CSR byte count of the metric, and the layout's own bytes;
+        # FP32: the layout's own bytes (its CSR count includes column indices
+        # the layout never reads, so that rate is only a CSR equivalent)
+        "spmv_fp64": {"ms": spmv_ms, "gbs": spmv_gbs, "algorithmic_bytes": alg,
+                      "layout_gbs": lay64 / (spmv_ms * 1e-3) / 1e9, "layout_bytes": lay64},
+        "spmv_fp32": {"ms": spmv32_ms, "gbs": lay32 / (spmv32_ms * 1e-3) / 1e9, "layout_bytes": lay32,
+                      "csr_equiv_gbs": alg32 / (spmv32_ms * 1e-3) / 1e9, "csr_bytes": alg32},
         "ns_steps": ns,
         "roofline": roofline,
         "roofline_solve": roofline_solve,
