@@ -330,7 +330,11 @@ def main():
     # input once, the output once): no column index per entry, so fewer
     # than the CSR count
     lay64, lay32 = A.product_bytes(), A.product_bytes(fp32=True)
-    for leg, byts, lay in ((fp64, it_fp64, lay64 + 88 * N), (mixed, it_mixed, lay32 + 44 * N)):
+    # the FP64 solve multiplies a bitwise symmetric stencil operator through
+    # its symmetric form (the slots of offsets >= 0): its own bytes
+    sym_state, sym64 = A.sym_state()
+    own64 = sym64 if sym_state == 1 else lay64
+    for leg, byts, lay in ((fp64, it_fp64, own64 + 88 * N), (mixed, it_mixed, lay32 + 44 * N)):
         ms_it = leg["ms"] / max(1, leg["iters"])
         leg["ms_per_iter"] = ms_it
         leg["iter_alg_bytes"] = byts  # CSR count of the product + the vector streams
@@ -368,21 +372,32 @@ def main():
     # same instances stepped one after another
     batch = None if args.skip_batch else batch_workload(args, 11)
 
-    # The dominant kernel of the path: the FP64 product k_iter_spmv inside
-    # the FP64 PCG (its median launch, timed by events inside the solves),
-    # achieved = the CSR algorithmic bytes 12 nnz + 4 (rows+1) + 8 rows +
-    # 8 cols per launch; the layout's own bytes beside it.
+    # The dominant kernel of the path: the FP64 product inside the FP64 PCG
+    # (its median launch, timed by events inside the solves), achieved = the
+    # bytes the product's own layout moves per launch (over the symmetric
+    # form: the values of the slots of offsets >= 0, slot offsets and codes,
+    # tails, the input and output once).  The CSR count 12 nnz + 4 (rows+1)
+    # + 8 rows + 8 cols per launch beside it is only a CSR equivalent: the
+    # layout moves fewer bytes, so that rate can exceed the peak.
     us64, us32 = fp64["product_us"], mixed["product_us"]
     # operators of at most 2^19 rows form the direction inside the product
-    k64, k32 = ("k_iter_dirprod", "k_m_dirprod") if info["nrows"] <= 1 << 19 else ("k_iter_spmv", "k_m_spmv")
+    if info["nrows"] <= 1 << 19:
+        k64, k32, form = "k_iter_dirprod", "k_m_dirprod", "stencil layout"
+    elif sym_state == 1:
+        k64, k32, form = "k_iter_spmv_sym", "k_m_spmv", "symmetric form of the stencil layout"
+    else:
+        k64, k32, form = "k_iter_spmv<", "k_m_spmv", "stencil layout"
     ceiling = stream_ceiling()
-    ach64 = alg / (us64 * 1e-6) / 1e9
+    ach64 = own64 / (us64 * 1e-6) / 1e9
+    traffic64 = measured_traffic(n, k64)
     roofline = {
         "bound": "hbm", "achieved": round(ach64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(ach64 / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n, k64),
-        "algorithmic_bytes": alg, "us_per_launch": round(us64, 2),
-        "layout_bytes": lay64, "frac_layout": round(lay64 / (us64 * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
-        "kernel": f"{k64} (FP64 stencil-layout product of the FP64 PCG, timed inside the solves), {n}^3"}
+        "frac": round(ach64 / HBM_PEAK_GBS, 4), "traffic": traffic64,
+        "algorithmic_bytes": own64, "us_per_launch": round(us64, 2),
+        "frac_traffic": round(traffic64 / (us64 * 1e-6) / 1e9 / HBM_PEAK_GBS, 4) if traffic64 else None,
+        "csr_bytes": alg, "csr_equiv_gbs": round(alg / (us64 * 1e-6) / 1e9, 1), "full_layout_bytes": lay64,
+        "kernel": f"{k64.rstrip('<')} (FP64 product of the FP64 PCG over the {form}, timed inside the solves), "
+                  f"{n}^3"}
     # The dominant kernel of the timed mixed solve, its FP32 product: achieved
     # = the layout's bytes (the FP32 CSR count includes column indices this
     # layout never reads, and would put the rate above the peak).
@@ -474,7 +489,8 @@ def summarize_pmc(fetch_dir, write_dir, kernels):
 
 # kernel-name keys (substrings of the profiled names; "k_iter_dir(" is the
 # four-launch form's direction sweep, not k_iter_dir_rows)
-PCG_KERNELS = ("k_iter_dir_rows", "k_iter_dir(", "k_iter_spmv", "k_fold_two", "k_alpha_fin", "k_iter_update_rows",
+PCG_KERNELS = ("k_iter_dir_rows", "k_iter_dir(", "k_iter_spmv<", "k_iter_spmv_sym", "k_fold_two", "k_alpha_fin",
+               "k_iter_update_rows",
                "k_iter_update_grid", "k_beta_fin", "k_m_dir", "k_m_spmv", "k_fold_dn", "k_m_update", "k_stencil<0")
 
 
@@ -487,7 +503,8 @@ def summary_traffic(args):
     lines = [
         f"# HBM traffic per launch of the {n}^3 PCG kernels (rocprofv3 PMC: TCC FETCH_SIZE and WRITE_SIZE in KiB per",
         "# dispatch, median over the dispatches of `bench.py --probe spmv`: 5 standalone FP64 products, 8",
-        "# iterations of the FP64 PCG, 8 of the mixed PCG, 5 copies of 2^27 doubles)",
+        "# iterations of the FP64 PCG, 8 of the mixed PCG, 8 of the FP64 PCG without the symmetric form, 5 copies",
+        "# of 2^27 doubles)",
         "#",
         "# commands (run from the repository root on one MI355X):",
         f"#   rocprofv3 --pmc FETCH_SIZE --output-format csv -d out/pmc_fetch -o run -- python3 -u bench.py "
@@ -629,7 +646,8 @@ def probe_stream(args):
 
 def probe_spmv(args):
     """Five FP64 SpMVs on the n^3 Laplacian, eight iterations of the FP64 and
-    of the mixed PCG on it (their in-solve products), and five copies of 2^27
+    of the mixed PCG on it (their in-solve products), eight of the FP64 PCG
+    without the symmetric form, and five copies of 2^27
     doubles (1 GiB read + 1 GiB written each): the launches a rocprofv3 --pmc
     run counts, the copy calibrating FETCH_SIZE/WRITE_SIZE against known
     bytes."""
@@ -643,7 +661,9 @@ def probe_spmv(args):
     y = dev.array(cav.n_all)
     for _ in range(5):
         A.spmv(x, y)
-    # 8 iterations of each PCG (their products: k_iter_spmv, k_m_spmv)
+    # 8 iterations of each PCG (their products: k_iter_spmv_sym over the
+    # symmetric form, k_m_spmv), and 8 of the FP64 PCG on the operator
+    # built without the symmetric form (k_iter_spmv)
     A32 = hip.Matrix.from_csr(dev, L, fp32=True)
     solver = hip.PCG(A32, cav.n_all)
     b = dev.to_device(np.concatenate([cavity.rhs_normal(cav.n_inner, 1), np.zeros(cav.n_all - cav.n_inner)]))
@@ -652,6 +672,11 @@ def probe_spmv(args):
         solver.solve(b, y, z, maxit=8, tol=1e-10, precision=prec, poll=32)
     solver.destroy()
     A32.destroy()
+    F = hip.Matrix.from_csr(dev, L, sym=False)
+    solver = hip.PCG(F, cav.n_all)
+    solver.solve(b, y, z, maxit=8, tol=1e-10, poll=32)
+    solver.destroy()
+    F.destroy()
     nc = 1 << 27
     a, b = dev.array(nc), dev.array(nc)
     hip.fill(a, 1.0)

@@ -76,6 +76,20 @@ struct gip_mat {
     gip::StencilHdr *d_st_hdr;    // nslices (read for kind-2 slices only)
     gip::StencilTail *d_st_tail;  // nslices
     int32_t *d_st_tcol;    // st_tail_nnz
+    // symmetric form of the slot values (gip_layout.h, plan_symmetric), used
+    // by the FP64 PCG's product: sym_nu arrays of nslices * 64 values (0: the
+    // layout has none), sym_d the paired offsets, made and checked on the
+    // device from the slot values at the first solve after a change of the
+    // values (val_version)
+    int32_t sym_nu;
+    int32_t sym_d[gip::kSymMaxD];
+    int64_t sym_own_slots;  // slots of offsets >= 0 over all slices
+    int8_t *d_st_code;  // nslices * 8 slot codes
+    uint64_t val_version;
+    double *d_sym_val;
+    int32_t *d_sym_flag;
+    uint64_t sym_version;  // val_version + 1 the symmetric values were made from (0: never)
+    int32_t sym_ok;        // 1: in use, -1: the values are not symmetric
     // a_ii per row (the stored values of a DIAG operator), for Jacobi
     double *d_diag;
     int32_t diag_missing;
@@ -87,6 +101,10 @@ struct gip_mat {
 
 namespace gip {
 int reduce_grid(int n);
+// Makes (once per version of the values) and checks the symmetric form of
+// a stencil operator's slot values; *ok is true when the PCG's product may
+// read them.  Allocates on first use.
+int mat_sym_prepare(gip_ctx *c, gip_mat *A, bool *ok);
 // Looks up the no-alloc PCG scratch array's address on `device` (current);
 // called once per device by gip_init.
 int pcg_scratch_setup(int device);

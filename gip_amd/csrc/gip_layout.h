@@ -55,6 +55,7 @@ static_assert(sizeof(StencilTail) == 16, "slice tail record layout");
 struct StencilPlan {
     int32_t ns = 0, w = 0;
     std::vector<int32_t> off;     // ns * w offsets
+    std::vector<int8_t> nsel;     // per slice: slots in use (the rest: offset 0, value 0)
     std::vector<StencilHdr> hdr;  // ns slice headers
     std::vector<StencilTail> tail;  // ns compact tail records
     std::vector<int32_t> tcol;    // tail columns
@@ -104,9 +105,12 @@ inline bool plan_stencil(int32_t nrows, int32_t nnz, const int32_t *rowptr, cons
     P.ns = ns;
     P.w = w;
     P.off.assign((size_t)ns * w, 0);
-    for (int32_t s = 0; s < ns; ++s)
+    P.nsel.assign((size_t)ns, 0);
+    for (int32_t s = 0; s < ns; ++s) {
+        P.nsel[(size_t)s] = (int8_t)nsel[(size_t)s];
         for (int32_t k = 0; k < nsel[(size_t)s]; ++k)
             P.off[(size_t)s * w + k] = sel[(size_t)s * kStencilMaxW + k];  // unused slots: offset 0, value 0
+    }
     P.tail_base = (int64_t)ns * w * kStencilSlice;
     P.hdr.assign((size_t)ns, StencilHdr{});
     P.tcol.clear();
@@ -165,6 +169,72 @@ inline bool plan_stencil(int32_t nrows, int32_t nnz, const int32_t *rowptr, cons
     if (P.stored > INT32_MAX) return false;
     for (size_t t = 0; t < tail_entry.size(); ++t) P.pos[(size_t)tail_entry[t]] = (int32_t)(P.tail_base + (int64_t)t);
     return true;
+}
+
+// Symmetric form of the slot values.  For an operator whose entries satisfy
+// a_{r,r+d} == a_{r+d,r} bitwise, the value of a slot of negative offset -d
+// in row r is the value of the slot of offset +d in row r - d, so only the
+// slots of offsets >= 0 are stored.  The paired offsets D = {0} and every
+// d with -d a slot offset somewhere (ascending, at most kSymMaxD) have one
+// array each, indexed by offset: array j holds, for every row r,
+// a_{r,r+D[j]}, or 0 when slice r/64 has no slot of that offset.  A
+// slice's other slots of offsets >= 0 (a ghost column's, say) take the
+// arrays after them, numbered within the slice.  Slot k of slice s reads
+// code[s 8 + k]: j < 16 (own row, array j), 16 + j (row r - D[j], array j;
+// 0 when r - D[j] < 0) or -1 (an unused slot: 0).  Whether the values
+// really are symmetric is checked on the device against the full slot
+// values before the symmetric form is used (and again after every change
+// of the values); this plan only fixes the structure.
+constexpr int kSymMaxD = 4;   // paired offsets
+constexpr int kSymMaxU = 8;   // stored arrays
+constexpr int kSymLower = 16;  // codes from kSymLower on: a slot of negative offset
+
+struct SymPlan {
+    int32_t nd = 0, nu = 0;
+    int32_t d[kSymMaxD] = {};
+    int64_t own_slots = 0;     // slots of offsets >= 0 over all slices (the values a product streams)
+    std::vector<int8_t> code;  // ns * 8
+};
+
+inline bool plan_symmetric(const StencilPlan &P, SymPlan &Y) {
+    Y = SymPlan{};
+    if (P.w <= 0 || P.w > 8 || (int64_t)P.nsel.size() != P.ns) return false;
+    std::vector<int32_t> ds{0};
+    for (int32_t s = 0; s < P.ns; ++s)
+        for (int32_t k = 0; k < P.nsel[(size_t)s]; ++k) {
+            const int32_t o = P.off[(size_t)s * P.w + k];
+            if (o == INT32_MIN) return false;
+            if (o < 0) ds.push_back(-o);
+        }
+    std::sort(ds.begin(), ds.end());
+    ds.erase(std::unique(ds.begin(), ds.end()), ds.end());
+    if (ds.size() > (size_t)kSymMaxD) return false;
+    Y.nd = (int32_t)ds.size();
+    for (int32_t j = 0; j < Y.nd; ++j) Y.d[j] = ds[(size_t)j];
+    Y.code.assign((size_t)P.ns * 8, (int8_t)-1);
+    int32_t nextra = 0;
+    for (int32_t s = 0; s < P.ns; ++s) {
+        int32_t e = 0;
+        for (int32_t k = 0; k < P.nsel[(size_t)s]; ++k) {
+            const int32_t o = P.off[(size_t)s * P.w + k];
+            const int32_t a = o < 0 ? -o : o;
+            const auto it = std::lower_bound(ds.begin(), ds.end(), a);
+            const bool paired = it != ds.end() && *it == a;
+            const int32_t j = (int32_t)(it - ds.begin());
+            int32_t cd;
+            if (o < 0)
+                cd = kSymLower + j;
+            else if (paired)
+                cd = j;
+            else
+                cd = Y.nd + e++;
+            if (o >= 0) ++Y.own_slots;
+            Y.code[(size_t)s * 8 + k] = (int8_t)cd;
+        }
+        nextra = std::max(nextra, e);
+    }
+    Y.nu = Y.nd + nextra;
+    return Y.nu <= kSymMaxU;
 }
 
 // Position of tail entry `rank` of lane `lane` in the slice's tail

@@ -323,7 +323,8 @@ __global__ __launch_bounds__(kBlock) void k_x_flush(PcgDev d, const double *__re
 // path (FP64 solve at 256^3: 1256 -> 1238 ms); a finished solve then still
 // reads the operator, but stores nothing.  (The same reordering in the
 // long direction and update sweeps made the solve slower: 1303 ms.)
-template <int W, bool SMALL>
+// SYM: the slot values from the operator's symmetric form (stencil layout).
+template <int W, bool SMALL, bool SYM = false>
 __device__ inline void iter_spmv(const EllView &A, const PcgDev &d, const double *__restrict__ p) {
     if constexpr (!SMALL) {
         const int st = state_load(d.state);  // tested after the row's loads, before any store
@@ -333,7 +334,10 @@ __device__ inline void iter_spmv(const EllView &A, const PcgDev &d, const double
         bool any = false;
         if (row < d.n) {
             const SelfOperand<double> po{p, row, 0.0, false};
-            any = row_dot<W, double, double>(A, A.val, po, row, sum);
+            if constexpr (SYM)
+                any = row_dot_sym<W, double, double>(A, A.val, A.sym, po, row, sum);
+            else
+                any = row_dot<W, double, double>(A, A.val, po, row, sum);
             if (!any) sum = d.Ap[row];  // empty row: Ap keeps its value
             pr = po.found ? po.self : p[row];
         }
@@ -373,6 +377,11 @@ __global__ __launch_bounds__(kBlock) void k_iter_spmv(EllView A, PcgDev d, const
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_iter_spmv_small(EllView A, PcgDev d, const double *__restrict__ p) {
     iter_spmv<W, true>(A, d, p);
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_iter_spmv_sym(EllView A, PcgDev d, const double *__restrict__ p) {
+    iter_spmv<W, false, true>(A, d, p);
 }
 
 // The direction p_k = 1.0 s + beta p_{k-1} read at any column j of the
@@ -889,7 +898,9 @@ __global__ __launch_bounds__(kBlock) void k_m_dirprod(EllView A, PcgDev d, const
     }
 }
 
-// K1 in FP32 (SMALL as in iter_spmv)
+// K1 in FP32 (SMALL as in iter_spmv; the symmetric form of the slot values
+// measured slower here, 121-131 against 118.5 us at 256^3, so the FP32
+// product keeps the full slot values)
 template <int W, bool SMALL>
 __device__ inline void m_spmv(const EllView &A, const PcgDev &d, const float *__restrict__ p) {
     if constexpr (!SMALL) {
@@ -1291,6 +1302,14 @@ static int pcg_grid(int n) {
         }                                                                       \
     } while (0)
 #define DISPATCH_W(A, KERNEL, ...) DISPATCH_WG(A, S->grid, KERNEL, __VA_ARGS__)
+// the products over the symmetric form: one workgroup per chunk
+#define SYM_LAUNCH(A, KERNEL, ...)                                                    \
+    do {                                                                              \
+        if (kernel_width(A) == -7)                                                    \
+            KERNEL<-7><<<S->nchunk, kBlock, 0, st>>>(__VA_ARGS__);                    \
+        else                                                                          \
+            KERNEL<-8><<<S->nchunk, kBlock, 0, st>>>(__VA_ARGS__);                    \
+    } while (0)
 
 static PcgDev make_dev(gip_pcg *S, const double *b, double *x, int precond) {
     PcgDev d;
@@ -1474,6 +1493,14 @@ static int pcg_solve_impl(gip_pcg *S, const double *b, double *x, const double *
         int rc = coarse_setup(S);
         if (rc) return rc;
     }
+    // the symmetric form of a 7- or 8-slot stencil operator for the FP64
+    // products of the long iterations (made and checked here after a change
+    // of values; the no-alloc solve never has those iterations)
+    bool sym = false;
+    if (!mixed && S->fold && (kernel_width(A) == -7 || kernel_width(A) == -8)) {
+        int rc = mat_sym_prepare(c, const_cast<gip_mat *>(A), &sym);
+        if (rc) return rc;
+    }
     PcgDev d = make_dev(S, b, x, o->precond ? 1 : 0);
     EllView v = view(A);
     const int pollk = o->poll > 0 ? o->poll : 16;
@@ -1531,7 +1558,10 @@ static int pcg_solve_impl(gip_pcg *S, const double *b, double *x, const double *
                 if (S->fold && !S->rows) {
                     if (k > 0) k_iter_dir<<<S->grid, kBlock, 0, st>>>(d, S->p, k);
                     const bool smp = sample_begin(k);
-                    DISPATCH_WG(A, S->nchunk, k_iter_spmv, v, d, S->p);
+                    if (sym)
+                        SYM_LAUNCH(A, k_iter_spmv_sym, v, d, S->p);
+                    else
+                        DISPATCH_WG(A, S->nchunk, k_iter_spmv, v, d, S->p);
                     sample_end(smp);
                     k_fold_dn<<<S->gdn, kBlock, 0, st>>>(d);
                     k_iter_update_grid<<<S->grid, kBlock, 0, st>>>(d, k);
@@ -1539,7 +1569,10 @@ static int pcg_solve_impl(gip_pcg *S, const double *b, double *x, const double *
                     // (partials: p.Ap in chunk, then r.s in chunk and r.r in chunk_s)
                     if (k > 0) k_iter_dir_rows<<<S->nchunk, kBlock, 0, st>>>(d, S->p, k);
                     const bool smp = sample_begin(k);
-                    DISPATCH_WG(A, S->nchunk, k_iter_spmv, v, d, S->p);
+                    if (sym)
+                        SYM_LAUNCH(A, k_iter_spmv_sym, v, d, S->p);
+                    else
+                        DISPATCH_WG(A, S->nchunk, k_iter_spmv, v, d, S->p);
                     sample_end(smp);
                     k_fold_two<<<S->gdn, kBlock, 0, st>>>(d, d.chunk, nullptr, P_DN, P_DN);
                     k_alpha_fin<<<1, kBlock, 0, st>>>(d, k);

@@ -20,6 +20,11 @@ struct EllView {
     const StencilTail *st_tail;
     int st_w, tail_base, cmax;
     int ell_lg;  // sliced ELL: log2 of the rows per slice
+    // symmetric form of the slot values (null when not in use)
+    const int8_t *st_code;
+    const double *sym;
+    int sym_nu, sym_stride;  // arrays of sym_stride values (every slice's rows), one after another
+    int sym_d[kSymMaxD];
 };
 
 static EllView view(const gip_mat *A) {
@@ -39,6 +44,11 @@ static EllView view(const gip_mat *A) {
     v.tail_base = (int)A->st_tail_base;
     v.cmax = A->cols_needed > 0 ? A->cols_needed - 1 : 0;
     v.ell_lg = A->ell_lg;
+    v.st_code = A->d_st_code;
+    v.sym = A->sym_ok > 0 ? A->d_sym_val : nullptr;
+    v.sym_nu = A->sym_nu;
+    v.sym_stride = A->nslices * 64;
+    for (int j = 0; j < kSymMaxD; ++j) v.sym_d[j] = A->sym_d[j];
     return v;
 }
 
@@ -129,6 +139,59 @@ __device__ inline bool stencil_row(const EllView &A, const V *__restrict__ val, 
         return stencil_tail_hdr(A, val, x, s, row, sum);
     }
     return true;
+}
+
+// Stencil-layout row sum over the symmetric form of the slot values
+// (gip_layout.h, plan_symmetric): the same values in the same order as
+// stencil_row, a slot of offset -d taking its value from row - d's slot of
+// offset +d.  The slice's 8 slot codes are one scalar load.
+template <int SW, class T, class V, class X>
+__device__ inline bool stencil_row_sym(const EllView &A, const V *__restrict__ val, const V *__restrict__ sym,
+                                       const X &x, int row, T &sum) {
+    const int s = __builtin_amdgcn_readfirstlane(row >> 6);
+    const int lane = row & 63;
+    const size_t stride = (size_t)A.sym_stride;
+    const int32_t *off = A.st_off + (size_t)s * SW;
+    const uint64_t codes = *reinterpret_cast<const uint64_t *>(A.st_code + (size_t)s * 8);
+    int c[SW];
+    V v[SW];
+#pragma unroll
+    for (int k = 0; k < SW; ++k) {
+        const int cc = row + off[k];
+        c[k] = cc < 0 ? 0 : (cc > A.cmax ? A.cmax : cc);
+        // array and row of the slot's value from the (scalar) code, so that
+        // every slot is one unconditional load, issued before any sum
+        const int code = (int)(int8_t)(codes >> (8 * k));
+        const bool lower = code >= kSymLower;
+        const int j = lower ? code - kSymLower : (code < 0 ? 0 : code);
+        const int d = !lower ? 0 : j == 0 ? A.sym_d[0] : j == 1 ? A.sym_d[1] : j == 2 ? A.sym_d[2] : A.sym_d[3];
+        const int pr = row - d, prc = pr < 0 ? 0 : pr;
+        const V u = sym[(size_t)j * stride + prc];
+        v[k] = (code < 0 || pr < 0) ? V(0) : u;
+    }
+#pragma unroll
+    for (int k = 0; k < SW; ++k) sum = sum + x.at(c[k]) * v[k];
+    const StencilTail *tl = A.st_tail + s;
+    const int kind = tl->kind;
+    if (kind == 1) {
+        const uint64_t bit = 1ull << lane;
+        const uint64_t m0 = tl->m0;
+        if (m0 & bit) {
+            const int e = tl->tsoff + __popcll(m0 & (bit - 1ull));
+            sum = sum + x.at(A.st_tcol[e]) * val[A.tail_base + e];
+        }
+    } else if (kind == 2) {
+        return stencil_tail_hdr(A, val, x, s, row, sum);
+    }
+    return true;
+}
+
+template <int W, class T, class V, class X>
+__device__ inline bool row_dot_sym(const EllView &A, const V *__restrict__ val, const V *__restrict__ sym,
+                                   const X &x, int row, T &sum) {
+    static_assert(W < 0, "the symmetric form is a stencil layout");
+    sum = T(0);
+    return stencil_row_sym<-W, T, V, X>(A, val, sym, x, row, sum);
 }
 
 // The operator's part of a stencil row (slot columns and values, and for a

@@ -220,6 +220,53 @@ __global__ __launch_bounds__(kBlock) void k_to_f32(int64_t n, const double *__re
 
 static inline int nblocks(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
+// The symmetric form's arrays from the slot values: array j (values
+// [j rows_pad, (j+1) rows_pad)) holds at row r the value of the row's slot
+// of code j, 0 when its slice has none.  One thread per row, padding rows
+// of the last slice included.
+__global__ __launch_bounds__(kBlock) void k_sym_make(int64_t rows_pad, int w, int nu, const int8_t *__restrict__ code,
+                                                     const double *__restrict__ val, double *__restrict__ sym) {
+    const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (r >= rows_pad) return;
+    const int64_t sl = r >> 6;
+    const int lane = (int)(r & 63);
+    const uint64_t codes = *reinterpret_cast<const uint64_t *>(code + sl * 8);
+    for (int j = 0; j < nu; ++j) {
+        double v = 0.0;
+        for (int k = 0; k < w; ++k)
+            if ((int)(int8_t)(codes >> (8 * k)) == j) v = val[(sl * w + k) * 64 + lane];
+        sym[j * rows_pad + r] = v;
+    }
+}
+
+// Every slot the symmetric form derives must hold the bits it derives: a
+// slot of offset -d_j the value of array j at row - d_j (0 before row 0),
+// an unused slot +0.  Any mismatch raises the flag.
+__global__ __launch_bounds__(kBlock) void k_sym_check(int nrows, int64_t rows_pad, int w, int d0, int d1, int d2, int d3,
+                                                      const int8_t *__restrict__ code, const double *__restrict__ val,
+                                                      const double *__restrict__ sym, int32_t *__restrict__ flag) {
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= nrows) return;
+    const int64_t sl = r >> 6;
+    const int lane = r & 63;
+    const uint64_t codes = *reinterpret_cast<const uint64_t *>(code + sl * 8);
+    bool bad = false;
+    for (int k = 0; k < w; ++k) {
+        const int cd = (int)(int8_t)(codes >> (8 * k));
+        if (cd >= 0 && cd < kSymLower) continue;  // stored as is
+        double want = 0.0;
+        if (cd >= kSymLower) {
+            const int j = cd - kSymLower;
+            const int d = j == 0 ? d0 : j == 1 ? d1 : j == 2 ? d2 : d3;
+            const int pr = r - d;
+            if (pr >= 0) want = sym[j * rows_pad + pr];
+        }
+        const double got = val[(sl * w + k) * 64 + lane];
+        if (__double_as_longlong(got) != __double_as_longlong(want)) bad = true;
+    }
+    if (bad) atomicOr(flag, 1);
+}
+
 template <class T>
 static int upload(gip_ctx *c, T **dptr, const T *host, size_t count) {
     GIP_HIP_TRY(hipMalloc(dptr, sizeof(T) * (count ? count : 1)));
@@ -230,7 +277,8 @@ static int upload(gip_ctx *c, T **dptr, const T *host, size_t count) {
 static void free_mat_arrays(gip_mat *A) {
     void *ptrs[] = {A->d_rowptr,   A->d_col,     A->d_val,      A->d_val32,    A->d_slice_off, A->d_ell_col,
                     A->d_ell_val,  A->d_ell_val32, A->d_csr2ell, A->d_diag,     A->d_tsrc,      A->d_tdst,
-                    A->d_st_off,   A->d_st_hdr,   A->d_st_tail,  A->d_st_tcol};
+                    A->d_st_off,   A->d_st_hdr,   A->d_st_tail,  A->d_st_tcol,  A->d_st_code,   A->d_sym_val,
+                    A->d_sym_flag};
     for (void *p : ptrs)
         if (p) hipFree(p);
 }
@@ -346,6 +394,39 @@ int spmv32_launch(gip_ctx *c, const gip_mat *A, const float *x, float *y) {
         return set_error(GIP_EINVAL, "FP32 SpMV needs an ELL, stencil or CSR operator");
     }
     GIP_LAUNCH_CHECK("spmv32 kernel");
+    return GIP_OK;
+}
+
+int mat_sym_prepare(gip_ctx *c, gip_mat *A, bool *ok) {
+    *ok = false;
+    if (A->format != GIP_FMT_STENCIL || A->sym_nu <= 0 || !A->d_st_code || A->nrows <= 0) return GIP_OK;
+    if (A->sym_version == A->val_version + 1) {
+        *ok = A->sym_ok > 0;
+        return GIP_OK;
+    }
+    const int64_t rows_pad = (int64_t)A->nslices * 64, n = rows_pad * A->sym_nu;
+    if (!A->d_sym_val) GIP_HIP_TRY(hipMalloc(&A->d_sym_val, sizeof(double) * n));
+    if (!A->d_sym_flag) GIP_HIP_TRY(hipMalloc(&A->d_sym_flag, sizeof(int32_t)));
+    k_sym_make<<<nblocks(rows_pad), kBlock, 0, c->stream>>>(rows_pad, A->st_w, A->sym_nu, A->d_st_code, A->d_ell_val,
+                                                           A->d_sym_val);
+    GIP_HIP_TRY(hipMemsetAsync(A->d_sym_flag, 0, sizeof(int32_t), c->stream));
+    k_sym_check<<<nblocks(A->nrows), kBlock, 0, c->stream>>>(A->nrows, rows_pad, A->st_w, A->sym_d[0], A->sym_d[1],
+                                                            A->sym_d[2], A->sym_d[3], A->d_st_code, A->d_ell_val,
+                                                            A->d_sym_val, A->d_sym_flag);
+    GIP_LAUNCH_CHECK("symmetric slot values");
+    int32_t flag = 1;
+    GIP_HIP_TRY(hipMemcpyAsync(&flag, A->d_sym_flag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    GIP_HIP_TRY(hipStreamSynchronize(c->stream));
+    A->sym_version = A->val_version + 1;
+    if (flag) {
+        // not symmetric: the arrays go until the values change again
+        GIP_HIP_TRY(hipFree(A->d_sym_val));
+        A->d_sym_val = nullptr;
+        A->sym_ok = -1;
+        return GIP_OK;
+    }
+    A->sym_ok = 1;
+    *ok = true;
     return GIP_OK;
 }
 
@@ -492,6 +573,13 @@ int gip_mat_create(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, const 
         if (!rc) rc = upload(c, &A->d_st_tail, plan.tail.data(), plan.tail.size());
         if (!rc) rc = upload(c, &A->d_st_tcol, plan.tcol.data(), plan.tcol.size());
         if (!rc) rc = upload(c, &A->d_col, col, (size_t)nnz);
+        SymPlan sym;
+        if (!rc && !(flags & GIP_MAT_NO_SYM) && plan_symmetric(plan, sym)) {
+            rc = upload(c, &A->d_st_code, sym.code.data(), sym.code.size());
+            A->sym_nu = sym.nu;
+            A->sym_own_slots = sym.own_slots;
+            for (int j = 0; j < kSymMaxD; ++j) A->sym_d[j] = sym.d[j];
+        }
         if (!rc) {
             // the host arrays above are uploaded asynchronously: wait before they go
             hipError_t e = hipStreamSynchronize(c->stream);
@@ -568,6 +656,22 @@ int gip_mat_product_bytes(const gip_mat *A, int32_t fp32, int64_t *bytes) {
             b = nnz * (vb + 4) + 4 * (rows + 1) + vecs;
     }
     *bytes = b;
+    return GIP_OK;
+}
+
+int gip_mat_sym_state(const gip_mat *A, int32_t *state, int64_t *bytes) {
+    if (!A || !state) return set_error(GIP_EINVAL, "gip_mat_sym_state: bad argument");
+    const bool current = A->sym_version == A->val_version + 1;
+    *state = A->format != GIP_FMT_STENCIL || A->sym_nu <= 0 ? 0 : current ? A->sym_ok : 2;
+    // the symmetric product's own bytes: the values of the slots of offsets
+    // >= 0, the slot offsets and codes, the tail records, values and
+    // columns, the input once, the output once (the values of the slots of
+    // negative offset are re-reads of the same arrays)
+    if (bytes)
+        *bytes = *state == 0 ? 0
+                             : A->sym_own_slots * 64 * 8 +
+                                   (int64_t)A->nslices * (4 * A->st_w + 8 + (int64_t)sizeof(StencilTail)) +
+                                   (int64_t)A->st_tail_nnz * 12 + 8 * ((int64_t)A->cols_needed + A->nrows);
     return GIP_OK;
 }
 
@@ -692,6 +796,7 @@ int gip_mat_set_values(gip_mat *A, const double *host) {
     if (!A || (A->nnz && !host)) return set_error(GIP_EINVAL, "gip_mat_set_values: bad argument");
     if (!A->nnz) return GIP_OK;
     gip_ctx *c = A->ctx;
+    ++A->val_version;
     if (A->format == GIP_FMT_ELL || A->format == GIP_FMT_STENCIL) {
         double *tmp = nullptr;
         GIP_HIP_TRY(hipMalloc(&tmp, sizeof(double) * A->nnz));
@@ -736,6 +841,7 @@ int gip_dcsrmv_ab(gip_ctx *c, const gip_mat *A, const double *x, double *y, doub
 int gip_dcsrmv_vals(gip_ctx *c, const gip_mat *A, const double *x, gip_mat *B) {
     if (!c || !A || !B || (A->nrows && !x)) return set_error(GIP_EINVAL, "gip_dcsrmv_vals: bad argument");
     if (A->nrows != B->nnz) return set_error(GIP_EINVAL, "gip_dcsrmv_vals: rows of A must equal nnz of B");
+    ++B->val_version;
     int rc;
     if (B->format == GIP_FMT_ELL || B->format == GIP_FMT_STENCIL)
         rc = spmv_launch(c, A, x, B->d_ell_val, 1.0, 0.0, 0, B->d_csr2ell);
@@ -757,6 +863,7 @@ int gip_dcsrmv_vals(gip_ctx *c, const gip_mat *A, const double *x, gip_mat *B) {
 int gip_dcsrmv_vals_ab(gip_ctx *c, const gip_mat *A, const double *x, gip_mat *B, double alpha, double beta) {
     if (!c || !A || !B || (A->nrows && !x)) return set_error(GIP_EINVAL, "gip_dcsrmv_vals_ab: bad argument");
     if (A->nrows != B->nnz) return set_error(GIP_EINVAL, "gip_dcsrmv_vals_ab: rows of A must equal nnz of B");
+    ++B->val_version;
     int rc;
     if (B->format == GIP_FMT_ELL || B->format == GIP_FMT_STENCIL)
         rc = spmv_launch(c, A, x, B->d_ell_val, alpha, beta, 1, B->d_csr2ell);

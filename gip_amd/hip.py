@@ -20,6 +20,8 @@ MAT_FP32 = 1 << 8
 MAT_ELL_SLICE32 = 1 << 9
 MAT_ELL_SLICE128 = 1 << 10
 ELL_SLICE_FLAGS = {32: MAT_ELL_SLICE32, 64: 0, 128: MAT_ELL_SLICE128}
+MAT_NO_SYM = 1 << 11
+SYM_STATES = {0: "none", 1: "in use", -1: "not symmetric", 2: "not made"}
 PCG_FP64, PCG_MIXED = 0, 1
 OK, EINVAL, EHIP, ENOMEM, EBREAKDOWN, ENOCONV = 0, -1, -2, -3, -4, -5
 
@@ -63,6 +65,7 @@ SIGNATURES = {
     "gip_mat_info": ([c_void_p, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32), POINTER(c_int64)], c_int),
     "gip_mat_cols_needed": ([c_void_p, POINTER(c_int32)], c_int),
     "gip_mat_product_bytes": ([c_void_p, c_int32, POINTER(c_int64)], c_int),
+    "gip_mat_sym_state": ([c_void_p, POINTER(c_int32), POINTER(c_int64)], c_int),
     "gip_mat_set_lanes": ([c_void_p, c_int32], c_int),
     "gip_mat_autotune": ([c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_int32, c_void_p,
                           c_void_p, c_int32, POINTER(c_void_p), c_void_p], c_int),
@@ -195,7 +198,8 @@ class DeviceArray:
 class Matrix:
     """A device operator built from host CSR arrays (int32 rowptr/col, float64 val)."""
 
-    def __init__(self, dev: Device, rowptr, col, val, nrows=None, ncols=0, fmt=FMT_AUTO, fp32=False, slice=64):
+    def __init__(self, dev: Device, rowptr, col, val, nrows=None, ncols=0, fmt=FMT_AUTO, fp32=False, slice=64,
+                 sym=True):
         self.dev = dev
         rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
         col = np.ascontiguousarray(col, dtype=np.int32)
@@ -205,7 +209,7 @@ class Matrix:
         self.ptr = c_void_p()
         if slice not in ELL_SLICE_FLAGS:
             raise ValueError("slice: rows per ELL slice, 32, 64 or 128")
-        flags = fmt | (MAT_FP32 if fp32 else 0) | ELL_SLICE_FLAGS[slice]
+        flags = fmt | (MAT_FP32 if fp32 else 0) | ELL_SLICE_FLAGS[slice] | (0 if sym else MAT_NO_SYM)
         check(lib().gip_mat_create(dev.ctx, self.nrows, int(ncols), self.nnz, rowptr.ctypes.data, col.ctypes.data,
                                    val.ctypes.data, flags, byref(self.ptr)))
         dev._adopt(self)
@@ -247,6 +251,14 @@ class Matrix:
         b = c_int64()
         check(lib().gip_mat_product_bytes(self.ptr, 1 if fp32 else 0, byref(b)))
         return b.value
+
+    def sym_state(self):
+        """(state, bytes) of the symmetric form (gip_mat_sym_state): state 0
+        none, 2 not made yet, 1 in use, -1 values not symmetric; bytes one
+        FP64 product moves with it."""
+        st, b = c_int32(), c_int64()
+        check(lib().gip_mat_sym_state(self.ptr, byref(st), byref(b)))
+        return st.value, b.value
 
     def values(self):
         out = np.empty(self.nnz, np.float64)

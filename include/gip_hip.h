@@ -95,6 +95,17 @@ enum { GIP_MAT_FP32 = 1 << 8 }; /* also keep an FP32 copy of the values (mixed p
  * irregular rows (two slices per wavefront), 128 amortises the slice
  * offsets; the autotuner tries all three. */
 enum { GIP_MAT_ELL_SLICE32 = 1 << 9, GIP_MAT_ELL_SLICE128 = 1 << 10 };
+/* A stencil-layout operator whose values are bitwise symmetric
+ * (a_{r,r+d} == a_{r+d,r}) is multiplied inside the FP64 PCG from the
+ * values of its slots of offsets d >= 0 only, each slot of offset -d
+ * reading row r - d's slot of offset +d: the same values in the same
+ * order, so the same bits, with about 4/7 of the value traffic for a
+ * 7-point stencil (the FP32 product of the mixed solve keeps all slots).
+ * The symmetric arrays are made and checked against every slot on the
+ * device at the first large solve (more than 2^19 rows) after the values
+ * change (gip_mat_set_values, gip_dcsrmv_vals*); an operator that fails
+ * the check keeps the full product.  GIP_MAT_NO_SYM turns this off. */
+enum { GIP_MAT_NO_SYM = 1 << 11 };
 int gip_mat_create(gip_ctx *ctx, int32_t nrows, int32_t ncols, int32_t nnz, const int32_t *rowptr, const int32_t *col,
                    const double *val, int32_t flags, gip_mat **A);
 int gip_mat_destroy(gip_mat *A);
@@ -109,6 +120,13 @@ int gip_mat_info(const gip_mat *A, int32_t *nrows, int32_t *nnz, int32_t *format
  * rows), so it moves fewer bytes than the CSR count of the same operator.
  * No counterpart in the reference: the unit of a layout's roofline. */
 int gip_mat_product_bytes(const gip_mat *A, int32_t fp32, int64_t *bytes);
+/* State of the symmetric form (GIP_MAT_NO_SYM above): 0 none (not a
+ * stencil-layout operator, turned off, or offsets it cannot pair), 2 not
+ * yet made for the current values, 1 in use, -1 the values are not
+ * symmetric.  bytes (optional): what one FP64 product moves with it (the
+ * values of the slots of offsets >= 0, the slot metadata, the tails, the
+ * input and output once; 0 for state 0).  No counterpart in the reference. */
+int gip_mat_sym_state(const gip_mat *A, int32_t *state, int64_t *bytes);
 /* Largest column index + 1: the length an input vector of a product must have. */
 int gip_mat_cols_needed(const gip_mat *A, int32_t *ncols);
 /* CSR operators: lanes per row (1, 2, 4, 8 = 64, 32, 16, 8 rows per

@@ -3,7 +3,11 @@
 // layout, forms y = A x from the layout exactly as the device kernels do
 // (slots in order with clamped columns, then the tail) and from the CSR
 // arrays in stored order, and prints one JSON line with the number of rows
-// whose results differ bitwise and the layout statistics.
+// whose results differ bitwise and the layout statistics.  It then forms
+// the symmetric form of the slot values (plan_symmetric) as the device
+// does: the stored arrays (one per |offset|), the check that every slot of
+// negative offset equals its partner row's slot (the device check), and
+// the products from the arrays, counted against the full-layout products.
 //   layout_check <nrows> <nnz> <rowptr.i32> <col.i32> <val.f64> <x.f64> <ncols_x> [max_tail_frac]
 #include <cstdio>
 #include <cstdlib>
@@ -96,9 +100,60 @@ int main(int argc, char **argv) {
         if ((tl.kind == 0) != (h.any == 0) || tl.tsoff != h.tsoff || tl.m0 != h.m[0]) ++bad_pos;
         if (empty != empty_csr || (!empty && std::memcmp(&got, &want, sizeof(double)) != 0)) ++bad_rows;
     }
+    // the symmetric form
+    gip::SymPlan Y;
+    const bool sym_planned = gip::plan_symmetric(P, Y);
+    bool sym_ok = false;
+    long sym_bad_rows = 0;
+    if (sym_planned) {
+        const int64_t rows_pad = (int64_t)P.ns * 64;
+        std::vector<double> U((size_t)(rows_pad * Y.nu), 0.0);
+        auto code = [&](int64_t s, int k) { return (int)Y.code[(size_t)(s * 8 + k)]; };
+        for (int64_t r = 0; r < rows_pad; ++r) {
+            const int64_t s = r / 64, lane = r % 64;
+            for (int j = 0; j < Y.nu; ++j) {
+                double v = 0.0;
+                for (int k = 0; k < P.w; ++k)
+                    if (code(s, k) == j) v = st[(size_t)((s * P.w + k) * 64 + lane)];
+                U[(size_t)(j * rows_pad + r)] = v;
+            }
+        }
+        // the value a slot takes in the symmetric product
+        auto symval = [&](int32_t r, int k) {
+            const int64_t s = r / 64, lane = r % 64;
+            const int cd = code(s, k);
+            if (cd < 0) return 0.0;
+            if (cd < gip::kSymLower) return U[(size_t)(cd * rows_pad + r)];
+            const int j = cd - gip::kSymLower;
+            const int64_t pr = (int64_t)r - Y.d[j];
+            return pr < 0 ? 0.0 : U[(size_t)(j * rows_pad + pr)];
+        };
+        sym_ok = true;
+        for (int32_t r = 0; r < nrows && sym_ok; ++r)
+            for (int k = 0; k < P.w; ++k) {
+                const double a = st[(size_t)(((int64_t)(r / 64) * P.w + k) * 64 + r % 64)], b = symval(r, k);
+                if (std::memcmp(&a, &b, sizeof(double)) != 0) {
+                    sym_ok = false;
+                    break;
+                }
+            }
+        // slot sums from the symmetric values against the full slot sums
+        for (int32_t r = 0; r < nrows; ++r) {
+            const int32_t s = r / 64, lane = r % 64;
+            double full = 0.0, sym = 0.0;
+            for (int32_t k = 0; k < P.w; ++k) {
+                int32_t c = r + P.off[(size_t)s * P.w + k];
+                c = c < 0 ? 0 : (c > cmax ? cmax : c);
+                full = full + x[(size_t)c] * st[(size_t)(((int64_t)s * P.w + k) * 64 + lane)];
+                sym = sym + x[(size_t)c] * symval(r, k);
+            }
+            if (std::memcmp(&full, &sym, sizeof(double)) != 0) ++sym_bad_rows;
+        }
+    }
     std::printf("{\"planned\": true, \"w\": %d, \"slices\": %d, \"tail\": %zu, \"stored\": %lld, \"bytes\": %lld, "
-                "\"bad_pos\": %ld, \"bad_cols\": %ld, \"bad_rows\": %ld}\n",
+                "\"bad_pos\": %ld, \"bad_cols\": %ld, \"bad_rows\": %ld, \"sym_planned\": %s, \"sym_nu\": %d, "
+                "\"sym_nd\": %d, \"sym_ok\": %s, \"sym_bad_rows\": %ld}\n",
                 P.w, P.ns, P.tcol.size(), (long long)P.stored, (long long)gip::stencil_bytes(P, nrows), bad_pos,
-                bad_cols, bad_rows);
+                bad_cols, bad_rows, sym_planned ? "true" : "false", Y.nu, Y.nd, sym_ok ? "true" : "false", sym_bad_rows);
     return 0;
 }

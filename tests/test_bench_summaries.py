@@ -38,16 +38,20 @@ def test_traffic_is_scaled_by_the_copy_kernel(tmp_path):
         ("void gip::k_iter_spmv<-7>(gip::EllView, gip::PcgDev, double const*)", 3000.0),  # median 2000
         ("gip::k_iter_dir(gip::PcgDev, double*, int)", 10.0),
         ("gip::k_iter_dir_rows(gip::PcgDev, double*, int)", 20.0),
+        ("void gip::k_iter_spmv_sym<-7>(gip::EllView, gip::PcgDev, double const*)", 400.0),
     ])
     write = _pmc_dir(tmp_path, "WRITE_SIZE", [
         ("gip::k_copy(int, double const*, double*)", copy_kib),
         ("void gip::k_iter_spmv<-7>(gip::EllView, gip::PcgDev, double const*)", 500.0),
         ("gip::k_iter_dir(gip::PcgDev, double*, int)", 1.0),
         ("gip::k_iter_dir_rows(gip::PcgDev, double*, int)", 2.0),
+        ("void gip::k_iter_spmv_sym<-7>(gip::EllView, gip::PcgDev, double const*)", 50.0),
     ])
     res, fs, ws = bench.summarize_pmc(fetch, write, bench.PCG_KERNELS)
     assert fs == 2.0 and ws == 1.0
-    assert res["k_iter_spmv"]["bytes"] == 1024 * (2000.0 * 2 + 500.0)
+    # "k_iter_spmv<" is the full-layout product only, not the symmetric one
+    assert res["k_iter_spmv<"]["bytes"] == 1024 * (2000.0 * 2 + 500.0)
+    assert res["k_iter_spmv_sym"]["bytes"] == 1024 * (400.0 * 2 + 50.0)
     # "k_iter_dir(" is the grid-stride sweep only, not k_iter_dir_rows
     assert res["k_iter_dir("]["bytes"] == 1024 * (10.0 * 2 + 1.0)
     assert res["k_iter_dir_rows"]["bytes"] == 1024 * (20.0 * 2 + 2.0)
@@ -64,7 +68,7 @@ def test_traffic_summary_round_trips_into_the_roofline(tmp_path, monkeypatch):
     prof.mkdir()
     (prof / "pcg_kernels_12_traffic.txt").write_text(out)
     monkeypatch.setattr(bench, "ROOT", tmp_path)
-    assert bench.measured_traffic(12, "k_iter_spmv") == int(1024 * (700.0 * 2 + 100.0))
+    assert bench.measured_traffic(12, "k_iter_spmv<") == int(1024 * (700.0 * 2 + 100.0))
     assert bench.measured_traffic(12, "k_m_spmv") is None
     assert bench.measured_traffic(13, "k_iter_spmv") is None
 

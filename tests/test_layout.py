@@ -4,7 +4,10 @@ tests/layout_check.cpp plans the layout of a CSR operator and forms y = A x
 from it exactly as the device kernels do (slots in order with clamped
 columns, then the row's tail); the products must equal the CSR products in
 stored order bitwise, every stored entry must land in exactly one value
-slot, and every tail column must lie inside the operator.
+slot, and every tail column must lie inside the operator.  The symmetric
+form of the slot values (plan_symmetric, used by the PCG's product) must
+pass its check on the generated Laplacians, fail it when one value is
+perturbed, and give the full-layout slot sums bitwise whenever it passes.
 """
 import json
 import shutil
@@ -130,3 +133,38 @@ def test_irregular_operator_is_not_planned(checker):
     cl = np.sort(rng.integers(0, n, size=(n, per)), axis=1).ravel()
     r = _run(checker, rp, cl, rng.standard_normal(n * per))
     assert not r["planned"]
+
+
+@pytest.mark.parametrize("n,beta", [(24, 0.0), (24, 1.5), (46, 0.0)])
+def test_symmetric_form_of_the_laplacian(checker, n, beta):
+    c = cavity.generate(n, beta=beta, gamma=1e-2)
+    L = c.op("L")
+    r = _run(checker, L.rowptr, L.col, L.val)
+    assert r["planned"] and _clean(r), r
+    # paired offsets 0, 1, n, n^2, and arrays for the boundary slices' ghost slots
+    assert r["sym_planned"] and r["sym_nd"] == 4 and 4 <= r["sym_nu"] <= 8, r
+    assert r["sym_ok"] and r["sym_bad_rows"] == 0, r
+
+
+def test_symmetric_form_refuses_a_perturbed_value(checker):
+    c = cavity.generate(24, gamma=1e-2)
+    L = c.op("L")
+    val = L.val.copy()
+    # one off-diagonal entry of an interior row, changed in its last bit
+    r0 = 24 * 24 * 12 + 24 * 12 + 12
+    js = [j for j in range(L.rowptr[r0], L.rowptr[r0 + 1]) if L.col[j] == r0 + 24]
+    assert js
+    val[js[0]] = np.nextafter(val[js[0]], 0.0)
+    r = _run(checker, L.rowptr, L.col, val)
+    assert r["planned"] and _clean(r) and r["sym_planned"], r
+    assert not r["sym_ok"], r
+
+
+def test_symmetric_form_passes_only_where_it_is_exact(checker):
+    """Whatever the operator, a passed check means bitwise equal slot sums."""
+    c = cavity.generate(16, beta=1.5, gamma=1e-2)
+    for name in cavity.OPS:
+        A = c.op(name)
+        r = _run(checker, A.rowptr, A.col, A.val)
+        if r["planned"] and r["sym_planned"] and r["sym_ok"]:
+            assert r["sym_bad_rows"] == 0, (name, r)
