@@ -90,8 +90,11 @@ struct gip_mat {
     int32_t *d_sym_flag;
     uint64_t sym_version;  // val_version + 1 the symmetric values were made from (0: never)
     int32_t sym_ok;        // 1: in use, -1: the values are not symmetric
-    // a_ii per row (the stored values of a DIAG operator), for Jacobi
+    // a_ii per row (the stored values of a DIAG operator), for Jacobi; for
+    // the other layouts refreshed from d_diag_src (each row's last diagonal
+    // entry's position in the value storage, -1 for none) when values change
     double *d_diag;
+    int32_t *d_diag_src;
     int32_t diag_missing;
     // lazily built transpose: structure once, values refreshed per use
     gip_mat *transpose;

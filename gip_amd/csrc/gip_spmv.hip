@@ -220,6 +220,17 @@ __global__ __launch_bounds__(kBlock) void k_to_f32(int64_t n, const double *__re
 
 static inline int nblocks(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
+// The Jacobi diagonal from the stored values after they change: the value
+// at each row's diagonal position (its last diagonal entry).
+__global__ __launch_bounds__(kBlock) void k_diag_refresh(int n, const int32_t *__restrict__ src,
+                                                         const double *__restrict__ store, double *__restrict__ diag) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        const int32_t j = src[i];
+        diag[i] = j >= 0 ? store[j] : 0.0;
+    }
+}
+
 // The symmetric form's arrays from the slot values: array j (values
 // [j rows_pad, (j+1) rows_pad)) holds at row r the value of the row's slot
 // of code j, 0 when its slice has none.  One thread per row, padding rows
@@ -278,7 +289,7 @@ static void free_mat_arrays(gip_mat *A) {
     void *ptrs[] = {A->d_rowptr,   A->d_col,     A->d_val,      A->d_val32,    A->d_slice_off, A->d_ell_col,
                     A->d_ell_val,  A->d_ell_val32, A->d_csr2ell, A->d_diag,     A->d_tsrc,      A->d_tdst,
                     A->d_st_off,   A->d_st_hdr,   A->d_st_tail,  A->d_st_tcol,  A->d_st_code,   A->d_sym_val,
-                    A->d_sym_flag};
+                    A->d_sym_flag, A->d_diag_src};
     for (void *p : ptrs)
         if (p) hipFree(p);
 }
@@ -430,6 +441,16 @@ int mat_sym_prepare(gip_ctx *c, gip_mat *A, bool *ok) {
     return GIP_OK;
 }
 
+// After a change of an operator's stored values: the Jacobi diagonal
+// follows them (a DIAG operator stores it directly).
+static int diag_refresh(gip_ctx *c, gip_mat *A) {
+    if (A->format == GIP_FMT_DIAG || !A->d_diag_src || A->nrows <= 0) return GIP_OK;
+    const double *store = (A->format == GIP_FMT_ELL || A->format == GIP_FMT_STENCIL) ? A->d_ell_val : A->d_val;
+    k_diag_refresh<<<nblocks(A->nrows), kBlock, 0, c->stream>>>(A->nrows, A->d_diag_src, store, A->d_diag);
+    GIP_LAUNCH_CHECK("diagonal refresh");
+    return GIP_OK;
+}
+
 }  // namespace gip
 
 using namespace gip;
@@ -463,6 +484,7 @@ int gip_mat_create(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, const 
 
     // diagonal value per row (last diagonal entry wins, GIP_MatrixMulticore.cpp:191-204)
     std::vector<double> diag((size_t)nrows, 0.0);
+    std::vector<int32_t> diag_src((size_t)nrows, -1);  // CSR index of the row's last diagonal entry
     A->diag_missing = 0;
     bool is_diag = (nnz == nrows);
     for (int32_t i = 0; i < nrows; ++i) {
@@ -470,6 +492,7 @@ int gip_mat_create(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, const 
         for (int32_t j = rowptr[i]; j < rowptr[i + 1]; ++j)
             if (col[j] == i) {
                 diag[(size_t)i] = val[j];
+                diag_src[(size_t)i] = j;
                 found = true;
             }
         if (!found) A->diag_missing = 1;
@@ -524,6 +547,18 @@ int gip_mat_create(gip_ctx *c, int32_t nrows, int32_t ncols, int32_t nnz, const 
     // CSR structure always kept (transposes, value maps)
     rc = upload(c, &A->d_rowptr, rowptr, (size_t)nrows + 1);
     if (!rc) rc = upload(c, &A->d_diag, diag.data(), diag.size());
+    // the diagonal's positions in the value storage, to follow value changes
+    if (!rc && fmt != GIP_FMT_DIAG) {
+        for (int32_t i = 0; i < nrows; ++i) {
+            int32_t &j = diag_src[(size_t)i];
+            if (j < 0) continue;
+            if (fmt == GIP_FMT_ELL)
+                j = (int32_t)(soff[(size_t)(i / H)] + (int64_t)H * (j - rowptr[i]) + i % H);
+            else if (fmt == GIP_FMT_STENCIL)
+                j = plan.pos[(size_t)j];
+        }
+        rc = upload(c, &A->d_diag_src, diag_src.data(), diag_src.size());
+    }
     if (!rc && fmt == GIP_FMT_ELL) {
         A->nslices = ns;
         A->ell_stored = stored;
@@ -816,7 +851,7 @@ int gip_mat_set_values(gip_mat *A, const double *host) {
         if (A->d_val32) k_to_f32<<<nblocks(A->nnz), kBlock, 0, c->stream>>>(A->nnz, A->d_val, A->d_val32);
     }
     GIP_LAUNCH_CHECK("gip_mat_set_values");
-    return GIP_OK;
+    return diag_refresh(c, A);
 }
 
 int gip_mat_diag_inv(const gip_mat *A, double *dinv) {
@@ -853,7 +888,7 @@ int gip_dcsrmv_vals(gip_ctx *c, const gip_mat *A, const double *x, gip_mat *B) {
     if (B->d_ell_val32) k_to_f32<<<nblocks(B->ell_stored), kBlock, 0, c->stream>>>(B->ell_stored, B->d_ell_val, B->d_ell_val32);
     if (B->d_val32) k_to_f32<<<nblocks(B->nnz), kBlock, 0, c->stream>>>(B->nnz, B->d_val, B->d_val32);
     GIP_LAUNCH_CHECK("gip_dcsrmv_vals");
-    return GIP_OK;
+    return diag_refresh(c, B);
 }
 
 // B's values (CSR order) <- alpha A x + beta B's values, rows of A = nnz of
@@ -875,7 +910,7 @@ int gip_dcsrmv_vals_ab(gip_ctx *c, const gip_mat *A, const double *x, gip_mat *B
     if (B->d_ell_val32) k_to_f32<<<nblocks(B->ell_stored), kBlock, 0, c->stream>>>(B->ell_stored, B->d_ell_val, B->d_ell_val32);
     if (B->d_val32) k_to_f32<<<nblocks(B->nnz), kBlock, 0, c->stream>>>(B->nnz, B->d_val, B->d_val32);
     GIP_LAUNCH_CHECK("gip_dcsrmv_vals_ab");
-    return GIP_OK;
+    return diag_refresh(c, B);
 }
 
 int gip_scsrmv(gip_ctx *c, const gip_mat *A, const float *x, float *y) {

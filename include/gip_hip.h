@@ -151,8 +151,10 @@ int gip_mat_autotune(gip_ctx *ctx, int32_t nrows, int32_t ncols, int32_t nnz, co
 int gip_mat_get_values(const gip_mat *A, double *host_vals);
 int gip_mat_set_values(gip_mat *A, const double *host_vals);
 /* 1/a_ii per row into a device vector of nrows values, as
- * GIP_MatrixMulticore::setUpDiagonal (GIP_MatrixMulticore.cpp:177-210);
- * GIP_EINVAL if a row has no diagonal entry. */
+ * GIP_MatrixMulticore::setUpDiagonal (GIP_MatrixMulticore.cpp:177-210),
+ * from the current values (after gip_mat_set_values or gip_dcsrmv_vals*
+ * too; a solver takes them at gip_pcg_create, as the reference's solver
+ * at its setUp); GIP_EINVAL if a row has no diagonal entry. */
 int gip_mat_diag_inv(const gip_mat *A, double *dinv);
 
 /* ---- SpMV ----------------------------------------------------------------

@@ -281,6 +281,42 @@ def test_spmv_into_values_alpha_beta(dev, cav46, fmt):
         C.spmv_vals_ab(dev.to_device(mf), C, 1.0, 0.0)  # rows(C) != nnz(C)
 
 
+def _inv_diag(A, vals):
+    """1/a_ii from CSR values, the row's last diagonal entry."""
+    d = np.zeros(A.nrows)
+    for i in range(A.nrows):
+        for j in range(A.rowptr[i], A.rowptr[i + 1]):
+            if A.col[j] == i:
+                d[i] = vals[j]
+    with np.errstate(divide="ignore"):  # a zero diagonal gives inf on both sides
+        return 1.0 / d
+
+
+@pytest.mark.parametrize("fmt", [None, "ELL", "CSR"])
+def test_jacobi_diagonal_follows_value_changes(dev, cav46, fmt):
+    """gip_mat_diag_inv after gip_mat_set_values and after a product into
+    the values (gip_dcsrmv_vals) uses the new diagonal, in every layout."""
+    kw = {} if fmt is None else {"fmt": getattr(hip, "FMT_" + fmt)}
+    L = cav46.op("L")
+    A = hip.Matrix.from_csr(dev, L, **kw)
+    v2 = L.val * (1.0 + np.random.default_rng(5).random(L.val.size))
+    A.set_values(v2)
+    out = dev.array(L.nrows)
+    A.diag_inv(out)
+    assert np.array_equal(out.download(), _inv_diag(L, v2))
+    Ec, D = cav46.op("Ec"), cav46.op("D")
+    mf = np.random.default_rng(6).standard_normal(cav46.n_faces)
+    E = hip.Matrix.from_csr(dev, Ec)
+    C = hip.Matrix.from_csr(dev, D, **kw)
+    E.spmv_vals(dev.to_device(mf), C)
+    want = D.val.copy()
+    pyoracle.spmv(pyoracle.Csr.of(Ec), mf, want)
+    if all(any(D.col[j] == i for j in range(D.rowptr[i], D.rowptr[i + 1])) for i in range(D.nrows)):
+        out = dev.array(D.nrows)
+        C.diag_inv(out)
+        assert np.array_equal(out.download(), _inv_diag(D, want))
+
+
 def test_transposed_spmv(dev, cav46):
     for name in ("Mx", "SO", "Ec"):
         A = cav46.op(name)
