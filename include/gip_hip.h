@@ -105,8 +105,9 @@ enum { GIP_MAT_ELL_SLICE32 = 1 << 9, GIP_MAT_ELL_SLICE128 = 1 << 10 };
  * device at the first large solve (more than 2^19 rows) after the values
  * change (gip_mat_set_values, gip_dcsrmv_vals*); an operator that fails
  * the check keeps the full product.  That first solve allocates the arrays
- * (about 4/7 of the value array) and must not overlap another solve on the
- * same operator.  GIP_MAT_NO_SYM turns this off. */
+ * (one value per row for each stored offset: 6 for the cavity Laplacian's
+ * 7 slots) and must not overlap another solve on the same operator.
+ * GIP_MAT_NO_SYM turns this off. */
 enum { GIP_MAT_NO_SYM = 1 << 11 };
 int gip_mat_create(gip_ctx *ctx, int32_t nrows, int32_t ncols, int32_t nnz, const int32_t *rowptr, const int32_t *col,
                    const double *val, int32_t flags, gip_mat **A);
