@@ -44,7 +44,10 @@ static_assert(sizeof(StencilHdr) == 80, "slice header layout");
 
 // Compact per-slice tail record (16 bytes, read for every slice): kind 0 no
 // tail and no empty row; kind 1 only first tail entries (lane mask m0, no
-// empty row); kind 2 anything else (the full header decides).
+// empty row); kind 3 exactly one tail entry in the slice (no empty row):
+// its lane in bits 8..13 of `kind`, its position in tsoff and its column in
+// m0, so a wavefront has all three from the record; kind 2 anything else
+// (the full header decides).  The kind is `kind & 0xff`.
 struct StencilTail {
     int32_t tsoff;
     int32_t kind;
@@ -164,6 +167,10 @@ inline bool plan_stencil(int32_t nrows, int32_t nnz, const int32_t *rowptr, cons
         t.tsoff = h.tsoff;
         t.m0 = h.m[0];
         t.kind = !h.any ? 0 : (h.empty == 0 && h.m[1] == 0) ? 1 : 2;
+        if (t.kind == 1 && __builtin_popcountll(h.m[0]) == 1) {
+            t.kind = 3 | (__builtin_ctzll(h.m[0]) << 8);
+            t.m0 = (uint64_t)(uint32_t)P.tcol[(size_t)h.tsoff];
+        }
     }
     P.stored = P.tail_base + (int64_t)P.tcol.size();
     if (P.stored > INT32_MAX) return false;
@@ -251,7 +258,7 @@ inline int32_t stencil_tail_index(const StencilHdr &h, int lane, int rank) {
 // slices).
 inline int64_t stencil_bytes(const StencilPlan &P, int32_t) {
     int64_t general = 0;
-    for (const StencilTail &t : P.tail) general += t.kind == 2;
+    for (const StencilTail &t : P.tail) general += (t.kind & 0xff) == 2;
     return 8 * P.stored + 4 * (int64_t)P.tcol.size() + (int64_t)P.ns * (4 * P.w + (int64_t)sizeof(StencilTail)) +
            general * (int64_t)sizeof(StencilHdr);
 }

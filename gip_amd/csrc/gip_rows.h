@@ -96,6 +96,18 @@ __device__ inline bool stencil_tail_hdr(const EllView &A, const V *__restrict__ 
     return true;
 }
 
+// A slice's only tail entry (record kind 3): its lane, column and position
+// are wavefront-uniform, so its value and operand are loads every lane of
+// the wavefront shares, issued without waiting for a per-lane index.
+template <class T, class V, class X>
+__device__ inline void stencil_tail_one(const EllView &A, const V *__restrict__ val, const X &x, const StencilTail &tl,
+                                        int row, T &sum) {
+    const int lane_t = (tl.kind >> 8) & 63;
+    const V tv = val[A.tail_base + tl.tsoff];
+    const T xt = x.at((int)(uint32_t)tl.m0);
+    if ((row & 63) == lane_t) sum = sum + xt * tv;
+}
+
 // Stencil-layout row sum: the slots in order (column row + offset, clamped
 // into the vector for slots the row lacks, whose value is 0), then the
 // row's tail in stored order.  The wavefront's 64 rows are one slice, so
@@ -126,13 +138,14 @@ __device__ inline bool stencil_row(const EllView &A, const V *__restrict__ val, 
     }
     // compact 16-byte record per slice; the 80-byte header only for the
     // rare slices with empty rows or rows of several tail entries
-    const StencilTail *tl = A.st_tail + s;
-    const int kind = tl->kind;
-    if (kind == 1) {
+    const StencilTail tl = A.st_tail[s];
+    const int kind = tl.kind & 0xff;
+    if (kind == 3) {
+        stencil_tail_one(A, val, x, tl, row, sum);
+    } else if (kind == 1) {
         const uint64_t bit = 1ull << (row & 63);
-        const uint64_t m0 = tl->m0;
-        if (m0 & bit) {
-            const int e = tl->tsoff + __popcll(m0 & (bit - 1ull));
+        if (tl.m0 & bit) {
+            const int e = tl.tsoff + __popcll(tl.m0 & (bit - 1ull));
             sum = sum + x.at(A.st_tcol[e]) * val[A.tail_base + e];
         }
     } else if (kind == 2) {
@@ -171,13 +184,14 @@ __device__ inline bool stencil_row_sym(const EllView &A, const V *__restrict__ v
     }
 #pragma unroll
     for (int k = 0; k < SW; ++k) sum = sum + x.at(c[k]) * v[k];
-    const StencilTail *tl = A.st_tail + s;
-    const int kind = tl->kind;
-    if (kind == 1) {
+    const StencilTail tl = A.st_tail[s];
+    const int kind = tl.kind & 0xff;
+    if (kind == 3) {
+        stencil_tail_one(A, val, x, tl, row, sum);
+    } else if (kind == 1) {
         const uint64_t bit = 1ull << lane;
-        const uint64_t m0 = tl->m0;
-        if (m0 & bit) {
-            const int e = tl->tsoff + __popcll(m0 & (bit - 1ull));
+        if (tl.m0 & bit) {
+            const int e = tl.tsoff + __popcll(tl.m0 & (bit - 1ull));
             sum = sum + x.at(A.st_tcol[e]) * val[A.tail_base + e];
         }
     } else if (kind == 2) {
@@ -195,14 +209,14 @@ __device__ inline bool row_dot_sym(const EllView &A, const V *__restrict__ val, 
 }
 
 // The operator's part of a stencil row (slot columns and values, and for a
-// slice of kind 0 or 1 its one possible tail entry), loaded apart from the
+// slice of kind 0, 1 or 3 its one possible tail entry), loaded apart from the
 // operand so that a kernel can request it before its prologue; the operand
 // reads follow in stencil_row_fin, in the order stencil_row uses.
 template <int SW, class V>
 struct StencilPre {
     int c[SW];
     V v[SW];
-    int kind, slice, tcol;  // tcol < 0: no tail entry (kinds 0 and 1)
+    int kind, slice, tcol;  // tcol < 0: no tail entry (kinds 0, 1 and 3)
     V tval;
 };
 
@@ -219,10 +233,15 @@ __device__ inline void stencil_row_pre(const EllView &A, const V *__restrict__ v
     }
     const StencilTail tl = A.st_tail[s];  // the 16-byte record in one load
     q.slice = s;
-    q.kind = tl.kind;
+    q.kind = tl.kind & 0xff;
     q.tcol = -1;
     q.tval = V(0);
-    if (q.kind == 1) {
+    if (q.kind == 3) {
+        if ((row & 63) == ((tl.kind >> 8) & 63)) {
+            q.tcol = (int)(uint32_t)tl.m0;
+            q.tval = val[A.tail_base + tl.tsoff];
+        }
+    } else if (q.kind == 1) {
         const uint64_t bit = 1ull << (row & 63);
         if (tl.m0 & bit) {
             const int e = tl.tsoff + __popcll(tl.m0 & (bit - 1ull));
@@ -238,7 +257,7 @@ __device__ inline bool stencil_row_fin(const EllView &A, const V *__restrict__ v
     sum = T(0);
 #pragma unroll
     for (int k = 0; k < SW; ++k) sum = sum + x.at(q.c[k]) * q.v[k];
-    if (q.kind == 1) {
+    if (q.kind == 1 || q.kind == 3) {
         if (q.tcol >= 0) sum = sum + x.at(q.tcol) * q.tval;
     } else if (q.kind == 2) {
         return stencil_tail_hdr(A, val, x, q.slice, row, sum);

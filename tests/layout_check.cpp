@@ -60,7 +60,8 @@ int main(int argc, char **argv) {
     }
     int32_t cmax = 0;
     for (int32_t c : col) cmax = std::max(cmax, c);
-    long bad_rows = 0, bad_cols = 0;
+    long bad_rows = 0, bad_cols = 0, one_entry = 0;
+    for (const gip::StencilTail &t : P.tail) one_entry += (t.kind & 0xff) == 3;
     for (int32_t r = 0; r < nrows; ++r) {
         double want = 0.0;
         for (int32_t j = rp[(size_t)r]; j < rp[(size_t)r + 1]; ++j) want = want + x[(size_t)col[(size_t)j]] * val[(size_t)j];
@@ -75,7 +76,20 @@ int main(int argc, char **argv) {
             c = c < 0 ? 0 : (c > cmax ? cmax : c);
             got = got + x[(size_t)c] * st[(size_t)(((int64_t)s * P.w + k) * 64 + lane)];
         }
-        if (tl.kind == 1) {  // first tail entries only: the compact record decides
+        const int kind = tl.kind & 0xff;
+        if (kind == 3) {  // the slice's only tail entry: lane, position and column in the record
+            const int lt = (tl.kind >> 8) & 63;
+            const int32_t t = tl.tsoff, tc = (int32_t)(uint32_t)tl.m0;
+            if (__builtin_popcountll(h.m[0]) != 1 || lt != __builtin_ctzll(h.m[0]) || t != h.tsoff ||
+                h.m[1] != 0 || h.empty != 0 || t >= (int32_t)P.tcol.size() || tc != P.tcol[(size_t)t])
+                ++bad_pos;
+            else if (lane == lt) {
+                if (tc > cmax)
+                    ++bad_cols;
+                else
+                    got = got + x[(size_t)tc] * st[(size_t)(P.tail_base + t)];
+            }
+        } else if (kind == 1) {  // first tail entries only: the compact record decides
             const uint64_t bit = 1ull << lane;
             if (tl.m0 & bit) {
                 const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -85,7 +99,7 @@ int main(int argc, char **argv) {
                 else
                     got = got + x[(size_t)P.tcol[(size_t)t]] * st[(size_t)(P.tail_base + t)];
             }
-        } else if (tl.kind == 2) {
+        } else if (kind == 2) {
             const uint64_t bit = 1ull << lane;
             empty = (h.empty & bit) != 0;
             for (int j = 0; j < gip::kStencilMaxTail && (h.m[j] & bit); ++j) {
@@ -97,7 +111,7 @@ int main(int argc, char **argv) {
                 got = got + x[(size_t)P.tcol[(size_t)t]] * st[(size_t)(P.tail_base + t)];
             }
         }
-        if ((tl.kind == 0) != (h.any == 0) || tl.tsoff != h.tsoff || tl.m0 != h.m[0]) ++bad_pos;
+        if ((kind == 0) != (h.any == 0) || tl.tsoff != h.tsoff || (kind != 3 && tl.m0 != h.m[0])) ++bad_pos;
         if (empty != empty_csr || (!empty && std::memcmp(&got, &want, sizeof(double)) != 0)) ++bad_rows;
     }
     // the symmetric form
@@ -152,8 +166,9 @@ int main(int argc, char **argv) {
     }
     std::printf("{\"planned\": true, \"w\": %d, \"slices\": %d, \"tail\": %zu, \"stored\": %lld, \"bytes\": %lld, "
                 "\"bad_pos\": %ld, \"bad_cols\": %ld, \"bad_rows\": %ld, \"sym_planned\": %s, \"sym_nu\": %d, "
-                "\"sym_nd\": %d, \"sym_ok\": %s, \"sym_bad_rows\": %ld}\n",
+                "\"sym_nd\": %d, \"sym_ok\": %s, \"sym_bad_rows\": %ld, \"one_entry_slices\": %ld}\n",
                 P.w, P.ns, P.tcol.size(), (long long)P.stored, (long long)gip::stencil_bytes(P, nrows), bad_pos,
-                bad_cols, bad_rows, sym_planned ? "true" : "false", Y.nu, Y.nd, sym_ok ? "true" : "false", sym_bad_rows);
+                bad_cols, bad_rows, sym_planned ? "true" : "false", Y.nu, Y.nd, sym_ok ? "true" : "false", sym_bad_rows,
+                one_entry);
     return 0;
 }

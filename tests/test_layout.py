@@ -168,3 +168,17 @@ def test_symmetric_form_passes_only_where_it_is_exact(checker):
         r = _run(checker, A.rowptr, A.col, A.val)
         if r["planned"] and r["sym_planned"] and r["sym_ok"]:
             assert r["sym_bad_rows"] == 0, (name, r)
+
+
+def test_one_entry_tail_records_at_128(checker):
+    """From 128 cells per line on, a 64-row slice holds at most one
+    x-boundary row: its one ghost entry is the slice's whole tail, and the
+    compact record carries its lane, position and column (kind 3); the sums
+    stay bitwise."""
+    c = cavity.generate(128, gamma=1e-3, ops=cavity.OP_BITS["L"])
+    L = c.op("L")
+    r = _run(checker, L.rowptr, L.col, L.val)
+    assert r["planned"] and _clean(r), r
+    # the first and last slice of every line off the y and z boundaries
+    # (those lines' slices also hold their y/z ghosts' entries)
+    assert r["one_entry_slices"] == 2 * 126 * 126, r
