@@ -1348,6 +1348,8 @@ extern "C" {
 
 int gip_pcg_create(gip_ctx *c, const gip_mat *A, int32_t n_all, gip_pcg **out) {
     if (!c || !A || !out || n_all < A->nrows) return set_error(GIP_EINVAL, "gip_pcg_create: bad argument");
+    // one stream orders the operator's value updates and the solves
+    if (A->ctx != c) return set_error(GIP_EINVAL, "gip_pcg_create: operator belongs to another context");
     if (n_all < A->cols_needed) return set_error(GIP_EINVAL, "gip_pcg_create: vectors shorter than the operator's columns");
     if (A->format != GIP_FMT_ELL && A->format != GIP_FMT_CSR && A->format != GIP_FMT_STENCIL)
         return set_error(GIP_EINVAL, "gip_pcg_create: operator must be ELL, stencil or CSR");

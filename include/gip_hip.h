@@ -264,6 +264,8 @@ typedef struct gip_pcg_stats {
                             direction fused in on operators of <= 2^19 rows)     */
     int32_t product_samples; /* launches sampled                                  */
 } gip_pcg_stats;
+/* The operator must belong to ctx (GIP_EINVAL otherwise): its value updates
+ * and the solves are ordered on the one stream. */
 int gip_pcg_create(gip_ctx *ctx, const gip_mat *A, int32_t n_all, gip_pcg **solver);
 /* Solves rows [0,nrows(A)) of A x = b; x0 is copied into x first; vectors are
  * n_all long (entries past nrows(A) are carried over from x0). */

@@ -539,6 +539,10 @@ def test_pcg_argument_errors_and_statuses(dev, cav46):
     assert b"FP32" in lib.gip_last_error()
     out = c_void_p()
     assert lib.gip_pcg_create(dev.ctx, A64.ptr, n - 1, byref(out)) == hip.EINVAL  # vectors shorter than the rows
+    other = hip.Device(0)  # a second context (stream) on the same device
+    assert lib.gip_pcg_create(other.ctx, A64.ptr, nall, byref(out)) == hip.EINVAL
+    assert b"another context" in lib.gip_last_error()
+    other.close()
     # a tolerance not reached within maxit: status ENOCONV, the call itself succeeds
     r = solver.solve(db, dx, z, maxit=5, tol=1e-12)
     assert r.status == hip.ENOCONV and r.iters == 5
